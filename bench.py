@@ -1,0 +1,235 @@
+"""Benchmark of the MI355X state-vector engine (one JSON line on stdout).
+
+Step: one execute() of the 30-qubit QFT-structured circuit (H on every
+target followed by CP(pi/2^k) to all lower qubits, no swaps: 30 H + 435 CP)
+in complex128 through the engine's drop-in ZXH object, with the reference's
+flush rules (a U3 pass per H, one diagonal sweep per CP window).  The state is
+resident in HBM; execute() includes the reset of the 16 GiB state, as the
+reference's execute() does.  value = gates / second of the timed steps.
+
+Reported beside it (same JSON line):
+  roofline      dominant kernel of the step: algorithmic bytes per launch /
+                average launch time (HIP events on the launch stream)
+  diag_batches  one fused sweep per batch of B random P/CP gates, B in
+                {1,4,16,64,256}, uniform-superposition input, c128 and c64
+  u3_all        U3 on all 30 qubits as tile passes
+  c64           the same QFT step with a complex64 state
+  sampling      1e6 shots on the 30-qubit state
+  cpu_baseline  the reference's CPU path (oracle/_ref when built, else the
+                oracle port) on a bounded prefix of the same circuit, timed on
+                this host
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "30q gates/s + HBM TB/s vs peak; 28q adjoint grads/s (c64 vs c128 plain)"
+NQ = 30
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def family_stats(sim, families=("diag", "u3_tile", "u3_pair", "reset", "cp", "perm")):
+    out = {}
+    for f in families:
+        t = sim.timing(f)
+        if t["launches"]:
+            out[f] = t
+    return out
+
+
+def roofline_of(stats):
+    fam = max(stats, key=lambda k: stats[k]["ms"])
+    t = stats[fam]
+    per_launch_bytes = t["bytes"] / t["launches"]
+    avg_ms = t["ms"] / t["launches"]
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
+    return fam, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                 "kernel": fam, "bytes_per_launch": int(per_launch_bytes), "avg_launch_ms": round(avg_ms, 4),
+                 "launches": t["launches"]}
+
+
+def qft_step(Z, W, prec, steps, warmup):
+    gates = W.qft(NQ)
+    sim = Z.ZXH(NQ, False, True, precision=prec)
+    sim.gates(gates)
+    for i in range(warmup):
+        sim.execute()
+    from zxh_adae_amd import kernels as K
+    K.sync()
+    sim.reset_timing()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    for i in range(steps):
+        sim.execute()
+    K.sync()
+    dt = time.perf_counter() - t0
+    sim.set_timing(False)
+    stats = family_stats(sim)
+    return sim, len(gates), dt / steps, stats
+
+
+def diag_batches(Z, W, K, sim_state, prec, repeats=5):
+    ptr, m = sim_state
+    import numpy as np
+    from zxh_adae_amd.kernels import DeviceState, NO_SEL
+    st = DeviceState.__new__(DeviceState)
+    st.lib = Z._core._lib.load()
+    st.nbits = m
+    st.prec = 0 if prec == "c128" else 1
+    st.ptr = __import__("ctypes").c_void_p(ptr)
+    st.dtype = np.complex128 if prec == "c128" else np.complex64
+    st.nbytes = (1 << m) * (16 if prec == "c128" else 8)
+    out = {}
+    for B in (1, 4, 16, 64, 256):
+        gs = W.diag_batch(m, B, 100 + B)
+        s0 = [(1 << g[2]) for g in gs]
+        s1 = [NO_SEL if g[0] == W.P else (1 << g[1]) for g in gs]
+        th = [g[3] for g in gs]
+        st.diag(s0, s1, th)  # warmup
+        K.reset_timing()
+        K.set_timing(True)
+        for _ in range(repeats):
+            st.diag(s0, s1, th)
+        t = K.timing("diag")
+        K.set_timing(False)
+        avg_ms = t["ms"] / t["launches"]
+        gbps = t["bytes"] / t["launches"] / (avg_ms * 1e-3) / 1e9
+        out[str(B)] = {"ms": round(avg_ms, 4), "GB/s": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
+                       "gates/s": round(B / (avg_ms * 1e-3), 1)}
+    st.ptr = None  # owned by the simulator
+    return out
+
+
+def u3_all(Z, W, K, sim_state, prec, repeats=3):
+    import ctypes
+    import numpy as np
+    from zxh_adae_amd.kernels import DeviceState
+    ptr, m = sim_state
+    st = DeviceState.__new__(DeviceState)
+    st.lib = Z._core._lib.load()
+    st.nbits = m
+    st.prec = 0 if prec == "c128" else 1
+    st.ptr = ctypes.c_void_p(ptr)
+    descs = [((1 << g[2]), (1 << g[2]), W.u3_matrix(g[3], g[4], g[5])) for g in W.u3_all(m, 77)]
+    st.u3_batch(descs)
+    K.reset_timing()
+    K.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        st.u3_batch(descs)
+    K.sync()
+    wall = (time.perf_counter() - t0) / repeats
+    t = K.timing("u3_tile")
+    K.set_timing(False)
+    st.ptr = None
+    passes = t["launches"] / repeats
+    kern_ms = t["ms"] / repeats
+    gbps = t["bytes"] / t["launches"] / (t["ms"] / t["launches"] * 1e-3) / 1e9
+    return {"gates": m, "passes": passes, "ms": round(kern_ms, 3), "wall_ms": round(wall * 1e3, 3),
+            "gates/s": round(m / (kern_ms * 1e-3), 1), "GB/s_per_pass": round(gbps, 1),
+            "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
+def cpu_baseline(W, targets=4):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    gates = []
+    for t in range(targets):
+        gates.append((W.H, 0, t, 0.0, 0.0, 0.0))
+        for j in range(t):
+            gates.append((W.CP, j, t, 3.141592653589793 / (2 ** (t - j)), 0.0, 0.0))
+    kind = "reference" if O.ref_available() else "port"
+    cls = O.RefSim if kind == "reference" else O.OracleSim
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sim = cls(NQ, False, True)
+    sim.gates(gates)
+    t0 = time.perf_counter()
+    sim.execute()
+    dt = time.perf_counter() - t0
+    del sim
+    return {"value": round(len(gates) / dt, 3), "unit": "gates/s", "cores": cores, "kind": kind,
+            "sample": f"first {targets} targets of the 30-qubit QFT ({len(gates)} gates, complex128, "
+                      f"one execute() incl. its 16 GiB state reset), {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine is single-GPU by design",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import zxh_adae_amd as Z
+    from zxh_adae_amd import kernels as K
+    from zxh_adae_amd import workloads as W
+
+    Z.init()
+    log(f"QFT-{NQ} c128: warmup {args.warmup}, steps {args.steps}")
+    sim, ngates, t_step, stats = qft_step(Z, W, "c128", args.steps, args.warmup)
+    fam, roof = roofline_of(stats)
+    log(f"step {t_step * 1e3:.2f} ms, {ngates / t_step:.1f} gates/s; dominant {fam}: "
+        f"{roof['achieved']:.0f} GB/s ({roof['frac']:.1%})")
+    result = {
+        "metric": METRIC, "value": round(ngates / t_step, 3), "unit": "gates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"qft{NQ}_c128_execute", "qubits": NQ, "gates": ngates, "state": "complex128",
+                   "method": "plain (eager_expand_all)", "policy": "reference flush rules"},
+        "roofline": roof,
+        "kernels": {k: {"launches": v["launches"], "avg_ms": round(v["ms"] / v["launches"], 4),
+                        "GB/s": round(v["bytes"] / v["launches"] / (v["ms"] / v["launches"] * 1e-3) / 1e9, 1)}
+                    for k, v in stats.items()},
+    }
+
+    if not args.no_extras:
+        # uniform superposition input for the diagonal batches and the U3 pass
+        sim.clear_gates()
+        sim.gates(W.hadamard_all(NQ))
+        sim.execute()
+        log("diag batches c128")
+        result["diag_batches"] = {"c128": diag_batches(Z, W, K, sim.device_state(), "c128")}
+        log("u3 all c128")
+        result["u3_all"] = {"c128": u3_all(Z, W, K, sim.device_state(), "c128")}
+        log("sampling 1e6 shots")
+        sim.set_seed(5)
+        t0 = time.perf_counter()
+        sim.measure(1000000)
+        result["sampling"] = {"shots": 1000000, "qubits": NQ, "seconds": round(time.perf_counter() - t0, 4),
+                              "kernel_ms": round(sim.timing("sample")["ms"], 3)}
+        del sim
+        log(f"QFT-{NQ} c64")
+        sim64, _, t64, stats64 = qft_step(Z, W, "c64", args.steps, 1)
+        result["c64"] = {"gates/s": round(ngates / t64, 3), "ms_per_step": round(t64 * 1e3, 3)}
+        sim64.clear_gates()
+        sim64.gates(W.hadamard_all(NQ))
+        sim64.execute()
+        result["diag_batches"]["c64"] = diag_batches(Z, W, K, sim64.device_state(), "c64")
+        result["u3_all"]["c64"] = u3_all(Z, W, K, sim64.device_state(), "c64")
+        del sim64
+
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        result["cpu_baseline"] = cpu_baseline(W)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

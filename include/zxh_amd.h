@@ -1,0 +1,189 @@
+/*
+ * zxh_amd.h -- C ABI of the MI355X state-vector engine (libzxh_amd.so).
+ *
+ * Two layers, both plain C (no C++ or torch types in any signature):
+ *
+ *  1. A handle-based mirror of ZXH-Sim's Python core module, i.e. the class
+ *     the reference's pybind11 module exposes (reference
+ *     src/zxh/src/pybind/bindings.cpp:79-118, class declared in
+ *     src/zxh/include/zxhsim/zxh.h:11-53).  Each zxh_* call below names the
+ *     method it replaces.  A Python ctypes binding, a cgo or a JNI stub binds
+ *     these directly.
+ *
+ *  2. Kernel-level entry points over caller-owned device buffers, mirroring
+ *     the reference's internal backend API (src/zxh/include/zxhsim/kernels.h:11-46
+ *     and the sampler behind src/zxh/include/zxhsim/measure.h:12-51).
+ *
+ * Every function returns 0 on success or a ZXH_E* code; zxh_last_error()
+ * gives the message of the last failure on the calling thread.  The reference
+ * aborts the process on these conditions instead (runtime_general.cpp:79-89).
+ *
+ * Amplitude layout: interleaved complex, element x at index x (little-endian
+ * qubit order of the physical address), complex128 (2 x f64) or complex64
+ * (2 x f32).
+ */
+#ifndef ZXH_AMD_H
+#define ZXH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define ZXH_OK 0
+#define ZXH_EINVAL 1      /* bad argument (qubit out of range, bad size, ...) */
+#define ZXH_ESTATE 2      /* call out of order (e.g. get_state before execute) */
+#define ZXH_EZEROPROB 3   /* "state has zero total probability" */
+#define ZXH_EUNRESOLVED 4 /* "measure failed to resolve all local samples" */
+#define ZXH_ENOMEM 5      /* device or host allocation failed */
+#define ZXH_EHIP 6        /* HIP runtime error (launch, copy, sync) */
+#define ZXH_ENODEV 7      /* no usable GPU */
+
+/* ---- precision ---- */
+#define ZXH_C128 0 /* complex<double>, the reference's only type (types.h:11-12) */
+#define ZXH_C64 1  /* complex<float> state, f32 arithmetic, f64 reductions */
+
+/* ---- gate codes: the reference's gate_type_t order (defs.h:65-78) ---- */
+#define ZXH_G_BARRIER 1
+#define ZXH_G_Z 2
+#define ZXH_G_RZ 3
+#define ZXH_G_P 4
+#define ZXH_G_CRZ 5
+#define ZXH_G_CP 6
+#define ZXH_G_X 7
+#define ZXH_G_CX 8
+#define ZXH_G_H 9
+#define ZXH_G_U3 10
+#define ZXH_G_RX 11 /* recorded as U3(theta, pi/2, -pi/2) (zxh.cpp:303-310) */
+
+/* ---- batching policy ---- */
+#define ZXH_POLICY_REFERENCE 0 /* flush exactly where ZXH::execute flushes (zxh.cpp:437-548) */
+
+typedef struct zxh_sim zxh_sim;
+
+const char *zxh_last_error(void);
+
+/* runtime: replaces init/finalize/active/rank/nprocs (bindings.cpp:86-90,
+ * runtime_general.cpp:22-89).  Single GPU, rank 0 of 1. */
+int zxh_init(void);
+int zxh_finalize(void);
+int zxh_active(void);
+int zxh_rank(void);
+int zxh_nprocs(void);
+
+/* ZXH(n, disable_x=False, eager_expand_all=False)  (bindings.cpp:92-94);
+ * precision is ZXH_C128 for the reference's behaviour. */
+int zxh_create(size_t n, int disable_x, int eager_expand_all, int precision, zxh_sim **out);
+int zxh_destroy(zxh_sim *s);
+
+int zxh_set_seed(zxh_sim *s, uint64_t seed); /* ZXH::set_seed  (zxh.cpp:216-220) */
+int zxh_clear_seed(zxh_sim *s);              /* ZXH::clear_seed (zxh.cpp:222-226) */
+int zxh_clear_gates(zxh_sim *s);             /* ZXH::clear_gates (zxh.cpp:228-232) */
+
+/* gate recording: ZXH::Barrier/Rz/CRz/CP/P/Z/X/CX/H/U3/Rx (zxh.cpp:234-310).
+ * U3 argument order is (theta, lambda, phi), as in the reference. */
+int zxh_barrier(zxh_sim *s);
+int zxh_rz(zxh_sim *s, size_t q, double theta);
+int zxh_crz(zxh_sim *s, size_t cq, size_t q, double theta);
+int zxh_cp(zxh_sim *s, size_t cq, size_t q, double theta);
+int zxh_p(zxh_sim *s, size_t q, double theta);
+int zxh_z(zxh_sim *s, size_t q);
+int zxh_x(zxh_sim *s, size_t q);
+int zxh_cx(zxh_sim *s, size_t cq, size_t q);
+int zxh_h(zxh_sim *s, size_t q);
+int zxh_u3(zxh_sim *s, size_t q, double theta, double lambda, double phi);
+int zxh_rx(zxh_sim *s, size_t q, double theta);
+/* one call per gate code (ZXH_G_*), for bulk loaders */
+int zxh_gate(zxh_sim *s, int code, size_t cq, size_t q, double theta, double lambda, double phi);
+
+int zxh_execute(zxh_sim *s); /* ZXH::execute (zxh.cpp:312-553); synchronous */
+
+int zxh_measure(zxh_sim *s, size_t shots);                        /* ZXH::measure (zxh.cpp:555-559) */
+int zxh_measured_count(const zxh_sim *s, size_t *out);            /* ZXH::measured_count */
+/* ZXH::get_results: ceil(N/64) little-endian words per shot, bit q = qubit q
+ * (measure_result_pack.h:24-52) */
+int zxh_get_results(const zxh_sim *s, uint64_t *packed, size_t shots);
+/* physical (real) addresses of the last measure, ascending */
+int zxh_get_real_results(const zxh_sim *s, uint64_t *real, size_t shots);
+/* ZXH::get_state (zxh.cpp:579-636): 2^N complex128 values (interleaved
+ * doubles, out_len = 2^(N+1) doubles) in bit-reversed ("cudaq") order,
+ * multiplied by the global phase.  Works for both precisions. */
+int zxh_get_state(const zxh_sim *s, double *out, size_t out_len);
+int zxh_required_M(const zxh_sim *s, size_t *out); /* ZXH::required_M (zxh.cpp:638-641) */
+int zxh_num_qubits(const zxh_sim *s, size_t *out); /* ZXH::num_qubits */
+
+/* ---- extensions over the reference's class (no reference counterpart) ---- */
+int zxh_set_policy(zxh_sim *s, int policy);
+/* sampling segment bits, the ZXHSIM_SEG_MIN_I setting (sv_common.cpp:62-100) */
+int zxh_set_seg_bits(zxh_sim *s, size_t bits);
+/* raw physical state: 2^m amplitudes of the simulator's precision (host copy) */
+int zxh_get_phys_state(const zxh_sim *s, void *out, size_t out_bytes, size_t *m_out);
+/* replace the physical state after execute() (same width m, host data of the
+ * simulator's precision); used to sample a caller-given probability vector */
+int zxh_set_phys_state(zxh_sim *s, const void *in, size_t in_bytes);
+/* device pointer of the physical state and its width m (valid until the next
+ * execute()) */
+int zxh_state_device_ptr(const zxh_sim *s, void **dptr, size_t *m_out);
+/* per-kernel-family timing with HIP events (0 = off) */
+int zxh_set_timing(zxh_sim *s, int on);
+/* family: 0 diag, 1 u3 tile pass, 2 u3 pair (general), 3 perm, 4 cp,
+ * 5 sample, 6 reset.  Fills launches and total milliseconds. */
+int zxh_get_timing(zxh_sim *s, int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
+int zxh_reset_timing(zxh_sim *s);
+
+/* ---- kernel level: caller-owned device buffers (kernels.h:11-46) ---- */
+
+/* u3_batch_desc_t (defs.h:85-98): selector packed as alpha | beta<<63 */
+typedef struct {
+    uint64_t packed_sel;
+    uint64_t delta;
+    double u[8]; /* u00 re,im  u01 re,im  u10 re,im  u11 re,im */
+} zxh_u3_desc;
+
+int zxh_dev_alloc(size_t bytes, void **dptr);
+int zxh_dev_free(void *dptr);
+int zxh_dev_upload(void *dptr, const void *host, size_t bytes);
+int zxh_dev_download(void *host, const void *dptr, size_t bytes);
+int zxh_dev_sync(void);
+
+/* u3_block_batch_kernel (kernels.h:26): applies the descriptors (distinct
+ * target qubits, commuting) to 2^local_bits amplitudes in place. */
+int zxh_k_u3_batch(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc);
+/* diag_pending_push_p / try_push_cp / flush (kernels.h:36-41) as one call:
+ * ngates entries, sel1[i] == ~0 marks a P gate (Z/Rz already lowered to P). */
+int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
+               const double *theta, size_t ngates);
+/* x_block_kernel / cx_block_kernel / cp_kernel (kernels.h:18-19, 47) */
+int zxh_k_x(void *state, int precision, size_t local_bits, uint64_t delta, uint64_t packed_sel);
+int zxh_k_cx(void *state, int precision, size_t local_bits, uint64_t delta, uint64_t packed_sel_cq,
+             uint64_t packed_sel_q);
+int zxh_k_cp(void *state, int precision, size_t local_bits, uint64_t packed_sel_cq, uint64_t packed_sel_q,
+             double theta);
+/* multi-controlled X on physical bits: swaps amplitude pairs differing in
+ * bit target where every bit in ctrl_mask is set (no reference counterpart:
+ * the reference has no MCX; exact index work). */
+int zxh_k_mcx(void *state, int precision, size_t local_bits, uint64_t ctrl_mask, size_t target);
+/* streaming multinomial sampler over 2^m device amplitudes (measure_general.cpp,
+ * measure_streaming_impl.h): sorted thresholds generated from stream_seed as
+ * the reference does, segments of 2^min(m, seg_bits); writes shots physical
+ * addresses (ascending) to host memory. */
+int zxh_k_sample(const void *state, int precision, size_t m, size_t seg_bits, uint64_t stream_seed, size_t shots,
+                 uint64_t *out_real);
+/* sampler on a caller-provided sorted threshold array (host, length shots) */
+int zxh_k_sample_thresholds(const void *state, int precision, size_t m, size_t seg_bits, const double *thresholds,
+                            size_t shots, uint64_t *out_real);
+/* sum of |amplitude|^2 in the reference's pairwise order (f64) */
+int zxh_k_prob_total(const void *state, int precision, size_t m, double *out);
+/* HIP-event timing of the kernel-level calls (same families as zxh_get_timing) */
+int zxh_k_set_timing(int on);
+int zxh_k_get_timing(int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
+int zxh_k_reset_timing(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

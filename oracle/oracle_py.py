@@ -75,6 +75,7 @@ def oracle_lib() -> C.CDLL:
         lib.zo_mt64_nth.argtypes = [_u64, _sz]
         lib.zo_thresholds.argtypes = [_u64, _sz, C.c_double, C.c_double, _dp]
         lib.zo_sample_state.argtypes = [_dp, _sz, _sz, _u64, _sz, _u64p]
+        lib.zo_sample_thresholds.argtypes = [_dp, _sz, _sz, _dp, _sz, _u64p]
         lib.zo_prob_total.restype = C.c_double
         lib.zo_prob_total.argtypes = [_dp, _sz]
         lib.zo_k_u3_batch.argtypes = [_dp, _sz, C.POINTER(U3Desc), _sz]
@@ -260,6 +261,25 @@ def oracle_sample_state(psi: np.ndarray, seg_bits: int, stream_seed: int, shots:
     if rc:
         raise RuntimeError(f"oracle sampler failed ({rc})")
     return out[:shots]
+
+
+def oracle_sample_thresholds(psi: np.ndarray, seg_bits: int, thr: np.ndarray) -> np.ndarray:
+    """The reference's segment walk on caller-given sorted thresholds."""
+    lib = oracle_lib()
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    thr = np.ascontiguousarray(thr, dtype=np.float64)
+    m = int(psi.size).bit_length() - 1
+    out = np.zeros(max(thr.size, 1), dtype=np.uint64)
+    rc = lib.zo_sample_thresholds(_dptr(psi.view(np.float64)), m, seg_bits, _dptr(thr), thr.size, _u64ptr(out))
+    if rc:
+        raise RuntimeError(f"oracle sampler failed ({rc})")
+    return out[: thr.size]
+
+
+def prob_total(psi: np.ndarray) -> float:
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    m = int(psi.size).bit_length() - 1
+    return float(oracle_lib().zo_prob_total(_dptr(psi.view(np.float64)), m))
 
 
 def ref_sample_state(psi: np.ndarray, seed: int, shots: int) -> np.ndarray:

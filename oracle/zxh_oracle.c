@@ -1094,7 +1094,31 @@ static size_t upper_bound_clamped(const double *cdf, size_t n, double tau)
     return first == n ? n - 1 : first;
 }
 
+/* threshold source: the reference's stream, or a caller-given sorted array */
+typedef struct {
+    tstream ts;
+    const double *arr;
+    size_t pos, cnt;
+} tsource;
+static size_t src_rest(const tsource *s)
+{
+    return s->arr ? s->cnt - s->pos : s->ts.rest;
+}
+static double src_head(const tsource *s)
+{
+    return s->arr ? s->arr[s->pos] : s->ts.head;
+}
+static void src_pop(tsource *s)
+{
+    if (s->arr)
+        s->pos++;
+    else
+        ts_pop(&s->ts);
+}
+
 /* sample_local_streaming (measure_streaming_impl.h:97-147) on 2^m amplitudes */
+static int sample_streaming_src(const cplx *p, size_t m, size_t I, tsource *src, size_t cnt, uint64_t *results);
+
 static int sample_streaming(const cplx *p, size_t m, size_t I, double local_prob, uint64_t stream_seed, size_t cnt,
                             uint64_t *results)
 {
@@ -1102,6 +1126,26 @@ static int sample_streaming(const cplx *p, size_t m, size_t I, double local_prob
         return ZO_OK;
     if (local_prob <= 0.0)
         return ZO_ERR_ZERO_PROB;
+    tsource src;
+    memset(&src, 0, sizeof(src));
+    ts_init(&src.ts, cnt, 0.0, local_prob, stream_seed);
+    return sample_streaming_src(p, m, I, &src, cnt, results);
+}
+
+int zo_sample_thresholds(const double *psi, size_t m, size_t seg_bits, const double *thr, size_t cnt, uint64_t *out)
+{
+    const size_t I = m < seg_bits ? m : seg_bits;
+    tsource src;
+    memset(&src, 0, sizeof(src));
+    src.arr = thr;
+    src.cnt = cnt;
+    return sample_streaming_src((const cplx *)psi, m, I, &src, cnt, out);
+}
+
+static int sample_streaming_src(const cplx *p, size_t m, size_t I, tsource *src, size_t cnt, uint64_t *results)
+{
+    if (cnt == 0)
+        return ZO_OK;
     const size_t n = (size_t)1 << m;
     const size_t seg_len = (size_t)1 << I;
     double *cdf = (double *)malloc(seg_len * sizeof(double));
@@ -1113,14 +1157,12 @@ static int sample_streaming(const cplx *p, size_t m, size_t I, double local_prob
         free(pin);
         return ZO_ERR_ALLOC;
     }
-    tstream ts;
-    ts_init(&ts, cnt, 0.0, local_prob, stream_seed);
     seg_prefix sp;
     memset(&sp, 0, sizeof(sp));
     size_t pcnt = 0;
     size_t seg = 0;
     double cur_prefix = sp_prefix(&sp);
-    while (seg < n && ts.rest != 0) {
+    while (seg < n && src_rest(src) != 0) {
         /* segment_cdf_t::pre_cdf (measure_general.cpp:182-203) */
         const size_t valid = seg_len < n - seg ? seg_len : n - seg;
         size_t padded = 1;
@@ -1134,15 +1176,15 @@ static int sample_streaming(const cplx *p, size_t m, size_t I, double local_prob
         const double tail = cur_prefix + total;
         sp_push(&sp, total);
         /* consume_stream_in_segment */
-        while (ts.rest != 0 && ts.head < tail) {
+        while (src_rest(src) != 0 && src_head(src) < tail) {
             if (pcnt >= cnt) {
                 free(cdf);
                 free(work);
                 free(pin);
                 return ZO_ERR_UNRESOLVED;
             }
-            results[pcnt++] = seg + upper_bound_clamped(cdf, valid, ts.head);
-            ts_pop(&ts);
+            results[pcnt++] = seg + upper_bound_clamped(cdf, valid, src_head(src));
+            src_pop(src);
         }
         seg += seg_len;
         cur_prefix = sp_prefix(&sp);
@@ -1150,7 +1192,7 @@ static int sample_streaming(const cplx *p, size_t m, size_t I, double local_prob
     free(cdf);
     free(work);
     free(pin);
-    if (ts.rest != 0 || pcnt != cnt)
+    if (src_rest(src) != 0 || pcnt != cnt)
         return ZO_ERR_UNRESOLVED;
     return ZO_OK;
 }

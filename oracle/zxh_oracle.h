@@ -100,6 +100,8 @@ int zo_thresholds(uint64_t stream_seed, size_t cnt, double begin, double end, do
  * Writes cnt real addresses (ascending). */
 int zo_sample_state(const double *psi, size_t m, size_t seg_bits, uint64_t stream_seed, size_t cnt,
                     uint64_t *out_real);
+/* the same sampler on caller-given sorted thresholds */
+int zo_sample_thresholds(const double *psi, size_t m, size_t seg_bits, const double *thr, size_t cnt, uint64_t *out);
 double zo_prob_total(const double *psi, size_t m);
 
 /* kernels over a raw block of 2^local_bits amplitudes (kernel_general.cpp) */

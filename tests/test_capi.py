@@ -1,0 +1,35 @@
+"""The C-ABI library loads and exports every function include/zxh_amd.h
+declares (no compute call: this runs without a GPU)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+from zxh_adae_amd import _lib
+
+HEADER = os.path.join(ROOT, "include", "zxh_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(zxh_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_parses():
+    names = declared_functions()
+    assert "zxh_create" in names and "zxh_k_sample" in names and len(names) > 40
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="native library not built")
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="native library not built")
+def test_python_signatures_cover_header():
+    assert set(declared_functions()) == set(_lib.EXPORTED)

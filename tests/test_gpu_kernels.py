@@ -1,0 +1,267 @@
+"""Per-kernel parity of the HIP passes against the oracle (same inputs)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import sampler_model
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+MANIFEST = json.load(open(os.path.join(GOLDEN, "manifest.json")))
+SEL_NONE = 0xFFFFFFFFFFFFFFFF
+TOL128 = 1e-12  # north-star bound for f64 amplitudes
+TOL64 = 2e-5    # complex64 state with f32 arithmetic
+
+
+def rand_state(rng, n, dtype=np.complex128):
+    psi = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return (psi / np.linalg.norm(psi)).astype(dtype)
+
+
+def dev(gpu_engine, psi, prec="c128"):
+    from zxh_adae_amd.kernels import DeviceState
+    n = psi.size.bit_length() - 1
+    d = DeviceState(n, prec)
+    d.upload(psi)
+    return d
+
+
+def pack(alpha, beta):
+    return (alpha & ((1 << 63) - 1)) | ((1 << 63) if beta else 0)
+
+
+# ---------------- sampler ----------------
+
+SAMPLER_STATES = np.load(os.path.join(GOLDEN, "sampler_states.npz"))
+
+
+@pytest.mark.parametrize("seg", MANIFEST["sampler_segments"])
+def test_sampler_matches_reference_golden(gpu_engine, seg):
+    gold = np.load(os.path.join(GOLDEN, f"samp_seg{seg}.npz"))
+    for key in gold.files:
+        name, seed = key.split("__")
+        psi = SAMPLER_STATES[name]
+        d = dev(gpu_engine, psi)
+        got = d.sample(seg, O.mix_seed(int(seed), 0), gold[key].size)
+        assert np.array_equal(got, gold[key]), (name, seed, seg)
+
+
+@pytest.mark.parametrize("seg", [1, 2, 4, 5, 7, 18])
+def test_sampler_adversarial_thresholds(gpu_engine, seg):
+    rng = np.random.default_rng(100 + seg)
+    psi = (rng.normal(size=1 << 9) * np.exp(rng.normal(size=1 << 9) * 3)).astype(np.complex128)
+    cdf = sampler_model.reference_cdf(psi, seg)
+    bound = min(O.prob_total(psi), float(np.max(sampler_model.segment_tails(psi, seg))))
+    thr = np.concatenate([cdf, np.nextafter(cdf, 0.0), np.nextafter(cdf, np.inf)])
+    thr = np.sort(thr[(thr >= 0) & (thr < bound)])
+    want = O.oracle_sample_thresholds(psi, seg, thr)
+    d = dev(gpu_engine, psi)
+    assert np.array_equal(d.sample_thresholds(seg, thr), want)
+
+
+@pytest.mark.parametrize("m,seg", [(14, 18), (16, 5), (20, 18), (21, 9)])
+def test_sampler_random_states(gpu_engine, m, seg):
+    rng = np.random.default_rng(m * 31 + seg)
+    psi = rand_state(rng, m)
+    psi[rng.random(psi.size) < 0.3] = 0
+    d = dev(gpu_engine, psi)
+    for seed in (3, 99):
+        want = O.oracle_sample_state(psi, seg, O.mix_seed(seed, 0), 20000)
+        assert np.array_equal(d.sample(seg, O.mix_seed(seed, 0), 20000), want)
+    assert d.prob_total() == O.prob_total(psi)
+
+
+def test_sampler_c64_state_bit_exact_on_widened_probabilities(gpu_engine):
+    rng = np.random.default_rng(5)
+    psi = rand_state(rng, 13, np.complex64)
+    d = dev(gpu_engine, psi, "c64")
+    want = O.oracle_sample_state(psi.astype(np.complex128), 6, O.mix_seed(77, 0), 5000)
+    assert np.array_equal(d.sample(6, O.mix_seed(77, 0), 5000), want)
+
+
+def test_sampler_zero_state_raises(gpu_engine):
+    from zxh_adae_amd import ZXHError
+    d = dev(gpu_engine, np.zeros(1 << 6, dtype=np.complex128))
+    with pytest.raises(ZXHError):
+        d.sample(3, 1, 10)
+
+
+# ---------------- diagonal window ----------------
+
+def diag_window(rng, n, count, multi_bit=False):
+    s0, s1, th = [], [], []
+    for _ in range(count):
+        def sel():
+            if multi_bit and rng.random() < 0.5:
+                alpha = int(rng.integers(1, 1 << n))
+            else:
+                alpha = 1 << int(rng.integers(n)) if rng.random() > 0.05 else 0
+            return pack(alpha, bool(rng.random() < 0.3))
+        a = sel()
+        if rng.random() < 0.5:
+            s0.append(a)
+            s1.append(SEL_NONE)
+        else:
+            b = sel()
+            s0.append(a)
+            s1.append(b)
+        th.append(float(rng.uniform(0, 2 * math.pi)))
+    return np.array(s0, dtype=np.uint64), np.array(s1, dtype=np.uint64), np.array(th)
+
+
+def oracle_diag(psi, n, s0, s1, th):
+    out = psi.astype(np.complex128).copy()
+    lib = O.oracle_lib()
+    # apply in bucketable groups (the oracle rejects cross CP like the reference)
+    rc = lib.zo_k_diag(out.view(np.float64).ctypes.data_as(O._dp), n, s0.ctypes.data_as(O._u64p),
+                       s1.ctypes.data_as(O._u64p), th.ctypes.data_as(O._dp), th.size)
+    assert rc == 0
+    return out
+
+
+def bucketable(s0, s1):
+    def cls(p):
+        a = int(p) & ((1 << 63) - 1)
+        return 0 if a >> 8 == 0 else (1 if a & 0xFF == 0 else 2)
+    keep = [i for i in range(s0.size) if s1[i] == SEL_NONE or (cls(s0[i]) != 2 and cls(s1[i]) != 2)]
+    return np.array(keep, dtype=np.int64)
+
+
+@pytest.mark.parametrize("n,count", [(5, 20), (11, 40), (12, 1), (12, 64), (13, 256), (16, 100), (18, 16)])
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_diag_single_bit_selectors(gpu_engine, n, count, prec):
+    rng = np.random.default_rng(n * 1000 + count)
+    psi = rand_state(rng, n)
+    s0, s1, th = diag_window(rng, n, count)
+    want = oracle_diag(psi, n, s0, s1, th)
+    d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    d.diag(s0, s1, th)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
+@pytest.mark.parametrize("n", [6, 12, 14])
+def test_diag_multi_bit_selectors(gpu_engine, n):
+    rng = np.random.default_rng(7 + n)
+    psi = rand_state(rng, n)
+    s0, s1, th = diag_window(rng, n, 50, multi_bit=True)
+    k = bucketable(s0, s1)
+    s0, s1, th = s0[k], s1[k], th[k]
+    want = oracle_diag(psi, n, s0, s1, th)
+    d = dev(gpu_engine, psi)
+    d.diag(s0, s1, th)
+    assert np.max(np.abs(d.download() - want)) < TOL128
+
+
+# ---------------- U3 batches ----------------
+
+def rand_u(rng):
+    return O.u3_matrix(*rng.uniform(0, 2 * math.pi, 3))
+
+
+def oracle_u3(psi, n, descs):
+    out = psi.astype(np.complex128).copy()
+    arr = (O.U3Desc * len(descs))()
+    for i, (sel, delta, u) in enumerate(descs):
+        arr[i].packed_sel = sel
+        arr[i].delta = delta
+        f = np.asarray(u).reshape(4)
+        for k in range(4):
+            arr[i].u[2 * k] = f[k].real
+            arr[i].u[2 * k + 1] = f[k].imag
+    O.oracle_lib().zo_k_u3_batch(out.view(np.float64).ctypes.data_as(O._dp), n, arr, len(descs))
+    return out
+
+
+@pytest.mark.parametrize("n,targets", [
+    (4, [0, 1, 2, 3]), (12, list(range(12))), (13, [0, 5, 12]), (16, list(range(16))),
+    (16, [12, 13, 14, 15]), (17, [2, 9, 16]), (20, list(range(0, 20, 2))), (22, [21]), (12, [3]), (14, [4, 8, 13])])
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_u3_batch_plain_selectors(gpu_engine, n, targets, prec):
+    rng = np.random.default_rng(n * 7 + len(targets))
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in targets]
+    want = oracle_u3(psi, n, descs)
+    d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    d.u3_batch(descs)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
+def random_affine(rng, n, cx_count):
+    """Rows alpha_q of A and pair offsets delta_q = A^-1 e_q of an affine map
+    built by CX row transport from the identity (alpha_i . delta_j = [i == j])."""
+    rows = [1 << q for q in range(n)]
+    for _ in range(cx_count):
+        c, t = rng.choice(n, 2, replace=False)
+        rows[int(t)] ^= rows[int(c)]
+    # invert over GF(2): augment and eliminate
+    aug = [(rows[q], 1 << q) for q in range(n)]
+    for col in range(n):
+        piv = next(r for r in range(col, n) if (aug[r][0] >> col) & 1)
+        aug[col], aug[piv] = aug[piv], aug[col]
+        for r in range(n):
+            if r != col and (aug[r][0] >> col) & 1:
+                aug[r] = (aug[r][0] ^ aug[col][0], aug[r][1] ^ aug[col][1])
+    inv_rows = [aug[q][1] for q in range(n)]  # rows of A^-1
+    deltas = [sum(((inv_rows[r] >> q) & 1) << r for r in range(n)) for q in range(n)]
+    return rows, deltas
+
+
+@pytest.mark.parametrize("n,cx", [(8, 6), (13, 3), (15, 10)])
+def test_u3_batch_affine_selectors(gpu_engine, n, cx):
+    """Selectors and pair offsets of a CX-transported map: extra alpha bits
+    (per-pair flip), beta = 1 (swapped roles), multi-bit deltas (pair kernel)."""
+    rng = np.random.default_rng(n)
+    rows, deltas = random_affine(rng, n, cx)
+    for q in range(n):
+        assert bin(rows[q] & deltas[q]).count("1") % 2 == 1
+    psi = rand_state(rng, n)
+    descs = [(pack(rows[q], bool(rng.random() < 0.5)), deltas[q], rand_u(rng)) for q in range(n) if rng.random() < 0.7]
+    want = oracle_u3(psi, n, descs)
+    d = dev(gpu_engine, psi)
+    d.u3_batch(descs)
+    assert np.max(np.abs(d.download() - want)) < TOL128
+
+
+# ---------------- permutations and CP ----------------
+
+@pytest.mark.parametrize("n", [5, 13])
+def test_x_cx_cp_against_oracle(gpu_engine, n):
+    rng = np.random.default_rng(n + 40)
+    psi = rand_state(rng, n)
+    lib = O.oracle_lib()
+    want = psi.copy()
+    d = dev(gpu_engine, psi)
+    for step in range(6):
+        q, c = rng.choice(n, 2, replace=False)
+        sel_q = pack(1 << int(q) | (int(rng.integers(0, 1 << n)) & ~(1 << int(q)) if step % 2 else 0), step % 3 == 0)
+        sel_c = pack(1 << int(c), step % 2 == 1)
+        p = want.view(np.float64).ctypes.data_as(O._dp)
+        if step % 3 == 0:
+            lib.zo_k_x(p, n, 1 << int(q), sel_q)
+            d.x(1 << int(q), sel_q)
+        elif step % 3 == 1:
+            lib.zo_k_cx(p, n, 1 << int(q), sel_c, sel_q)
+            d.cx(1 << int(q), sel_c, sel_q)
+        else:
+            lib.zo_k_cp(p, n, sel_c, sel_q, 0.9 + step)
+            d.cp(sel_c, sel_q, 0.9 + step)
+    assert np.max(np.abs(d.download() - want)) < TOL128
+
+
+@pytest.mark.parametrize("n,ctrls,target", [(6, [0, 2], 5), (14, [1, 3, 7, 9, 11, 13], 0), (16, [15, 4], 8)])
+def test_mcx_is_exact_permutation(gpu_engine, n, ctrls, target):
+    rng = np.random.default_rng(n)
+    psi = rand_state(rng, n)
+    idx = np.arange(1 << n)
+    mask = sum(1 << c for c in ctrls)
+    perm = np.where((idx & mask) == mask, idx ^ (1 << target), idx)
+    d = dev(gpu_engine, psi)
+    d.mcx(mask, target)
+    assert np.array_equal(d.download(), psi[perm])

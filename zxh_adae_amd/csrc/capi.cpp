@@ -1,0 +1,491 @@
+// capi.cpp -- the extern "C" boundary declared in include/zxh_amd.h.
+#include "../../include/zxh_amd.h"
+
+#include "engine.h"
+
+#include <cstring>
+#include <new>
+
+using zxa::Sim;
+
+struct zxh_sim
+{
+    Sim *sim;
+};
+
+namespace
+{
+int fail(int code, const char *msg)
+{
+    zxa::set_error(msg);
+    return code;
+}
+#define CHECK_SIM(s)                                                                                                   \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        if (!(s) || !(s)->sim)                                                                                         \
+            return fail(ZXH_EINVAL, "null simulator handle");                                                         \
+    } while (0)
+} // namespace
+
+extern "C" {
+
+const char *zxh_last_error(void)
+{
+    return zxa::last_error();
+}
+
+int zxh_init(void)
+{
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(ZXH_ENODEV, "no HIP device available");
+    zxa::runtime_set_active(true);
+    return ZXH_OK;
+}
+
+int zxh_finalize(void)
+{
+    zxa::runtime_set_active(false);
+    return ZXH_OK;
+}
+
+int zxh_active(void)
+{
+    return zxa::runtime_active() ? 1 : 0;
+}
+
+int zxh_rank(void)
+{
+    return 0;
+}
+
+int zxh_nprocs(void)
+{
+    return 1;
+}
+
+int zxh_create(size_t n, int disable_x, int eager_expand_all, int precision, zxh_sim **out)
+{
+    if (!out)
+        return fail(ZXH_EINVAL, "null output handle");
+    *out = nullptr;
+    if (!zxa::runtime_active())
+        return fail(ZXH_ESTATE, "init must be called before constructing or using ZXH objects");
+    if (precision != ZXH_C128 && precision != ZXH_C64)
+        return fail(ZXH_EINVAL, "precision must be ZXH_C128 or ZXH_C64");
+    Sim *sim = new (std::nothrow) Sim(n, disable_x != 0, eager_expand_all != 0, precision);
+    if (!sim)
+        return fail(ZXH_ENOMEM, "host allocation failed");
+    int rc = sim->init();
+    if (rc)
+    {
+        delete sim;
+        return rc;
+    }
+    zxh_sim *h = new (std::nothrow) zxh_sim{sim};
+    if (!h)
+    {
+        delete sim;
+        return fail(ZXH_ENOMEM, "host allocation failed");
+    }
+    *out = h;
+    return ZXH_OK;
+}
+
+int zxh_destroy(zxh_sim *s)
+{
+    if (!s)
+        return ZXH_OK;
+    delete s->sim;
+    delete s;
+    return ZXH_OK;
+}
+
+int zxh_set_seed(zxh_sim *s, uint64_t seed)
+{
+    CHECK_SIM(s);
+    s->sim->set_seed(seed);
+    return ZXH_OK;
+}
+
+int zxh_clear_seed(zxh_sim *s)
+{
+    CHECK_SIM(s);
+    s->sim->clear_seed();
+    return ZXH_OK;
+}
+
+int zxh_clear_gates(zxh_sim *s)
+{
+    CHECK_SIM(s);
+    s->sim->clear_gates();
+    return ZXH_OK;
+}
+
+int zxh_gate(zxh_sim *s, int code, size_t cq, size_t q, double theta, double lambda, double phi)
+{
+    CHECK_SIM(s);
+    return s->sim->add_gate(code, cq, q, theta, lambda, phi);
+}
+
+int zxh_barrier(zxh_sim *s)
+{
+    return zxh_gate(s, ZXH_G_BARRIER, 0, 0, 0.0, 0.0, 0.0);
+}
+int zxh_rz(zxh_sim *s, size_t q, double theta)
+{
+    return zxh_gate(s, ZXH_G_RZ, 0, q, theta, 0.0, 0.0);
+}
+int zxh_crz(zxh_sim *s, size_t cq, size_t q, double theta)
+{
+    return zxh_gate(s, ZXH_G_CRZ, cq, q, theta, 0.0, 0.0);
+}
+int zxh_cp(zxh_sim *s, size_t cq, size_t q, double theta)
+{
+    return zxh_gate(s, ZXH_G_CP, cq, q, theta, 0.0, 0.0);
+}
+int zxh_p(zxh_sim *s, size_t q, double theta)
+{
+    return zxh_gate(s, ZXH_G_P, 0, q, theta, 0.0, 0.0);
+}
+int zxh_z(zxh_sim *s, size_t q)
+{
+    return zxh_gate(s, ZXH_G_Z, 0, q, 0.0, 0.0, 0.0);
+}
+int zxh_x(zxh_sim *s, size_t q)
+{
+    return zxh_gate(s, ZXH_G_X, 0, q, 0.0, 0.0, 0.0);
+}
+int zxh_cx(zxh_sim *s, size_t cq, size_t q)
+{
+    return zxh_gate(s, ZXH_G_CX, cq, q, 0.0, 0.0, 0.0);
+}
+int zxh_h(zxh_sim *s, size_t q)
+{
+    return zxh_gate(s, ZXH_G_H, 0, q, 0.0, 0.0, 0.0);
+}
+int zxh_u3(zxh_sim *s, size_t q, double theta, double lambda, double phi)
+{
+    return zxh_gate(s, ZXH_G_U3, 0, q, theta, lambda, phi);
+}
+int zxh_rx(zxh_sim *s, size_t q, double theta)
+{
+    return zxh_gate(s, ZXH_G_RX, 0, q, theta, 0.0, 0.0);
+}
+
+int zxh_execute(zxh_sim *s)
+{
+    CHECK_SIM(s);
+    return s->sim->execute();
+}
+
+int zxh_measure(zxh_sim *s, size_t shots)
+{
+    CHECK_SIM(s);
+    return s->sim->measure(shots);
+}
+
+int zxh_measured_count(const zxh_sim *s, size_t *out)
+{
+    CHECK_SIM(s);
+    *out = s->sim->measured_count();
+    return ZXH_OK;
+}
+
+int zxh_get_results(const zxh_sim *s, uint64_t *packed, size_t shots)
+{
+    CHECK_SIM(s);
+    return s->sim->get_results(packed, shots);
+}
+
+int zxh_get_real_results(const zxh_sim *s, uint64_t *real, size_t shots)
+{
+    CHECK_SIM(s);
+    return s->sim->get_real_results(real, shots);
+}
+
+int zxh_get_state(const zxh_sim *s, double *out, size_t out_len)
+{
+    CHECK_SIM(s);
+    return s->sim->get_state(out, out_len);
+}
+
+int zxh_required_M(const zxh_sim *s, size_t *out)
+{
+    CHECK_SIM(s);
+    *out = s->sim->required_M();
+    return ZXH_OK;
+}
+
+int zxh_num_qubits(const zxh_sim *s, size_t *out)
+{
+    CHECK_SIM(s);
+    *out = s->sim->num_qubits();
+    return ZXH_OK;
+}
+
+int zxh_set_policy(zxh_sim *s, int policy)
+{
+    CHECK_SIM(s);
+    if (policy != ZXH_POLICY_REFERENCE)
+        return fail(ZXH_EINVAL, "unknown batching policy");
+    return ZXH_OK;
+}
+
+int zxh_set_seg_bits(zxh_sim *s, size_t bits)
+{
+    CHECK_SIM(s);
+    if (bits > 40)
+        return fail(ZXH_EINVAL, "segment bits out of range");
+    s->sim->set_seg_bits(bits);
+    return ZXH_OK;
+}
+
+int zxh_get_phys_state(const zxh_sim *s, void *out, size_t out_bytes, size_t *m_out)
+{
+    CHECK_SIM(s);
+    return s->sim->get_phys_state(out, out_bytes, m_out);
+}
+
+int zxh_set_phys_state(zxh_sim *s, const void *in, size_t in_bytes)
+{
+    CHECK_SIM(s);
+    return s->sim->set_phys_state(in, in_bytes);
+}
+
+int zxh_state_device_ptr(const zxh_sim *s, void **dptr, size_t *m_out)
+{
+    CHECK_SIM(s);
+    return s->sim->device_ptr(dptr, m_out);
+}
+
+int zxh_set_timing(zxh_sim *s, int on)
+{
+    CHECK_SIM(s);
+    s->sim->ctx().timing = on != 0;
+    return ZXH_OK;
+}
+
+int zxh_get_timing(zxh_sim *s, int family, uint64_t *launches, double *total_ms, uint64_t *bytes)
+{
+    CHECK_SIM(s);
+    if (family < 0 || family >= zxa::F_COUNT)
+        return fail(ZXH_EINVAL, "timing family out of range");
+    int rc = s->sim->ctx().collect();
+    if (rc)
+        return rc;
+    const zxa::FamilyTiming &f = s->sim->ctx().fam[family];
+    *launches = f.launches;
+    *total_ms = f.ms;
+    *bytes = f.bytes;
+    return ZXH_OK;
+}
+
+int zxh_reset_timing(zxh_sim *s)
+{
+    CHECK_SIM(s);
+    int rc = s->sim->ctx().collect();
+    for (auto &f : s->sim->ctx().fam)
+        f = zxa::FamilyTiming();
+    return rc;
+}
+
+// ---- kernel level ----
+
+int zxh_dev_alloc(size_t bytes, void **dptr)
+{
+    if (!dptr)
+        return fail(ZXH_EINVAL, "null output pointer");
+    hipError_t e = hipMalloc(dptr, bytes);
+    if (e != hipSuccess)
+    {
+        *dptr = nullptr;
+        return fail(ZXH_ENOMEM, hipGetErrorString(e));
+    }
+    return ZXH_OK;
+}
+
+int zxh_dev_free(void *dptr)
+{
+    return zxa::hip_check(hipFree(dptr), "hipFree");
+}
+
+int zxh_dev_upload(void *dptr, const void *host, size_t bytes)
+{
+    return zxa::hip_check(hipMemcpy(dptr, host, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+}
+
+int zxh_dev_download(void *host, const void *dptr, size_t bytes)
+{
+    return zxa::hip_check(hipMemcpy(host, dptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+}
+
+int zxh_dev_sync(void)
+{
+    int rc = 0;
+    zxa::Ctx *c = zxa::default_ctx(&rc);
+    if (!c)
+        return rc;
+    rc = zxa::hip_check(hipStreamSynchronize(c->stream), "stream sync");
+    if (rc)
+        return rc;
+    return zxa::hip_check(hipDeviceSynchronize(), "device sync");
+}
+
+#define WITH_CTX(c)                                                                                                    \
+    int rc_ctx_ = 0;                                                                                                   \
+    zxa::Ctx *c = zxa::default_ctx(&rc_ctx_);                                                                          \
+    if (!c)                                                                                                            \
+        return rc_ctx_;
+
+static int finish(zxa::Ctx *c, int rc)
+{
+    if (rc)
+        return rc;
+    return zxa::hip_check(hipStreamSynchronize(c->stream), "stream sync");
+}
+
+static int check_kernel_args(const void *state, int precision, size_t local_bits)
+{
+    if (!state)
+        return fail(ZXH_EINVAL, "null state pointer");
+    if (precision != ZXH_C128 && precision != ZXH_C64)
+        return fail(ZXH_EINVAL, "precision must be ZXH_C128 or ZXH_C64");
+    if (local_bits > 40)
+        return fail(ZXH_EINVAL, "local_bits out of range");
+    return ZXH_OK;
+}
+
+int zxh_k_u3_batch(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    static_assert(sizeof(zxh_u3_desc) == sizeof(zxa::U3Gate), "descriptor layout");
+    return finish(c, zxa::u3_apply(*c, state, precision, unsigned(local_bits),
+                                   reinterpret_cast<const zxa::U3Gate *>(descs), ndesc));
+}
+
+int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
+               const double *theta, size_t ngates)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    std::vector<zxa::DiagGate> g(ngates);
+    for (size_t i = 0; i < ngates; i++)
+        g[i] = {sel0[i], sel1[i], theta[i]};
+    return finish(c, zxa::diag_apply(*c, state, precision, unsigned(local_bits), g.data(), ngates));
+}
+
+int zxh_k_x(void *state, int precision, size_t local_bits, uint64_t delta, uint64_t packed_sel)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return finish(c, zxa::x_apply(*c, state, precision, unsigned(local_bits), delta, packed_sel));
+}
+
+int zxh_k_cx(void *state, int precision, size_t local_bits, uint64_t delta, uint64_t packed_sel_cq,
+             uint64_t packed_sel_q)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return finish(c, zxa::cx_apply(*c, state, precision, unsigned(local_bits), delta, packed_sel_cq, packed_sel_q));
+}
+
+int zxh_k_cp(void *state, int precision, size_t local_bits, uint64_t packed_sel_cq, uint64_t packed_sel_q,
+             double theta)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return finish(c, zxa::cp_apply(*c, state, precision, unsigned(local_bits), packed_sel_cq, packed_sel_q, theta));
+}
+
+int zxh_k_mcx(void *state, int precision, size_t local_bits, uint64_t ctrl_mask, size_t target)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return finish(c, zxa::mcx_apply(*c, state, precision, unsigned(local_bits), ctrl_mask, unsigned(target)));
+}
+
+int zxh_k_sample(const void *state, int precision, size_t m, size_t seg_bits, uint64_t stream_seed, size_t shots,
+                 uint64_t *out_real)
+{
+    int rc = check_kernel_args(state, precision, m);
+    if (rc)
+        return rc;
+    if (shots == 0)
+        return ZXH_OK;
+    WITH_CTX(c);
+    double total = 0.0;
+    rc = zxa::sample_prepare(*c, state, precision, unsigned(m), unsigned(seg_bits), &total);
+    if (rc)
+        return rc;
+    if (!(total > 0.0))
+        return fail(ZXH_EZEROPROB, "state has zero total probability");
+    std::vector<double> thr(shots);
+    zxa::threshold_stream(stream_seed, shots, 0.0, total, thr.data());
+    return zxa::sample_search(*c, state, precision, thr.data(), shots, out_real);
+}
+
+int zxh_k_sample_thresholds(const void *state, int precision, size_t m, size_t seg_bits, const double *thresholds,
+                            size_t shots, uint64_t *out_real)
+{
+    int rc = check_kernel_args(state, precision, m);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return zxa::sample_apply(*c, state, precision, unsigned(m), unsigned(seg_bits), thresholds, shots, out_real);
+}
+
+int zxh_k_prob_total(const void *state, int precision, size_t m, double *out)
+{
+    int rc = check_kernel_args(state, precision, m);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return zxa::prob_total(*c, state, precision, unsigned(m), out);
+}
+
+int zxh_k_set_timing(int on)
+{
+    WITH_CTX(c);
+    c->timing = on != 0;
+    return ZXH_OK;
+}
+
+int zxh_k_get_timing(int family, uint64_t *launches, double *total_ms, uint64_t *bytes)
+{
+    WITH_CTX(c);
+    if (family < 0 || family >= zxa::F_COUNT)
+        return fail(ZXH_EINVAL, "timing family out of range");
+    int rc = c->collect();
+    if (rc)
+        return rc;
+    *launches = c->fam[family].launches;
+    *total_ms = c->fam[family].ms;
+    *bytes = c->fam[family].bytes;
+    return ZXH_OK;
+}
+
+int zxh_k_reset_timing(void)
+{
+    WITH_CTX(c);
+    int rc = c->collect();
+    for (auto &f : c->fam)
+        f = zxa::FamilyTiming();
+    return rc;
+}
+
+} // extern "C"

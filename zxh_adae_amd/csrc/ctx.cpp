@@ -1,0 +1,269 @@
+// ctx.cpp -- device context, scratch buffers, parameter ring, errors, and the
+// host threshold stream of the sampler.
+#include "dev.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <random>
+
+// The threshold recurrence must round every product and sum separately, as
+// the reference's x86-64 build does.
+#pragma clang fp contract(off)
+
+namespace zxa
+{
+
+namespace
+{
+thread_local std::string g_err;
+}
+
+void set_error(const std::string &msg)
+{
+    g_err = msg;
+}
+
+const char *last_error()
+{
+    return g_err.c_str();
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess)
+        return OK;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return EHIP;
+}
+
+int DevBuf::ensure(size_t need)
+{
+    if (need <= bytes && ptr)
+        return OK;
+    release();
+    size_t cap = std::max<size_t>(need, 4096);
+    hipError_t e = hipMalloc(&ptr, cap);
+    if (e != hipSuccess)
+    {
+        ptr = nullptr;
+        bytes = 0;
+        set_error(std::string("hipMalloc scratch failed: ") + hipGetErrorString(e));
+        return ENOMEM_;
+    }
+    bytes = cap;
+    return OK;
+}
+
+void DevBuf::release()
+{
+    if (ptr)
+        (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+int ParamRing::init()
+{
+    if (host_)
+        return OK;
+    ZXA_HIP(hipHostMalloc(&host_, size_t(kSlots) * kSlotBytes, hipHostMallocDefault));
+    ZXA_HIP(hipMalloc(&dev_, size_t(kSlots) * kSlotBytes));
+    for (int i = 0; i < kSlots; i++)
+        ZXA_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
+    return OK;
+}
+
+void ParamRing::release()
+{
+    for (int i = 0; i < kSlots; i++)
+    {
+        if (ev_[i])
+        {
+            if (used_[i])
+                (void)hipEventSynchronize(ev_[i]);
+            (void)hipEventDestroy(ev_[i]);
+        }
+        ev_[i] = nullptr;
+        used_[i] = false;
+    }
+    if (dev_)
+        (void)hipFree(dev_);
+    if (host_)
+        (void)hipHostFree(host_);
+    dev_ = host_ = nullptr;
+}
+
+int ParamRing::stage(const void *src, size_t bytes, hipStream_t stream, void **dev)
+{
+    if (bytes > kSlotBytes)
+    {
+        set_error("parameter block exceeds ring slot size");
+        return EINVAL_;
+    }
+    const int s = next_;
+    next_ = (next_ + 1) % kSlots;
+    if (used_[s])
+        ZXA_HIP(hipEventSynchronize(ev_[s]));
+    char *h = static_cast<char *>(host_) + size_t(s) * kSlotBytes;
+    char *d = static_cast<char *>(dev_) + size_t(s) * kSlotBytes;
+    std::memcpy(h, src, bytes);
+    ZXA_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+    cur_ = s;
+    *dev = d;
+    return OK;
+}
+
+int ParamRing::done(hipStream_t stream)
+{
+    if (cur_ < 0)
+        return OK;
+    ZXA_HIP(hipEventRecord(ev_[cur_], stream));
+    used_[cur_] = true;
+    cur_ = -1;
+    return OK;
+}
+
+int Ctx::init()
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+    {
+        set_error("no HIP device available");
+        return ENODEV_;
+    }
+    ZXA_HIP(hipGetDevice(&device));
+    hipDeviceProp_t prop;
+    ZXA_HIP(hipGetDeviceProperties(&prop, device));
+    num_cus = prop.multiProcessorCount;
+    ZXA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    return ring.init();
+}
+
+void Ctx::release()
+{
+    if (stream)
+        (void)hipStreamSynchronize(stream);
+    ring.release();
+    scratch_a.release();
+    scratch_b.release();
+    scratch_c.release();
+    scratch_d.release();
+    (void)collect();
+    for (hipEvent_t e : pool_)
+        (void)hipEventDestroy(e);
+    pool_.clear();
+    if (stream)
+        (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+int Ctx::take(hipEvent_t *e)
+{
+    if (!pool_.empty())
+    {
+        *e = pool_.back();
+        pool_.pop_back();
+        return OK;
+    }
+    return hip_check(hipEventCreate(e), "hipEventCreate");
+}
+
+int Ctx::tic()
+{
+    if (!timing)
+        return OK;
+    ZXA_TRY(take(&cur_));
+    ZXA_HIP(hipEventRecord(cur_, stream));
+    return OK;
+}
+
+int Ctx::toc(int family, uint64_t bytes)
+{
+    if (!timing || !cur_)
+        return OK;
+    hipEvent_t b = nullptr;
+    ZXA_TRY(take(&b));
+    ZXA_HIP(hipEventRecord(b, stream));
+    pending_.push_back({family, bytes, cur_, b});
+    cur_ = nullptr;
+    if (pending_.size() >= 4096)
+        return collect();
+    return OK;
+}
+
+int Ctx::collect()
+{
+    for (const Pending &p : pending_)
+    {
+        ZXA_HIP(hipEventSynchronize(p.b));
+        float ms = 0.f;
+        ZXA_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+        fam[p.family].launches++;
+        fam[p.family].ms += ms;
+        fam[p.family].bytes += p.bytes;
+        pool_.push_back(p.a);
+        pool_.push_back(p.b);
+    }
+    pending_.clear();
+    return OK;
+}
+
+Ctx *default_ctx(int *rc)
+{
+    static std::mutex mu;
+    static Ctx *ctx = nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ctx)
+    {
+        Ctx *c = new Ctx();
+        int r = c->init();
+        if (r)
+        {
+            delete c;
+            *rc = r;
+            return nullptr;
+        }
+        ctx = c;
+    }
+    *rc = OK;
+    return ctx;
+}
+
+// SplitMix64 finaliser (measure_general.cpp:16-22).
+uint64_t mix_seed(uint64_t base, uint64_t salt)
+{
+    uint64_t z = base + 0x9e3779b97f4a7c15ULL + salt;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+// Sorted order statistics of cnt uniforms on [begin, end): each new head is
+// begin + (end - begin) * (1 - u^(1/rest)) with u from mt19937_64 through
+// uniform_real_distribution<double>(0, 1), clamped below by DBL_MIN
+// (threshold_stream_t, measure_common.cpp:14-71).  The engine and the
+// distribution are the standard library's, as in the reference.
+void threshold_stream(uint64_t stream_seed, size_t cnt, double begin, double end, double *out)
+{
+    std::mt19937_64 rng(stream_seed);
+    std::uniform_real_distribution<double> dist(0.0, 1.0);
+    size_t rest = cnt;
+    double lo = begin;
+    for (size_t k = 0; k < cnt; k++)
+    {
+        double u = dist(rng);
+        u = std::max(u, std::numeric_limits<double>::min());
+        const double span = end - lo;
+        const double factor = 1.0 - std::pow(u, 1.0 / static_cast<double>(rest));
+        const double head = lo + span * factor;
+        out[k] = head;
+        lo = head;
+        rest--;
+    }
+}
+
+} // namespace zxa

@@ -1,0 +1,218 @@
+// dev.h -- shared host/device definitions of the MI355X engine.
+//
+// One Ctx per simulator (plus one process-wide Ctx for the kernel-level C
+// entry points) owns a HIP stream of default priority, grow-only device
+// scratch buffers and a ring of parameter slots for per-launch tables.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace zxa
+{
+
+enum Prec
+{
+    C128 = 0,
+    C64 = 1
+};
+
+inline size_t elem_bytes(int prec)
+{
+    return prec == C128 ? 16 : 8;
+}
+
+// Status codes (mirror include/zxh_amd.h).
+enum Status
+{
+    OK = 0,
+    EINVAL_ = 1,
+    ESTATE = 2,
+    EZEROPROB = 3,
+    EUNRESOLVED = 4,
+    ENOMEM_ = 5,
+    EHIP = 6,
+    ENODEV_ = 7
+};
+
+// Thread-local error message, read through zxh_last_error().
+void set_error(const std::string &msg);
+const char *last_error();
+
+// Returns EHIP with the message recorded when e != hipSuccess.
+int hip_check(hipError_t e, const char *what);
+#define ZXA_HIP(call)                                                                                                  \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        int rc_ = ::zxa::hip_check((call), #call);                                                                     \
+        if (rc_)                                                                                                       \
+            return rc_;                                                                                                \
+    } while (0)
+#define ZXA_TRY(call)                                                                                                  \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        int rc_ = (call);                                                                                              \
+        if (rc_)                                                                                                       \
+            return rc_;                                                                                                \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf
+{
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    ~DevBuf()
+    {
+        release();
+    }
+};
+
+// Kernel families for event timing.
+enum Family
+{
+    F_DIAG = 0,
+    F_U3TILE = 1,
+    F_U3PAIR = 2,
+    F_PERM = 3,
+    F_CP = 4,
+    F_SAMPLE = 5,
+    F_RESET = 6,
+    F_COUNT = 8
+};
+
+struct FamilyTiming
+{
+    uint64_t launches = 0;
+    double ms = 0.0;
+    uint64_t bytes = 0; // algorithmic HBM bytes of the timed launches
+};
+
+// Ring of parameter slots: pinned host staging + device copy + event.  A slot
+// is reused only after the event recorded behind its last consumer fired.
+class ParamRing
+{
+  public:
+    static constexpr int kSlots = 16;
+    static constexpr size_t kSlotBytes = 256 * 1024;
+    int init();
+    void release();
+    // Copies bytes from src into the next slot (async H2D on stream) and
+    // returns the device address.  Call done() after the consuming launches.
+    int stage(const void *src, size_t bytes, hipStream_t stream, void **dev);
+    int done(hipStream_t stream);
+
+  private:
+    void *host_ = nullptr;
+    void *dev_ = nullptr;
+    hipEvent_t ev_[kSlots] = {};
+    bool used_[kSlots] = {};
+    int next_ = 0;
+    int cur_ = -1;
+};
+
+struct Ctx
+{
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    ParamRing ring;
+    DevBuf scratch_a; // diag per-chunk coefficients
+    DevBuf scratch_b; // sampler tree levels
+    DevBuf scratch_c; // sampler thresholds / results
+    DevBuf scratch_d; // sampler per-segment tails
+    bool timing = false;
+    FamilyTiming fam[F_COUNT];
+    // sampler state between prepare and search (tree in scratch_b, tails in scratch_d)
+    alignas(8) unsigned char sample_levels[64 * 8] = {};
+    unsigned sample_m = 0, sample_I = 0;
+    bool sample_ready = false;
+    template <typename T>
+    T &sample_levels_as()
+    {
+        static_assert(sizeof(T) <= sizeof(sample_levels), "levels table too large");
+        return *reinterpret_cast<T *>(sample_levels);
+    }
+    int init();
+    void release();
+    ~Ctx()
+    {
+        release();
+    }
+    // Timing brackets around one launch (no-ops when timing is off).  Events
+    // are recorded on the launch stream without synchronising; collect()
+    // waits for them and accumulates per-family durations.
+    int tic();
+    int toc(int family, uint64_t bytes);
+    int collect();
+
+  private:
+    struct Pending
+    {
+        int family;
+        uint64_t bytes;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending_;
+    std::vector<hipEvent_t> pool_;
+    hipEvent_t cur_ = nullptr;
+    int take(hipEvent_t *e);
+};
+
+// Process-wide context for the kernel-level entry points.
+Ctx *default_ctx(int *rc);
+
+// ---- launchers (implemented in the .hip files) ----
+
+struct DiagGate
+{
+    uint64_t s0;
+    uint64_t s1; // ~0 for a P gate
+    double theta;
+};
+constexpr uint64_t kNoSel = ~uint64_t(0);
+
+// One diagonal sweep applying every gate of the window.
+int diag_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const DiagGate *gates, size_t ngates);
+
+struct U3Gate
+{
+    uint64_t packed_sel; // alpha | beta << 63
+    uint64_t delta;
+    double u[8];
+};
+
+// Applies commuting gates on distinct targets (a fused U3 cluster).
+int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates);
+// One general pair sweep (any delta / selector).
+int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
+
+int x_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t delta, uint64_t packed_sel);
+int cx_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t delta, uint64_t sel_cq, uint64_t sel_q);
+int cp_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t sel_cq, uint64_t sel_q, double theta);
+int mcx_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t ctrl_mask, unsigned target);
+
+// State helpers.
+int state_reset(Ctx &ctx, void *state, int prec, size_t alloc_elems);
+
+// Sampling in two steps: prepare builds the probability tree of 2^m
+// amplitudes and returns the total (the reference's local_prob); search
+// resolves sorted thresholds (host array) against it.
+int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, double *total);
+int sample_search(Ctx &ctx, const void *state, int prec, const double *thresholds, size_t shots, uint64_t *out_real);
+// Both steps with caller-given thresholds.
+int sample_apply(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, const double *thresholds,
+                 size_t shots, uint64_t *out_real);
+// Root of the pairwise |psi|^2 tree (the reference's total probability).
+int prob_total(Ctx &ctx, const void *state, int prec, unsigned m, double *out);
+
+// Host threshold stream (measure_common.cpp:14-71) from an already mixed seed.
+void threshold_stream(uint64_t stream_seed, size_t cnt, double begin, double end, double *out);
+uint64_t mix_seed(uint64_t base, uint64_t salt);
+
+} // namespace zxa

@@ -1,0 +1,683 @@
+// engine.cpp -- the ZXH simulator object over HIP passes.
+//
+// Host algorithm restated from the reference's src/zxh/src/core/zxh.cpp
+// (execute 312-553, calc_mem 648-681, solve_expand 701-719, expand_all
+// 721-729, apply_x/apply_cx 731-780, apply_u3 782-825, get_state 579-636),
+// src/zxh/src/core/bitmat.cpp (GF(2) solve) and
+// src/zxh/src/measure/measure_common.cpp (measure 73-92).  The affine map is
+// held as N row masks of at most 63 physical columns.
+#include "engine.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstring>
+#include <random>
+
+namespace zxa
+{
+
+namespace
+{
+constexpr double kPi = 3.14159265358979323846;
+// kU3BatchEps is the float literal 1e-5f widened to double (zxh.cpp:29).
+const double kU3Eps = static_cast<double>(1e-5f);
+std::atomic<bool> g_active{false};
+
+std::complex<double> phase(double a)
+{
+    return std::exp(std::complex<double>(0.0, a));
+}
+
+uint64_t low_mask(size_t nbits)
+{
+    if (nbits == 0)
+        return 0;
+    if (nbits >= 64)
+        return ~uint64_t(0);
+    return (uint64_t(1) << nbits) - 1;
+}
+
+uint64_t pack_sel(uint64_t alpha, bool beta)
+{
+    return (alpha & ((uint64_t(1) << 63) - 1)) | (beta ? (uint64_t(1) << 63) : 0);
+}
+
+// U3 matrix of the reference (zxh.cpp:101-113).
+void u3_matrix(double theta, double lambda, double phi, std::complex<double> u[4])
+{
+    const double c = std::cos(theta / 2.0);
+    const double s = std::sin(theta / 2.0);
+    const std::complex<double> el = phase(lambda), ep = phase(phi), epl = phase(phi + lambda);
+    u[0] = std::complex<double>(c, 0.0);
+    u[1] = -el * s;
+    u[2] = ep * s;
+    u[3] = epl * c;
+}
+
+bool close_val(std::complex<double> a, std::complex<double> b)
+{
+    return std::abs(a - b) <= kU3Eps;
+}
+
+// GF(2) solve A x = e_q by Gauss-Jordan on N x m rows (bitmat.cpp:46-122).
+// A always has full column rank, so the solution is unique when it exists.
+bool gf2_solve(const std::vector<uint64_t> &rows, size_t m, size_t q, uint64_t *x)
+{
+    const size_t N = rows.size();
+    if (m == 0)
+    {
+        *x = 0;
+        return false;
+    }
+    std::vector<uint64_t> r(rows);
+    std::vector<uint8_t> rhs(N, 0);
+    rhs[q] = 1;
+    int pivot_row[64];
+    for (size_t c = 0; c < m; c++)
+        pivot_row[c] = -1;
+    size_t row = 0;
+    for (size_t c = 0; c < m && row < N; c++)
+    {
+        size_t piv = row;
+        while (piv < N && !((r[piv] >> c) & 1u))
+            piv++;
+        if (piv == N)
+            continue;
+        std::swap(r[piv], r[row]);
+        std::swap(rhs[piv], rhs[row]);
+        for (size_t k = 0; k < N; k++)
+            if (k != row && ((r[k] >> c) & 1u))
+            {
+                r[k] ^= r[row];
+                rhs[k] ^= rhs[row];
+            }
+        pivot_row[c] = int(row);
+        row++;
+    }
+    for (size_t k = 0; k < N; k++)
+        if (r[k] == 0 && rhs[k])
+            return false;
+    uint64_t sol = 0;
+    for (size_t c = 0; c < m; c++)
+        if (pivot_row[c] >= 0 && rhs[size_t(pivot_row[c])])
+            sol |= uint64_t(1) << c;
+    *x = sol;
+    return true;
+}
+
+// The reference's selector-class rule for diagonal buckets
+// (kernel_cuda.cu:222-232): a CP whose selector mixes lane and slice bits is
+// not batched; it flushes the window and runs alone.
+int sel_class(uint64_t packed)
+{
+    const uint64_t a = packed & ((uint64_t(1) << 63) - 1);
+    if ((a >> 8) == 0)
+        return 0;
+    if ((a & 0xffu) == 0)
+        return 1;
+    return 2;
+}
+} // namespace
+
+bool runtime_active()
+{
+    return g_active.load();
+}
+void runtime_set_active(bool on)
+{
+    g_active.store(on);
+}
+
+Sim::Sim(size_t n, bool disable_x, bool eager, int prec)
+    : N_(n), disable_x_(disable_x), eager_(eager), prec_(prec), rows_(n, 0), b_(n, 0)
+{
+}
+
+Sim::~Sim()
+{
+    if (dstate_)
+    {
+        (void)hipStreamSynchronize(ctx_.stream);
+        (void)hipFree(dstate_);
+    }
+}
+
+int Sim::init()
+{
+    return ctx_.init();
+}
+
+// Recording rules of ZXH::Barrier..Rx (zxh.cpp:234-310).
+int Sim::add_gate(int code, size_t cq, size_t q, double theta, double lambda, double phi)
+{
+    auto bad = [](const char *m) {
+        set_error(m);
+        return int(EINVAL_);
+    };
+    switch (code)
+    {
+    case G_BARRIER:
+        gates_.push_back({G_BARRIER, 0, 0, 0.0, 0.0, 0.0});
+        return OK;
+    case G_RZ:
+    case G_P:
+    case G_Z:
+    case G_X:
+    case G_H:
+        if (q >= N_)
+            return bad("target qubit out of range");
+        gates_.push_back({code, 0, q, (code == G_RZ || code == G_P) ? theta : 0.0, 0.0, 0.0});
+        return OK;
+    case G_U3:
+        if (q >= N_)
+            return bad("U3 target qubit out of range");
+        gates_.push_back({G_U3, 0, q, theta, lambda, phi});
+        return OK;
+    case G_RX:
+        if (q >= N_)
+            return bad("Rx target qubit out of range");
+        gates_.push_back({G_U3, 0, q, theta, kPi / 2.0, -kPi / 2.0});
+        return OK;
+    case G_CRZ:
+        if (cq >= N_ || q >= N_)
+            return bad("CRz qubit out of range");
+        gates_.push_back({G_P, 0, cq, -0.5 * theta, 0.0, 0.0});
+        gates_.push_back({G_CP, cq, q, theta, 0.0, 0.0});
+        return OK;
+    case G_CP:
+        if (cq >= N_ || q >= N_)
+            return bad("CP qubit out of range");
+        gates_.push_back({G_CP, cq, q, theta, 0.0, 0.0});
+        return OK;
+    case G_CX:
+        if (cq >= N_ || q >= N_)
+            return bad("CX qubit out of range");
+        gates_.push_back({G_CX, cq, q, 0.0, 0.0, 0.0});
+        return OK;
+    default:
+        return bad("unknown gate code");
+    }
+}
+
+bool Sim::solve(size_t q, uint64_t *x) const
+{
+    return gf2_solve(rows_, m_, q, x);
+}
+
+// make_selector + make_local_selector with block_start = 0 (zxh.cpp:55-60, 683-691)
+uint64_t Sim::local_selector(size_t q) const
+{
+    return pack_sel(rows_[q] & low_mask(active_bits()), b_[q] != 0);
+}
+
+// calc_mem (zxh.cpp:648-681)
+size_t Sim::required_M() const
+{
+    std::vector<uint64_t> rows(N_, 0);
+    size_t m = 0;
+    for (const GateRec &g : gates_)
+    {
+        bool needs = false;
+        if (g.type == G_X || g.type == G_CX)
+        {
+            if (!disable_x_)
+            {
+                if (g.type == G_CX)
+                    rows[g.q] ^= rows[g.cq];
+            }
+            else
+                needs = true;
+        }
+        else if (g.type == G_H || g.type == G_U3)
+            needs = true;
+        if (needs)
+        {
+            uint64_t d;
+            if (!gf2_solve(rows, m, g.q, &d))
+            {
+                if (m >= 63)
+                    return 64; // beyond any state the engine can hold
+                rows[g.q] |= uint64_t(1) << m;
+                m++;
+            }
+        }
+    }
+    return m;
+}
+
+// sv_t::resize + reset (sv_common.cpp:286-328): the buffer only grows; the
+// first 2^M amplitudes are zeroed and psi[0] = 1.
+int Sim::resize_reset(size_t M)
+{
+    if (M > 40)
+    {
+        set_error("state needs more than 2^40 amplitudes");
+        return ENOMEM_;
+    }
+    const size_t cap = size_t(1) << M;
+    if (alloc_elems_ < cap)
+    {
+        if (dstate_)
+        {
+            ZXA_HIP(hipStreamSynchronize(ctx_.stream));
+            ZXA_HIP(hipFree(dstate_));
+            dstate_ = nullptr;
+            alloc_elems_ = 0;
+        }
+        hipError_t e = hipMalloc(&dstate_, cap * elem_bytes(prec_));
+        if (e != hipSuccess)
+        {
+            dstate_ = nullptr;
+            set_error(std::string("state allocation failed: ") + hipGetErrorString(e));
+            return ENOMEM_;
+        }
+        alloc_elems_ = cap;
+    }
+    M_ = M;
+    used_bits_ = 0;
+    return state_reset(ctx_, dstate_, prec_, cap);
+}
+
+int Sim::expand()
+{
+    if (used_bits_ >= M_)
+    {
+        set_error("expand exceeds max_bits");
+        return ESTATE;
+    }
+    used_bits_++;
+    return OK;
+}
+
+// solve_expand (zxh.cpp:701-719)
+uint64_t Sim::solve_expand(size_t k, int *rc)
+{
+    *rc = OK;
+    uint64_t delta = 0;
+    if (!solve(k, &delta))
+    {
+        *rc = expand();
+        if (*rc)
+            return 0;
+        rows_[k] |= uint64_t(1) << m_;
+        m_++;
+        delta = uint64_t(1) << (m_ - 1);
+    }
+    return delta;
+}
+
+int Sim::flush_diag(std::vector<DiagGate> &window)
+{
+    if (window.empty())
+        return OK;
+    int rc = diag_apply(ctx_, dstate_, prec_, active_bits(), window.data(), window.size());
+    window.clear();
+    return rc;
+}
+
+// flush_u3_batch (zxh.cpp:390-428): one descriptor per active accumulator,
+// in qubit order; the batch is applied as tile passes.
+int Sim::flush_u3(std::vector<Acc> &pend, size_t &count)
+{
+    if (count == 0)
+        return OK;
+    const unsigned lb = active_bits();
+    std::vector<U3Gate> descs;
+    for (size_t q = 0; q < N_; q++)
+    {
+        Acc &a = pend[q];
+        if (!a.active)
+            continue;
+        uint64_t d = 0;
+        if (!solve(q, &d))
+        {
+            set_error("flush_u3_batch requires no-expand U3 descriptors");
+            return ESTATE;
+        }
+        U3Gate g;
+        g.packed_sel = local_selector(q);
+        g.delta = d & low_mask(lb);
+        const std::complex<double> u[4] = {a.u00, a.u01, a.u10, a.u11};
+        for (int i = 0; i < 4; i++)
+        {
+            g.u[2 * i] = u[i].real();
+            g.u[2 * i + 1] = u[i].imag();
+        }
+        descs.push_back(g);
+    }
+    for (Acc &a : pend)
+        a = Acc();
+    count = 0;
+    if (descs.empty())
+        return OK;
+    return u3_apply(ctx_, dstate_, prec_, lb, descs.data(), descs.size());
+}
+
+int Sim::execute()
+{
+    executed_ = false;
+    result_count_ = 0;
+    const size_t target_M = eager_ ? N_ : required_M();
+    ZXA_TRY(resize_reset(target_M));
+    gphase_ = {1.0, 0.0};
+    std::fill(rows_.begin(), rows_.end(), 0);
+    std::fill(b_.begin(), b_.end(), 0);
+    m_ = 0;
+    if (eager_) // expand_all (zxh.cpp:721-729)
+    {
+        used_bits_ = N_;
+        for (size_t q = 0; q < N_; q++)
+            rows_[q] = uint64_t(1) << q;
+        m_ = N_;
+    }
+
+    std::vector<DiagGate> window;
+    std::vector<Acc> pend(N_);
+    size_t pend_count = 0;
+
+    size_t gi = 0;
+    while (gi < gates_.size())
+    {
+        const GateRec &g = gates_[gi];
+        if (g.type == G_H || g.type == G_U3)
+        {
+            // U3 cluster (zxh.cpp:440-470)
+            ZXA_TRY(flush_diag(window));
+            while (gi < gates_.size() && (gates_[gi].type == G_H || gates_[gi].type == G_U3))
+            {
+                const GateRec &ug = gates_[gi];
+                const double th = ug.type == G_H ? kPi / 2.0 : ug.theta;
+                const double la = ug.type == G_H ? kPi : ug.lambda;
+                const double ph = ug.type == G_H ? 0.0 : ug.phi;
+                std::complex<double> u[4];
+                u3_matrix(th, la, ph, u);
+                uint64_t delta = 0;
+                if (!solve(ug.q, &delta))
+                {
+                    // flush the batch, then the eager, expanding apply_u3
+                    ZXA_TRY(flush_u3(pend, pend_count));
+                    int rc = OK;
+                    const uint64_t d = solve_expand(ug.q, &rc);
+                    if (rc)
+                        return rc;
+                    U3Gate eg;
+                    eg.packed_sel = local_selector(ug.q);
+                    eg.delta = d & low_mask(active_bits());
+                    for (int i = 0; i < 4; i++)
+                    {
+                        eg.u[2 * i] = u[i].real();
+                        eg.u[2 * i + 1] = u[i].imag();
+                    }
+                    ZXA_TRY(u3_pair_apply(ctx_, dstate_, prec_, active_bits(), eg));
+                }
+                else
+                {
+                    // absorb_u3_gate + left_multiply_u3 (zxh.cpp:144-156, 363-388)
+                    Acc &a = pend[ug.q];
+                    const bool was = a.active;
+                    if (!was)
+                    {
+                        a = Acc();
+                        a.active = true;
+                    }
+                    const std::complex<double> o00 = a.u00, o01 = a.u01, o10 = a.u10, o11 = a.u11;
+                    a.u00 = u[0] * o00 + u[1] * o10;
+                    a.u01 = u[0] * o01 + u[1] * o11;
+                    a.u10 = u[2] * o00 + u[3] * o10;
+                    a.u11 = u[2] * o01 + u[3] * o11;
+                    a.active = !(close_val(a.u00, {1.0, 0.0}) && close_val(a.u01, {0.0, 0.0}) &&
+                                 close_val(a.u10, {0.0, 0.0}) && close_val(a.u11, {1.0, 0.0}));
+                    if (!was && a.active)
+                        pend_count++;
+                    else if (was && !a.active)
+                        pend_count--;
+                }
+                gi++;
+            }
+            ZXA_TRY(flush_u3(pend, pend_count));
+            continue;
+        }
+        switch (g.type)
+        {
+        case G_BARRIER:
+            ZXA_TRY(flush_diag(window));
+            break;
+        case G_Z:
+        case G_RZ: {
+            const double th = g.type == G_Z ? kPi : g.theta;
+            window.push_back({local_selector(g.q), kNoSel, th});
+            gphase_ *= phase(-0.5 * th);
+            break;
+        }
+        case G_P:
+            window.push_back({local_selector(g.q), kNoSel, g.theta});
+            break;
+        case G_X:
+            if (disable_x_)
+            {
+                ZXA_TRY(flush_diag(window));
+                int rc = OK;
+                const uint64_t d = solve_expand(g.q, &rc);
+                if (rc)
+                    return rc;
+                ZXA_TRY(x_apply(ctx_, dstate_, prec_, active_bits(), d & low_mask(active_bits()),
+                                local_selector(g.q)));
+            }
+            else
+                b_[g.q] ^= 1u;
+            break;
+        case G_CX:
+            if (disable_x_)
+            {
+                ZXA_TRY(flush_diag(window));
+                int rc = OK;
+                const uint64_t d = solve_expand(g.q, &rc);
+                if (rc)
+                    return rc;
+                ZXA_TRY(cx_apply(ctx_, dstate_, prec_, active_bits(), d & low_mask(active_bits()),
+                                 local_selector(g.cq), local_selector(g.q)));
+            }
+            else
+            {
+                rows_[g.q] ^= rows_[g.cq];
+                b_[g.q] = uint8_t(b_[g.q] != b_[g.cq]);
+            }
+            break;
+        case G_CP: {
+            const uint64_t scq = local_selector(g.cq), sq = local_selector(g.q);
+            if (sel_class(scq) == 2 || sel_class(sq) == 2)
+            {
+                ZXA_TRY(flush_diag(window));
+                ZXA_TRY(cp_apply(ctx_, dstate_, prec_, active_bits(), scq, sq, g.theta));
+            }
+            else
+                window.push_back({scq, sq, g.theta});
+            break;
+        }
+        default:
+            break;
+        }
+        gi++;
+    }
+    ZXA_TRY(flush_diag(window));
+    ZXA_HIP(hipStreamSynchronize(ctx_.stream));
+    executed_ = true;
+    return OK;
+}
+
+// measure (measure_common.cpp:73-92) with the GPU sampler.
+int Sim::measure(size_t cnt)
+{
+    if (!executed_)
+    {
+        set_error("measure requires execute() to be called first");
+        return ESTATE;
+    }
+    if (cnt == 0)
+    {
+        result_count_ = 0;
+        return OK;
+    }
+    const unsigned m = active_bits();
+    const unsigned I = unsigned(std::min<size_t>(M_ <= seg_min_i_ ? M_ : seg_min_i_, m));
+    double total = 0.0;
+    ZXA_TRY(sample_prepare(ctx_, dstate_, prec_, m, I, &total));
+    if (!(total > 0.0))
+    {
+        set_error("state has zero total probability");
+        return EZEROPROB;
+    }
+    const uint64_t base = use_seed_ ? seed_ : uint64_t(std::random_device{}());
+    const uint64_t stream_seed = mix_seed(base, 0);
+    std::vector<double> thr(cnt);
+    threshold_stream(stream_seed, cnt, 0.0, total, thr.data());
+    real_results_.resize(cnt);
+    ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr.data(), cnt, real_results_.data()));
+    // pack_virtual_results_host (measure_result_pack.h:24-44)
+    const size_t wc = (N_ + 63) / 64;
+    packed_.assign(cnt * wc, 0);
+    bool identity = (b_.end() == std::find(b_.begin(), b_.end(), uint8_t(1))) && N_ <= 64;
+    for (size_t q = 0; q < N_ && identity; q++)
+        identity = rows_[q] == (uint64_t(1) << q);
+    if (identity)
+    {
+        for (size_t i = 0; i < cnt; i++)
+            packed_[i] = real_results_[i];
+    }
+    else
+    {
+        for (size_t i = 0; i < cnt; i++)
+        {
+            const uint64_t r = real_results_[i];
+            uint64_t *dst = packed_.data() + i * wc;
+            for (size_t q = 0; q < N_; q++)
+                if (((__builtin_popcountll(r & rows_[q]) & 1) ^ b_[q]) != 0)
+                    dst[q >> 6] |= uint64_t(1) << (q & 63);
+        }
+    }
+    result_count_ = cnt;
+    return OK;
+}
+
+int Sim::get_results(uint64_t *packed, size_t cnt) const
+{
+    if (cnt > result_count_)
+    {
+        set_error("get_results count exceeds cached result count");
+        return EINVAL_;
+    }
+    const size_t wc = (N_ + 63) / 64;
+    std::memcpy(packed, packed_.data(), cnt * wc * sizeof(uint64_t));
+    return OK;
+}
+
+int Sim::get_real_results(uint64_t *real, size_t cnt) const
+{
+    if (cnt > result_count_)
+    {
+        set_error("get_results count exceeds cached result count");
+        return EINVAL_;
+    }
+    std::memcpy(real, real_results_.data(), cnt * sizeof(uint64_t));
+    return OK;
+}
+
+int Sim::get_phys_state(void *out, size_t out_bytes, size_t *m_out) const
+{
+    if (!executed_)
+    {
+        set_error("get_state requires execute() to be called first");
+        return ESTATE;
+    }
+    const size_t bytes = (size_t(1) << used_bits_) * elem_bytes(prec_);
+    if (out_bytes < bytes)
+    {
+        set_error("output buffer too small");
+        return EINVAL_;
+    }
+    ZXA_HIP(hipMemcpy(out, dstate_, bytes, hipMemcpyDeviceToHost));
+    *m_out = used_bits_;
+    return OK;
+}
+
+int Sim::set_phys_state(const void *in, size_t in_bytes)
+{
+    if (!executed_)
+    {
+        set_error("set_phys_state requires execute() to be called first");
+        return ESTATE;
+    }
+    const size_t bytes = (size_t(1) << used_bits_) * elem_bytes(prec_);
+    if (in_bytes != bytes)
+    {
+        set_error("input size does not match the physical state");
+        return EINVAL_;
+    }
+    ZXA_HIP(hipMemcpy(dstate_, in, bytes, hipMemcpyHostToDevice));
+    result_count_ = 0;
+    return OK;
+}
+
+int Sim::device_ptr(void **p, size_t *m) const
+{
+    if (!executed_)
+    {
+        set_error("no state before execute()");
+        return ESTATE;
+    }
+    *p = dstate_;
+    *m = used_bits_;
+    return OK;
+}
+
+// get_state (zxh.cpp:579-636): virt = A*real ^ b, stored at the bit-reversed
+// index, times the global phase.
+int Sim::get_state(double *out, size_t out_len) const
+{
+    if (!executed_)
+    {
+        set_error("get_state requires execute() to be called first");
+        return ESTATE;
+    }
+    if (N_ >= 40)
+    {
+        set_error("get_state full state export requires fewer than 40 qubits");
+        return EINVAL_;
+    }
+    const size_t vsize = size_t(1) << N_;
+    if (out_len < 2 * vsize)
+    {
+        set_error("output buffer too small");
+        return EINVAL_;
+    }
+    const size_t real_size = size_t(1) << m_;
+    std::vector<std::complex<double>> real_state(real_size);
+    if (prec_ == C128)
+    {
+        ZXA_HIP(hipMemcpy(real_state.data(), dstate_, real_size * 16, hipMemcpyDeviceToHost));
+    }
+    else
+    {
+        std::vector<std::complex<float>> tmp(real_size);
+        ZXA_HIP(hipMemcpy(tmp.data(), dstate_, real_size * 8, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < real_size; i++)
+            real_state[i] = std::complex<double>(tmp[i].real(), tmp[i].imag());
+    }
+    std::fill(out, out + 2 * vsize, 0.0);
+    for (size_t real = 0; real < real_size; real++)
+    {
+        uint64_t virt = 0;
+        for (size_t q = 0; q < N_; q++)
+            virt |= uint64_t((__builtin_popcountll(real & rows_[q]) & 1) ^ b_[q]) << q;
+        uint64_t rev = 0;
+        for (size_t i = 0; i < N_; i++)
+            rev = (rev << 1) | ((virt >> i) & 1u);
+        const std::complex<double> v = real_state[real] * gphase_;
+        out[2 * rev] = v.real();
+        out[2 * rev + 1] = v.imag();
+    }
+    return OK;
+}
+
+} // namespace zxa

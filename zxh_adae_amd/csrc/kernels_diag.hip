@@ -1,0 +1,496 @@
+// kernels_diag.hip -- the diagonal-window sweep (P / CP / Rz / Z batches).
+//
+// Replaces diag_block_kernel_cuda and the CPU apply_diag_window (reference
+// src/zxh/src/kernel/kernel_cuda.cu:505-614, kernel_general.cpp:72-113): every
+// amplitude x is multiplied by exp(i*theta(x)) with
+//     theta(x) = sum_g theta_g * [S0_g(x)] * [S1_g(x)],
+//     S(x) = parity(x & alpha) ^ beta          (S1 == 1 for a P gate),
+// in one read-modify-write sweep over the state.
+//
+// Fast path (every selector has at most one alpha bit, the plain method and
+// most ZXH windows): theta(x) is then a polynomial of degree <= 2 in the
+// address bits,
+//     theta(x) = K + sum_j L_j x_j + sum_{j<k} Q_jk x_j x_k .
+// The address splits into lane bits a (16 contiguous bytes per lane, 64
+// lanes), row bits w (64 rows of 1 KiB per chunk) and chunk bits h.  Then
+//     theta = lane(a; h) + rowlo(a; w_lo) + row(w; h)
+// where lane() is per lane and chunk, row() is uniform per row of the chunk,
+// and rowlo() is linear in the four row bits a wave iterates over.  Each
+// factor becomes a unit complex number computed once per lane or row, so an
+// amplitude costs three complex products (no sincos per element), and the
+// sweep stays HBM-bound at any batch size.
+//
+// General path (multi-bit selectors after CX transport, or states narrower
+// than one chunk): per-amplitude evaluation of every gate, like the reference.
+#include "dev.h"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace zxa
+{
+namespace
+{
+
+constexpr int kThreads = 256;
+constexpr int kRowBits = 6;   // 64 rows per chunk
+constexpr int kRowsPerWave = 16;
+constexpr int kCoefStride = 16; // doubles per chunk in the coefficient array
+
+struct cd
+{
+    double x, y;
+};
+struct cf
+{
+    float x, y;
+};
+
+__device__ __forceinline__ cd cmul(cd a, cd b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cf cmulf(cf a, cf b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cd cis(double a)
+{
+    double s, c;
+    sincos(a, &s, &c);
+    return {c, s};
+}
+__device__ __forceinline__ cf tof(cd a)
+{
+    return {static_cast<float>(a.x), static_cast<float>(a.y)};
+}
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffu), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                static_cast<unsigned>(lo));
+}
+
+template <bool kC64>
+struct Geo
+{
+    static constexpr int EPL = kC64 ? 2 : 1;     // amplitudes per lane (16 bytes)
+    static constexpr int LB = 6 + (kC64 ? 1 : 0); // lane-element bits
+    static constexpr int CB = LB + kRowBits;      // chunk bits
+    static constexpr int NCO = LB + kRowBits;     // per-chunk linear coefficients
+};
+
+// Per-chunk coefficients: c_j(h) = sum_{k in h} Q_jk h_k for every bit j of
+// a and w, and C(h) = K + sum_{k in h} h_k (L_k + sum_{l>k} Q_kl h_l).
+__global__ __launch_bounds__(kThreads) void diag_chunk_coef_kernel(const double *__restrict__ cq,
+                                                                   const double *__restrict__ hL,
+                                                                   const double *__restrict__ hQ, double K, int nco,
+                                                                   int nh, uint64_t nchunks, double *__restrict__ out)
+{
+    const uint64_t c = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (c >= nchunks)
+        return;
+    double *o = out + c * kCoefStride;
+    for (int j = 0; j < nco; j++)
+    {
+        double s = 0.0;
+        for (int i = 0; i < nh; i++)
+            if ((c >> i) & 1u)
+                s += cq[j * nh + i];
+        o[j] = s;
+    }
+    double C = K;
+    for (int i = 0; i < nh; i++)
+    {
+        if (!((c >> i) & 1u))
+            continue;
+        double t = hL[i];
+        for (int l = i + 1; l < nh; l++)
+            if ((c >> l) & 1u)
+                t += hQ[i * nh + l];
+        C += t;
+    }
+    o[nco] = C;
+}
+
+// tabs: lane_tab[2^LB] | dco[2^LB * 6] | row_tab[64]
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void diag_poly_kernel(void *__restrict__ state, const double *__restrict__ tabs,
+                                                             const double *__restrict__ coef, uint64_t nchunks)
+{
+    using G = Geo<kC64>;
+    constexpr int EPL = G::EPL;
+    constexpr int LB = G::LB;
+    const double *lane_tab = tabs;
+    const double *dco = tabs + (1 << LB);
+    const double *row_tab = dco + (1 << LB) * kRowBits;
+
+    const int lane = threadIdx.x & 63;
+    const int v = threadIdx.x >> 6;
+
+    // chunk-independent per-lane factors
+    double aw[EPL];
+    cd glo[EPL][4], ghi[EPL][4];
+#pragma unroll
+    for (int e = 0; e < EPL; e++)
+    {
+        const int xa = lane * EPL + e;
+        const double *d = dco + xa * kRowBits;
+        aw[e] = lane_tab[xa] + ((v & 1) ? d[4] : 0.0) + ((v & 2) ? d[5] : 0.0);
+        glo[e][0] = {1.0, 0.0};
+        glo[e][1] = cis(d[0]);
+        glo[e][2] = cis(d[1]);
+        glo[e][3] = cis(d[0] + d[1]);
+        ghi[e][0] = {1.0, 0.0};
+        ghi[e][1] = cis(d[2]);
+        ghi[e][2] = cis(d[3]);
+        ghi[e][3] = cis(d[2] + d[3]);
+    }
+    const int xw_own = v * kRowsPerWave + (lane & (kRowsPerWave - 1));
+    const double brow = row_tab[xw_own];
+
+    for (uint64_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x)
+    {
+        const size_t base = static_cast<size_t>(chunk) << G::CB;
+        const double *cc = coef + chunk * kCoefStride;
+
+        if constexpr (!kC64)
+        {
+            double2 *p = static_cast<double2 *>(state);
+            double2 data[kRowsPerWave];
+#pragma unroll
+            for (int t = 0; t < kRowsPerWave; t++)
+                data[t] = p[base + (static_cast<size_t>(v * kRowsPerWave + t) << LB) + lane];
+
+            double ang = aw[0];
+#pragma unroll
+            for (int j = 0; j < LB; j++)
+                if ((lane >> j) & 1)
+                    ang += cc[j];
+            const cd E = cis(ang);
+            double rang = brow + cc[G::NCO];
+#pragma unroll
+            for (int k = 0; k < kRowBits; k++)
+                if ((xw_own >> k) & 1)
+                    rang += cc[LB + k];
+            const cd R = cis(rang);
+            cd q[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                q[u] = cmul(E, ghi[0][u]);
+#pragma unroll
+            for (int t = 0; t < kRowsPerWave; t++)
+            {
+                const cd rt = {readlane_d(R.x, t), readlane_d(R.y, t)};
+                const cd ph = cmul(cmul(q[t >> 2], glo[0][t & 3]), rt);
+                const cd a = cmul({data[t].x, data[t].y}, ph);
+                data[t].x = a.x;
+                data[t].y = a.y;
+            }
+#pragma unroll
+            for (int t = 0; t < kRowsPerWave; t++)
+                p[base + (static_cast<size_t>(v * kRowsPerWave + t) << LB) + lane] = data[t];
+        }
+        else
+        {
+            float4 *p = static_cast<float4 *>(state);
+            const size_t base4 = base >> 1;
+            float4 data[kRowsPerWave];
+#pragma unroll
+            for (int t = 0; t < kRowsPerWave; t++)
+                data[t] = p[base4 + (static_cast<size_t>(v * kRowsPerWave + t) << (LB - 1)) + lane];
+
+            cf q[EPL][4];
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+            {
+                const int xa = lane * EPL + e;
+                double ang = aw[e];
+#pragma unroll
+                for (int j = 0; j < LB; j++)
+                    if ((xa >> j) & 1)
+                        ang += cc[j];
+                const cd E = cis(ang);
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    q[e][u] = tof(cmul(E, ghi[e][u]));
+            }
+            double rang = brow + cc[G::NCO];
+#pragma unroll
+            for (int k = 0; k < kRowBits; k++)
+                if ((xw_own >> k) & 1)
+                    rang += cc[LB + k];
+            const cd R = cis(rang);
+            const float rx = static_cast<float>(R.x), ry = static_cast<float>(R.y);
+            cf gl[EPL][4];
+#pragma unroll
+            for (int e = 0; e < EPL; e++)
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    gl[e][u] = tof(glo[e][u]);
+#pragma unroll
+            for (int t = 0; t < kRowsPerWave; t++)
+            {
+                const cf rt = {__int_as_float(__builtin_amdgcn_readlane(__float_as_int(rx), t)),
+                               __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ry), t))};
+                const cf ph0 = cmulf(cmulf(q[0][t >> 2], gl[0][t & 3]), rt);
+                const cf ph1 = cmulf(cmulf(q[1][t >> 2], gl[1][t & 3]), rt);
+                const cf a0 = cmulf({data[t].x, data[t].y}, ph0);
+                const cf a1 = cmulf({data[t].z, data[t].w}, ph1);
+                data[t] = make_float4(a0.x, a0.y, a1.x, a1.y);
+            }
+#pragma unroll
+            for (int t = 0; t < kRowsPerWave; t++)
+                p[base4 + (static_cast<size_t>(v * kRowsPerWave + t) << (LB - 1)) + lane] = data[t];
+        }
+    }
+}
+
+__device__ __forceinline__ bool sel_eval(uint64_t packed, uint64_t x)
+{
+    const uint64_t alpha = packed & ((uint64_t(1) << 63) - 1);
+    return ((__popcll(x & alpha) & 1u) ^ static_cast<unsigned>(packed >> 63)) != 0u;
+}
+
+// General path: theta(x) accumulated over every gate of the window.
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void diag_general_kernel(void *__restrict__ state,
+                                                                const DiagGate *__restrict__ gates, int ngates,
+                                                                uint64_t nelems)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t x = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; x < nelems; x += stride)
+    {
+        double th = 0.0;
+        for (int g = 0; g < ngates; g++)
+        {
+            const DiagGate gg = gates[g];
+            if (sel_eval(gg.s0, x) && (gg.s1 == kNoSel || sel_eval(gg.s1, x)))
+                th += gg.theta;
+        }
+        if (th == 0.0)
+            continue;
+        const cd ph = cis(th);
+        if constexpr (!kC64)
+        {
+            double2 *p = static_cast<double2 *>(state);
+            const double2 a = p[x];
+            const cd r = cmul({a.x, a.y}, ph);
+            p[x] = make_double2(r.x, r.y);
+        }
+        else
+        {
+            float2 *p = static_cast<float2 *>(state);
+            const float2 a = p[x];
+            const cf r = cmulf({a.x, a.y}, tof(ph));
+            p[x] = make_float2(r.x, r.y);
+        }
+    }
+}
+
+template <typename K>
+int persistent_grid(Ctx &ctx, K kernel, uint64_t work_items)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 2;
+    const uint64_t cap = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(ctx.num_cus > 0 ? ctx.num_cus : 256);
+    return static_cast<int>(work_items < cap ? work_items : cap);
+}
+
+// Adds theta * [S0][S1] to the polynomial (K, L, Q) for selectors with at
+// most one alpha bit: [x_j ^ beta] = beta + (1 - 2 beta) x_j.
+struct Lin
+{
+    int bit;  // -1: constant
+    double c; // constant part
+    double s; // slope of x_bit
+};
+Lin lin_of(uint64_t packed)
+{
+    const uint64_t alpha = packed & ((uint64_t(1) << 63) - 1);
+    const bool beta = (packed >> 63) & 1u;
+    if (alpha == 0)
+        return {-1, beta ? 1.0 : 0.0, 0.0};
+    return {__builtin_ctzll(alpha), beta ? 1.0 : 0.0, beta ? -1.0 : 1.0};
+}
+
+int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
+{
+    const bool c64 = prec == C64;
+    const int LB = c64 ? Geo<true>::LB : Geo<false>::LB;
+    const int NCO = LB + kRowBits;
+    const int nh = static_cast<int>(n) - NCO;
+    std::vector<double> L(n, 0.0), Q(size_t(n) * n, 0.0);
+    double K = 0.0;
+    for (size_t i = 0; i < ngates; i++)
+    {
+        const DiagGate &g = gates[i];
+        const Lin a = lin_of(g.s0);
+        const Lin b = g.s1 == kNoSel ? Lin{-1, 1.0, 0.0} : lin_of(g.s1);
+        const double th = g.theta;
+        K += th * a.c * b.c;
+        if (b.bit >= 0)
+            L[b.bit] += th * a.c * b.s;
+        if (a.bit >= 0)
+            L[a.bit] += th * b.c * a.s;
+        if (a.bit >= 0 && b.bit >= 0)
+        {
+            const double q = th * a.s * b.s;
+            if (a.bit == b.bit)
+                L[a.bit] += q;
+            else
+                Q[size_t(std::min(a.bit, b.bit)) * n + std::max(a.bit, b.bit)] += q;
+        }
+    }
+    auto q_at = [&](int j, int k) { return Q[size_t(j) * n + k]; };
+
+    const int nlane = 1 << LB;
+    std::vector<double> blk;
+    blk.reserve(nlane * (1 + kRowBits) + 64 + NCO * nh + nh + nh * nh + 1);
+    // lane_tab
+    for (int xa = 0; xa < nlane; xa++)
+    {
+        double s = 0.0;
+        for (int j = 0; j < LB; j++)
+        {
+            if (!((xa >> j) & 1))
+                continue;
+            s += L[j];
+            for (int k = j + 1; k < LB; k++)
+                if ((xa >> k) & 1)
+                    s += q_at(j, k);
+        }
+        blk.push_back(s);
+    }
+    // dco
+    for (int xa = 0; xa < nlane; xa++)
+        for (int k = 0; k < kRowBits; k++)
+        {
+            double s = 0.0;
+            for (int j = 0; j < LB; j++)
+                if ((xa >> j) & 1)
+                    s += q_at(j, LB + k);
+            blk.push_back(s);
+        }
+    // row_tab
+    for (int xw = 0; xw < (1 << kRowBits); xw++)
+    {
+        double s = 0.0;
+        for (int k = 0; k < kRowBits; k++)
+        {
+            if (!((xw >> k) & 1))
+                continue;
+            s += L[LB + k];
+            for (int l = k + 1; l < kRowBits; l++)
+                if ((xw >> l) & 1)
+                    s += q_at(LB + k, LB + l);
+        }
+        blk.push_back(s);
+    }
+    const size_t tabs_len = blk.size();
+    // cq, hL, hQ
+    for (int j = 0; j < NCO; j++)
+        for (int i = 0; i < nh; i++)
+            blk.push_back(q_at(j, NCO + i));
+    for (int i = 0; i < nh; i++)
+        blk.push_back(L[NCO + i]);
+    for (int i = 0; i < nh; i++)
+        for (int l = 0; l < nh; l++)
+            blk.push_back(l > i ? q_at(NCO + i, NCO + l) : 0.0);
+
+    void *dblk = nullptr;
+    ZXA_TRY(ctx.ring.stage(blk.data(), blk.size() * sizeof(double), ctx.stream, &dblk));
+    const double *dtabs = static_cast<const double *>(dblk);
+    const double *dcq = dtabs + tabs_len;
+    const double *dhL = dcq + size_t(NCO) * nh;
+    const double *dhQ = dhL + nh;
+
+    const uint64_t nchunks = uint64_t(1) << nh;
+    ZXA_TRY(ctx.scratch_a.ensure(nchunks * kCoefStride * sizeof(double)));
+    double *coef = static_cast<double *>(ctx.scratch_a.ptr);
+    {
+        const unsigned grid = static_cast<unsigned>((nchunks + kThreads - 1) / kThreads);
+        hipLaunchKernelGGL(diag_chunk_coef_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, dcq, dhL, dhQ, K, NCO,
+                           nh, nchunks, coef);
+        ZXA_HIP(hipGetLastError());
+    }
+    const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
+    ZXA_TRY(ctx.tic());
+    if (c64)
+    {
+        static int grid_cap = -1;
+        const int grid = persistent_grid(ctx, diag_poly_kernel<true>, nchunks);
+        (void)grid_cap;
+        hipLaunchKernelGGL(diag_poly_kernel<true>, dim3(grid), dim3(kThreads), 0, ctx.stream, state, dtabs,
+                           static_cast<const double *>(coef), nchunks);
+    }
+    else
+    {
+        const int grid = persistent_grid(ctx, diag_poly_kernel<false>, nchunks);
+        hipLaunchKernelGGL(diag_poly_kernel<false>, dim3(grid), dim3(kThreads), 0, ctx.stream, state, dtabs,
+                           static_cast<const double *>(coef), nchunks);
+    }
+    ZXA_HIP(hipGetLastError());
+    ZXA_TRY(ctx.toc(F_DIAG, bytes));
+    return ctx.ring.done(ctx.stream);
+}
+
+int diag_general(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
+{
+    void *dg = nullptr;
+    ZXA_TRY(ctx.ring.stage(gates, ngates * sizeof(DiagGate), ctx.stream, &dg));
+    const uint64_t nelems = uint64_t(1) << n;
+    const uint64_t blocks = (nelems + kThreads - 1) / kThreads;
+    const unsigned grid = static_cast<unsigned>(blocks < 65536 ? blocks : 65536);
+    const uint64_t bytes = 2ull * nelems * elem_bytes(prec);
+    ZXA_TRY(ctx.tic());
+    if (prec == C64)
+        hipLaunchKernelGGL(diag_general_kernel<true>, dim3(grid), dim3(kThreads), 0, ctx.stream, state,
+                           static_cast<const DiagGate *>(dg), static_cast<int>(ngates), nelems);
+    else
+        hipLaunchKernelGGL(diag_general_kernel<false>, dim3(grid), dim3(kThreads), 0, ctx.stream, state,
+                           static_cast<const DiagGate *>(dg), static_cast<int>(ngates), nelems);
+    ZXA_HIP(hipGetLastError());
+    ZXA_TRY(ctx.toc(F_DIAG, bytes));
+    return ctx.ring.done(ctx.stream);
+}
+
+} // namespace
+
+int diag_apply(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
+{
+    if (ngates == 0)
+        return OK;
+    if (n > 40)
+    {
+        set_error("diag_apply: state wider than 40 bits");
+        return EINVAL_;
+    }
+    bool simple = true;
+    for (size_t i = 0; i < ngates && simple; i++)
+    {
+        const uint64_t a0 = gates[i].s0 & ((uint64_t(1) << 63) - 1);
+        if (__builtin_popcountll(a0) > 1 || (a0 >> n) != 0)
+            simple = false;
+        if (gates[i].s1 != kNoSel)
+        {
+            const uint64_t a1 = gates[i].s1 & ((uint64_t(1) << 63) - 1);
+            if (__builtin_popcountll(a1) > 1 || (a1 >> n) != 0)
+                simple = false;
+        }
+    }
+    const unsigned need = (prec == C64 ? Geo<true>::CB : Geo<false>::CB);
+    if (simple && n >= need)
+        return diag_fast(ctx, state, prec, n, gates, ngates);
+    // ring slot holds at most 256 KiB of gates; split very large windows
+    const size_t per = ParamRing::kSlotBytes / sizeof(DiagGate);
+    for (size_t off = 0; off < ngates; off += per)
+        ZXA_TRY(diag_general(ctx, state, prec, n, gates + off, std::min(per, ngates - off)));
+    return OK;
+}
+
+} // namespace zxa

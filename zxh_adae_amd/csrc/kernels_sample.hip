@@ -1,0 +1,374 @@
+// kernels_sample.hip -- multinomial sampling, bit-exact with the reference's
+// streaming CPU sampler (reference src/zxh/src/measure/measure_general.cpp,
+// measure_streaming_impl.h, measure_common.cpp).
+//
+// The reference builds, per segment of 2^I amplitudes, a CDF from a Blelloch
+// exclusive scan of p_i = re^2 + im^2, offsets it by a binary-counter prefix
+// of earlier segment totals, and resolves sorted thresholds with
+// std::upper_bound.  Every value it compares is a node sum of ONE balanced
+// binary tree over p (Blelloch up-sweep nodes, merge_prob_sum_range, and the
+// binary counter all add left+right children of aligned power-of-two blocks),
+// combined left to right in Fenwick order:
+//     excl_j   = (((0 + N_{l1}[.]) + N_{l2}[.]) + ...)   set bits of j, high to low
+//     prefix_s = same over segment indices, with levels >= I
+//     cdf_j    = (prefix_s + excl_j) + p_j,   tail_s = prefix_s + N_I[s]
+// So the GPU stores the tree levels >= LS (LS = min(4, I); lower levels are
+// recomputed from the 16 amplitudes of a block), and each shot performs
+// libstdc++'s exact upper_bound probe sequence, evaluating cdf at each probe.
+// A threshold belongs to the first segment whose running-max tail exceeds
+// it, which is where the reference's sequential segment walk places it.
+// No fused multiply-add is allowed anywhere on this path.
+#include "dev.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#pragma clang fp contract(off)
+
+namespace zxa
+{
+namespace
+{
+
+constexpr int kThreads = 256;
+constexpr int kSpanBits = 12; // input values per workgroup in the tree kernel
+constexpr int kMaxLevels = 64;
+
+struct Levels
+{
+    uint64_t off[kMaxLevels]; // offset of level l in the tree array (valid for LS <= l <= m)
+};
+
+__device__ __forceinline__ double p_of(const void *state, bool c64, uint64_t i)
+{
+    if (c64)
+    {
+        const float2 a = static_cast<const float2 *>(state)[i];
+        const double x = a.x, y = a.y;
+        return __dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y));
+    }
+    const double2 a = static_cast<const double2 *>(state)[i];
+    return __dadd_rn(__dmul_rn(a.x, a.x), __dmul_rn(a.y, a.y));
+}
+
+// One workgroup reduces up to 2^kSpanBits input values of level lin into the
+// levels lin+1 .. lin+span_bits and stores every level in [LS, m] it owns.
+// src_kind: 0 = c128 amplitudes, 1 = c64 amplitudes, 2 = tree nodes (f64).
+__global__ __launch_bounds__(kThreads) void tree_kernel(const void *__restrict__ src, int src_kind, int lin,
+                                                        int span_bits, int LS, int m, double *__restrict__ tree,
+                                                        const Levels lv)
+{
+    __shared__ double buf[1 << kSpanBits];
+    const int span = 1 << span_bits;
+    const uint64_t g0 = static_cast<uint64_t>(blockIdx.x) << span_bits;
+    for (int i = threadIdx.x; i < span; i += kThreads)
+    {
+        double v;
+        if (src_kind == 2)
+            v = static_cast<const double *>(src)[lv.off[lin] + g0 + i];
+        else
+            v = p_of(src, src_kind == 1, g0 + i);
+        buf[i] = v;
+        if (src_kind != 2 && lin >= LS)
+            tree[lv.off[lin] + g0 + i] = v;
+    }
+    __syncthreads();
+    for (int r = 1; r <= span_bits; r++)
+    {
+        const int cnt = span >> r;
+        const int l = lin + r;
+        double vals[(1 << kSpanBits) / kThreads / 2 > 0 ? (1 << kSpanBits) / kThreads / 2 : 1];
+        int nv = 0;
+        for (int k = threadIdx.x; k < cnt; k += kThreads)
+            vals[nv++] = __dadd_rn(buf[2 * k], buf[2 * k + 1]);
+        __syncthreads();
+        nv = 0;
+        for (int k = threadIdx.x; k < cnt; k += kThreads)
+        {
+            const double v = vals[nv++];
+            buf[k] = v;
+            if (l >= LS && l <= m)
+                tree[lv.off[l] + (g0 >> r) + k] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// prefix_s (Fenwick over levels >= I) and tail_s = prefix_s + N_I[s].
+__global__ __launch_bounds__(kThreads) void tails_kernel(const double *__restrict__ tree, const Levels lv, int I, int m,
+                                                         uint64_t nseg, double *__restrict__ prefix,
+                                                         double *__restrict__ tail)
+{
+    const uint64_t s = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (s >= nseg)
+        return;
+    double acc = 0.0;
+    for (int l = m - I - 1; l >= 0; l--)
+        if ((s >> l) & 1u)
+            acc = __dadd_rn(acc, tree[lv.off[I + l] + ((s >> l) - 1)]);
+    prefix[s] = acc;
+    tail[s] = __dadd_rn(acc, tree[lv.off[I] + s]);
+}
+
+// In-place running maximum over nseg values (one workgroup; max is exact).
+__global__ __launch_bounds__(kThreads) void cummax_kernel(double *__restrict__ v, uint64_t n)
+{
+    __shared__ double part[kThreads];
+    const uint64_t per = (n + kThreads - 1) / kThreads;
+    const uint64_t b = threadIdx.x * per;
+    const uint64_t e = b + per < n ? b + per : n;
+    double mx = -1.0;
+    for (uint64_t i = b; i < e; i++)
+        mx = fmax(mx, v[i]);
+    part[threadIdx.x] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double run = -1.0;
+        for (int t = 0; t < kThreads; t++)
+        {
+            const double cur = part[t];
+            part[t] = run; // exclusive running max
+            run = fmax(run, cur);
+        }
+    }
+    __syncthreads();
+    double run = part[threadIdx.x];
+    for (uint64_t i = b; i < e; i++)
+    {
+        run = fmax(run, v[i]);
+        v[i] = run;
+    }
+}
+
+template <int N>
+__device__ __forceinline__ double pick(const double (&a)[N], int idx)
+{
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        r = (i == idx) ? a[i] : r;
+    return r;
+}
+
+// One shot per thread.
+__global__ __launch_bounds__(kThreads) void search_kernel(const void *__restrict__ state, int c64,
+                                                          const double *__restrict__ tree, const Levels lv, int I,
+                                                          int LS, const double *__restrict__ cmax,
+                                                          const double *__restrict__ prefix, uint64_t nseg,
+                                                          const double *__restrict__ thr, uint64_t shots,
+                                                          uint64_t *__restrict__ out, int *__restrict__ err)
+{
+    const uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (k >= shots)
+        return;
+    const double tau = thr[k];
+    // first segment whose running-max tail exceeds tau
+    uint64_t lo = 0, len = nseg;
+    while (len > 0)
+    {
+        const uint64_t half = len >> 1;
+        if (tau < cmax[lo + half])
+            len = half;
+        else
+        {
+            lo = lo + half + 1;
+            len = len - half - 1;
+        }
+    }
+    if (lo >= nseg)
+    {
+        *err = 1;
+        out[k] = 0;
+        return;
+    }
+    const uint64_t seg = lo;
+    const uint64_t sbase = seg << I;
+    const double head = prefix[seg];
+    const int LB = LS < I ? LS : I; // low levels recomputed from amplitudes
+    const uint64_t valid = uint64_t(1) << I;
+
+    // libstdc++ upper_bound probe sequence over cdf[0, valid)
+    uint64_t first = 0;
+    len = valid;
+    while (len > 0)
+    {
+        const uint64_t half = len >> 1;
+        const uint64_t mid = first + half;
+        const uint64_t g = sbase + mid;
+        double acc = 0.0;
+        for (int l = I - 1; l >= LB; l--)
+            if ((mid >> l) & 1u)
+                acc = __dadd_rn(acc, tree[lv.off[l] + ((g >> l) - 1)]);
+        // levels below LB: balanced sums inside the 2^LB-amplitude block of g
+        const uint64_t gb = g & ~((uint64_t(1) << LB) - 1);
+        const int nb = 1 << LB;
+        double lv0[16], lv1[8], lv2[4], lv3[2];
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            lv0[i] = i < nb ? p_of(state, c64 != 0, gb + i) : 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            lv1[i] = __dadd_rn(lv0[2 * i], lv0[2 * i + 1]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            lv2[i] = __dadd_rn(lv1[2 * i], lv1[2 * i + 1]);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            lv3[i] = __dadd_rn(lv2[2 * i], lv2[2 * i + 1]);
+        const int off = static_cast<int>(g - gb);
+        if (off & 8)
+            acc = __dadd_rn(acc, lv3[0]);
+        if (off & 4)
+            acc = __dadd_rn(acc, pick(lv2, (off >> 2) - 1));
+        if (off & 2)
+            acc = __dadd_rn(acc, pick(lv1, (off >> 1) - 1));
+        if (off & 1)
+            acc = __dadd_rn(acc, pick(lv0, off - 1));
+        const double pmid = pick(lv0, off);
+        const double cdf = __dadd_rn(__dadd_rn(head, acc), pmid);
+        if (tau < cdf)
+            len = half;
+        else
+        {
+            first = mid + 1;
+            len = len - half - 1;
+        }
+    }
+    out[k] = sbase + (first == valid ? valid - 1 : first);
+}
+
+int build_tree(Ctx &ctx, const void *state, int prec, unsigned m, unsigned LS, Levels &lv, double **tree_out)
+{
+    uint64_t total = 0;
+    for (unsigned l = 0; l < kMaxLevels; l++)
+        lv.off[l] = 0;
+    for (unsigned l = LS; l <= m; l++)
+    {
+        lv.off[l] = total;
+        total += uint64_t(1) << (m - l);
+    }
+    ZXA_TRY(ctx.scratch_b.ensure(total * sizeof(double)));
+    double *tree = static_cast<double *>(ctx.scratch_b.ptr);
+    // level 0 -> up to kSpanBits, then from level kSpanBits, ...
+    int lin = 0;
+    bool first = true;
+    while (first || lin < int(m))
+    {
+        const int span_bits = std::min<int>(kSpanBits, int(m) - lin);
+        const uint64_t groups = uint64_t(1) << (int(m) - lin - span_bits);
+        const int kind = first ? (prec == C64 ? 1 : 0) : 2;
+        const void *src = first ? state : static_cast<const void *>(tree);
+        // a non-first pass reads level lin, which must be stored
+        if (!first && unsigned(lin) < LS)
+        {
+            set_error("sampler tree: unstored intermediate level");
+            return EINVAL_;
+        }
+        hipLaunchKernelGGL(tree_kernel, dim3(static_cast<unsigned>(groups)), dim3(kThreads), 0, ctx.stream, src, kind,
+                           lin, span_bits, int(LS), int(m), tree, lv);
+        ZXA_HIP(hipGetLastError());
+        first = false;
+        lin += span_bits;
+        if (span_bits == 0)
+            break;
+    }
+    *tree_out = tree;
+    return OK;
+}
+
+} // namespace
+
+int prob_total(Ctx &ctx, const void *state, int prec, unsigned m, double *out)
+{
+    Levels lv;
+    double *tree = nullptr;
+    const unsigned LS = std::min(4u, m);
+    ZXA_TRY(build_tree(ctx, state, prec, m, LS, lv, &tree));
+    ZXA_HIP(hipMemcpyAsync(out, tree + lv.off[m], sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    ZXA_HIP(hipStreamSynchronize(ctx.stream));
+    return OK;
+}
+
+int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, double *total)
+{
+    if (m > 40)
+    {
+        set_error("sampler: state wider than 40 bits");
+        return EINVAL_;
+    }
+    const unsigned I = std::min(m, seg_bits);
+    const unsigned LS = std::min(4u, I);
+    Levels &lv = ctx.sample_levels_as<Levels>();
+    double *tree = nullptr;
+    ZXA_TRY(build_tree(ctx, state, prec, m, LS, lv, &tree));
+    const uint64_t nseg = uint64_t(1) << (m - I);
+    ZXA_TRY(ctx.scratch_d.ensure(2 * nseg * sizeof(double)));
+    double *prefix = static_cast<double *>(ctx.scratch_d.ptr);
+    double *cmax = prefix + nseg;
+    hipLaunchKernelGGL(tails_kernel, dim3(static_cast<unsigned>((nseg + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       ctx.stream, tree, lv, int(I), int(m), nseg, prefix, cmax);
+    ZXA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(cummax_kernel, dim3(1), dim3(kThreads), 0, ctx.stream, cmax, nseg);
+    ZXA_HIP(hipGetLastError());
+    ZXA_HIP(hipMemcpyAsync(total, tree + lv.off[m], sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    ZXA_HIP(hipStreamSynchronize(ctx.stream));
+    ctx.sample_m = m;
+    ctx.sample_I = I;
+    ctx.sample_ready = true;
+    return OK;
+}
+
+int sample_search(Ctx &ctx, const void *state, int prec, const double *thresholds, size_t shots, uint64_t *out_real)
+{
+    if (shots == 0)
+        return OK;
+    if (!ctx.sample_ready)
+    {
+        set_error("sampler: search before prepare");
+        return ESTATE;
+    }
+    const unsigned m = ctx.sample_m, I = ctx.sample_I;
+    const unsigned LS = std::min(4u, I);
+    const Levels &lv = ctx.sample_levels_as<Levels>();
+    const double *tree = static_cast<const double *>(ctx.scratch_b.ptr);
+    const uint64_t nseg = uint64_t(1) << (m - I);
+    const double *prefix = static_cast<const double *>(ctx.scratch_d.ptr);
+    const double *cmax = prefix + nseg;
+    const size_t need = shots * (sizeof(double) + sizeof(uint64_t)) + 64;
+    ZXA_TRY(ctx.scratch_c.ensure(need));
+    double *dthr = static_cast<double *>(ctx.scratch_c.ptr);
+    uint64_t *dout = reinterpret_cast<uint64_t *>(dthr + shots);
+    int *derr = reinterpret_cast<int *>(dout + shots);
+    ZXA_HIP(hipMemcpyAsync(dthr, thresholds, shots * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
+    ZXA_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx.stream));
+    const unsigned grid = static_cast<unsigned>((shots + kThreads - 1) / kThreads);
+    ZXA_TRY(ctx.tic());
+    hipLaunchKernelGGL(search_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, state, prec == C64 ? 1 : 0, tree, lv,
+                       int(I), int(LS), cmax, prefix, nseg, dthr, uint64_t(shots), dout, derr);
+    ZXA_HIP(hipGetLastError());
+    ZXA_TRY(ctx.toc(F_SAMPLE, 0));
+    int herr = 0;
+    ZXA_HIP(hipMemcpyAsync(out_real, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream));
+    ZXA_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
+    ZXA_HIP(hipStreamSynchronize(ctx.stream));
+    if (herr)
+    {
+        set_error("measure failed to resolve all local samples");
+        return EUNRESOLVED;
+    }
+    return OK;
+}
+
+int sample_apply(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, const double *thresholds,
+                 size_t shots, uint64_t *out_real)
+{
+    if (shots == 0)
+        return OK;
+    double total = 0.0;
+    ZXA_TRY(sample_prepare(ctx, state, prec, m, seg_bits, &total));
+    return sample_search(ctx, state, prec, thresholds, shots, out_real);
+}
+
+} // namespace zxa

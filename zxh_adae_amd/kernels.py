@@ -1,0 +1,131 @@
+"""Kernel-level access to the HIP passes over a device state buffer.
+
+Mirrors the reference's internal backend API (src/zxh/include/zxhsim/kernels.h
+and the sampler of measure.h) through the C ABI's zxh_k_* entry points.  Used
+by the per-kernel parity tests and the benchmark.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+NO_SEL = 0xFFFFFFFFFFFFFFFF
+
+
+class U3Desc(C.Structure):
+    _fields_ = [("packed_sel", C.c_uint64), ("delta", C.c_uint64), ("u", C.c_double * 8)]
+
+
+def pack_sel(alpha: int, beta: bool) -> int:
+    return (alpha & ((1 << 63) - 1)) | ((1 << 63) if beta else 0)
+
+
+class DeviceState:
+    """A device buffer of 2^nbits amplitudes (complex128 or complex64)."""
+
+    def __init__(self, nbits: int, precision: str = "c128"):
+        self.lib = _lib.load()
+        self.nbits = int(nbits)
+        self.prec = _lib.ZXH_C128 if precision in ("c128", "complex128") else _lib.ZXH_C64
+        self.dtype = np.complex128 if self.prec == _lib.ZXH_C128 else np.complex64
+        self.nbytes = (1 << self.nbits) * np.dtype(self.dtype).itemsize
+        p = C.c_void_p()
+        check(self.lib.zxh_dev_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.lib.zxh_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+    def upload(self, psi: np.ndarray):
+        a = np.ascontiguousarray(psi, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(self.lib.zxh_dev_upload(self.ptr, a.ctypes.data, a.nbytes))
+
+    def download(self) -> np.ndarray:
+        out = np.empty(1 << self.nbits, dtype=self.dtype)
+        check(self.lib.zxh_dev_download(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    # ---- passes ----
+    def u3_batch(self, descs: Sequence[tuple]):
+        """descs: (packed_sel, delta, 2x2 complex matrix) per gate, distinct targets."""
+        arr = (U3Desc * max(len(descs), 1))()
+        for i, (sel, delta, u) in enumerate(descs):
+            arr[i].packed_sel = sel
+            arr[i].delta = delta
+            flat = np.asarray(u, dtype=np.complex128).reshape(4)
+            for k in range(4):
+                arr[i].u[2 * k] = flat[k].real
+                arr[i].u[2 * k + 1] = flat[k].imag
+        check(self.lib.zxh_k_u3_batch(self.ptr, self.prec, self.nbits, C.cast(arr, C.c_void_p), len(descs)))
+
+    def diag(self, sel0: Sequence[int], sel1: Sequence[int], theta: Sequence[float]):
+        s0 = np.ascontiguousarray(sel0, dtype=np.uint64)
+        s1 = np.ascontiguousarray(sel1, dtype=np.uint64)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        check(self.lib.zxh_k_diag(self.ptr, self.prec, self.nbits, s0.ctypes.data_as(_lib._u64p),
+                                  s1.ctypes.data_as(_lib._u64p), th.ctypes.data_as(_lib._dp), len(th)))
+
+    def x(self, delta: int, sel: int):
+        check(self.lib.zxh_k_x(self.ptr, self.prec, self.nbits, delta, sel))
+
+    def cx(self, delta: int, sel_cq: int, sel_q: int):
+        check(self.lib.zxh_k_cx(self.ptr, self.prec, self.nbits, delta, sel_cq, sel_q))
+
+    def cp(self, sel_cq: int, sel_q: int, theta: float):
+        check(self.lib.zxh_k_cp(self.ptr, self.prec, self.nbits, sel_cq, sel_q, theta))
+
+    def mcx(self, ctrl_mask: int, target: int):
+        check(self.lib.zxh_k_mcx(self.ptr, self.prec, self.nbits, ctrl_mask, target))
+
+    def sample(self, seg_bits: int, stream_seed: int, shots: int) -> np.ndarray:
+        out = np.zeros(max(shots, 1), dtype=np.uint64)
+        check(self.lib.zxh_k_sample(self.ptr, self.prec, self.nbits, seg_bits, stream_seed, shots,
+                                    out.ctypes.data_as(_lib._u64p)))
+        return out[:shots]
+
+    def sample_thresholds(self, seg_bits: int, thresholds: np.ndarray) -> np.ndarray:
+        thr = np.ascontiguousarray(thresholds, dtype=np.float64)
+        out = np.zeros(max(thr.size, 1), dtype=np.uint64)
+        check(self.lib.zxh_k_sample_thresholds(self.ptr, self.prec, self.nbits, seg_bits,
+                                               thr.ctypes.data_as(_lib._dp), thr.size,
+                                               out.ctypes.data_as(_lib._u64p)))
+        return out[: thr.size]
+
+    def prob_total(self) -> float:
+        out = C.c_double()
+        check(self.lib.zxh_k_prob_total(self.ptr, self.prec, self.nbits, C.byref(out)))
+        return out.value
+
+
+def sync():
+    check(_lib.load().zxh_dev_sync())
+
+
+FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6}
+
+
+def set_timing(on: bool):
+    check(_lib.load().zxh_k_set_timing(int(bool(on))))
+
+
+def reset_timing():
+    check(_lib.load().zxh_k_reset_timing())
+
+
+def timing(family: str) -> dict:
+    """Launch count, total milliseconds (HIP events on the launch stream) and
+    algorithmic bytes of the kernel-level calls of one family."""
+    launches, ms, nbytes = C.c_uint64(), C.c_double(), C.c_uint64()
+    check(_lib.load().zxh_k_get_timing(FAMILIES[family], C.byref(launches), C.byref(ms), C.byref(nbytes)))
+    return {"launches": launches.value, "ms": ms.value, "bytes": nbytes.value}
