@@ -164,6 +164,41 @@ def cpu_baseline(W, targets=4):
                       f"one execute() incl. its 16 GiB state reset), {dt:.1f} s"}
 
 
+def kernels_only(Z, K, W, repeats, prec):
+    """Each hot kernel on a 30-qubit state, repeats launches each, plus the
+    copy / in-place-scale calibration kernels (for profilers and counters)."""
+    import numpy as np
+    from zxh_adae_amd.kernels import DeviceState, NO_SEL
+    st = DeviceState(NQ, prec)
+    other = DeviceState(NQ, prec)
+    st.scale(0.0)  # defined contents (zeros) without a host upload
+    other.scale(0.0)
+    gs = W.diag_batch(NQ, 64, 164)
+    s0 = [(1 << g[2]) for g in gs]
+    s1 = [NO_SEL if g[0] == W.P else (1 << g[1]) for g in gs]
+    th = [g[3] for g in gs]
+    one = [((1 << 29), (1 << 29), W.u3_matrix(0.3, 0.2, 0.1))]
+    allq = [((1 << g[2]), (1 << g[2]), W.u3_matrix(g[3], g[4], g[5])) for g in W.u3_all(NQ, 77)]
+    K.reset_timing()
+    K.set_timing(True)
+    out = {}
+    for name, fn in (("copy", lambda: other.copy_from(st)), ("scale", lambda: st.scale(1.0, 0.0)),
+                     ("diag", lambda: st.diag(s0, s1, th)), ("u3_one", lambda: st.u3_batch(one)),
+                     ("u3_all", lambda: st.u3_batch(allq))):
+        log(f"kernel {name} x{repeats} ({prec})")
+        for _ in range(repeats):
+            fn()
+        K.sync()
+    for fam in ("copy", "scale", "diag", "u3_tile"):
+        t = K.timing(fam)
+        if t["launches"]:
+            avg = t["ms"] / t["launches"]
+            out[fam] = {"launches": t["launches"], "avg_ms": round(avg, 4),
+                        "GB/s": round(t["bytes"] / t["launches"] / (avg * 1e-3) / 1e9, 1)}
+    K.set_timing(False)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +206,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--kernels", choices=["c128", "c64"], default=None,
+                    help="only launch each hot kernel --steps times on a 30-qubit state (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine is single-GPU by design",
@@ -182,6 +219,10 @@ def main():
     from zxh_adae_amd import workloads as W
 
     Z.init()
+    if args.kernels:
+        print(json.dumps({"kernels": kernels_only(Z, K, W, args.steps, args.kernels), "precision": args.kernels,
+                          "qubits": NQ}), flush=True)
+        return
     log(f"QFT-{NQ} c128: warmup {args.warmup}, steps {args.steps}")
     sim, ngates, t_step, stats = qft_step(Z, W, "c128", args.steps, args.warmup)
     fam, roof = roofline_of(stats)

@@ -59,8 +59,12 @@ extern "C" {
 #define ZXH_G_U3 10
 #define ZXH_G_RX 11 /* recorded as U3(theta, pi/2, -pi/2) (zxh.cpp:303-310) */
 
-/* ---- batching policy ---- */
-#define ZXH_POLICY_REFERENCE 0 /* flush exactly where ZXH::execute flushes (zxh.cpp:437-548) */
+/* ---- batching policy ----
+ * Both policies apply the same operations the reference's execute() applies
+ * (same U3 clusters, same identity-skip decisions, same diagonal windows);
+ * they differ only in how many HBM sweeps carry them. */
+#define ZXH_POLICY_REFERENCE 0 /* one pass per flushed batch, as the reference (zxh.cpp:437-548) */
+#define ZXH_POLICY_FUSED 1     /* default: consecutive batches share a tile pass when they commute */
 
 typedef struct zxh_sim zxh_sim;
 
@@ -116,7 +120,7 @@ int zxh_required_M(const zxh_sim *s, size_t *out); /* ZXH::required_M (zxh.cpp:6
 int zxh_num_qubits(const zxh_sim *s, size_t *out); /* ZXH::num_qubits */
 
 /* ---- extensions over the reference's class (no reference counterpart) ---- */
-int zxh_set_policy(zxh_sim *s, int policy);
+int zxh_set_policy(zxh_sim *s, int policy); /* ZXH_POLICY_*; default ZXH_POLICY_FUSED */
 /* sampling segment bits, the ZXHSIM_SEG_MIN_I setting (sv_common.cpp:62-100) */
 int zxh_set_seg_bits(zxh_sim *s, size_t bits);
 /* raw physical state: 2^m amplitudes of the simulator's precision (host copy) */
@@ -130,7 +134,8 @@ int zxh_state_device_ptr(const zxh_sim *s, void **dptr, size_t *m_out);
 /* per-kernel-family timing with HIP events (0 = off) */
 int zxh_set_timing(zxh_sim *s, int on);
 /* family: 0 diag, 1 u3 tile pass, 2 u3 pair (general), 3 perm, 4 cp,
- * 5 sample, 6 reset.  Fills launches and total milliseconds. */
+ * 5 sample, 6 reset, 7 copy, 8 scale.  Fills launches, total milliseconds
+ * and algorithmic bytes. */
 int zxh_get_timing(zxh_sim *s, int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
 int zxh_reset_timing(zxh_sim *s);
 
@@ -152,6 +157,12 @@ int zxh_dev_sync(void);
 /* u3_block_batch_kernel (kernels.h:26): applies the descriptors (distinct
  * target qubits, commuting) to 2^local_bits amplitudes in place. */
 int zxh_k_u3_batch(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc);
+/* one fused tile pass: the U3 descriptors (single-bit pair offsets with
+ * alpha == delta, at most 4 targets below bit 4 and 8 above, distinct), then
+ * the diagonal window (selectors with at most one alpha bit), in one sweep
+ * (no reference counterpart: the reference runs these as separate sweeps) */
+int zxh_k_fused_pass(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc,
+                     const uint64_t *sel0, const uint64_t *sel1, const double *theta, size_t ndiag);
 /* diag_pending_push_p / try_push_cp / flush (kernels.h:36-41) as one call:
  * ngates entries, sel1[i] == ~0 marks a P gate (Z/Rz already lowered to P). */
 int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
@@ -177,7 +188,14 @@ int zxh_k_sample_thresholds(const void *state, int precision, size_t m, size_t s
                             size_t shots, uint64_t *out_real);
 /* sum of |amplitude|^2 in the reference's pairwise order (f64) */
 int zxh_k_prob_total(const void *state, int precision, size_t m, double *out);
-/* HIP-event timing of the kernel-level calls (same families as zxh_get_timing) */
+/* device copy of bytes (a multiple of 16); the bandwidth and HBM-counter
+ * calibration reference (no reference counterpart) */
+int zxh_k_copy(void *dst, const void *src, size_t bytes);
+/* state *= (re + i im) in place: renormalisation or a global phase (no
+ * reference counterpart) */
+int zxh_k_scale(void *state, int precision, size_t local_bits, double re, double im);
+/* HIP-event timing of the kernel-level calls (same families as zxh_get_timing,
+ * plus 7 copy and 8 scale) */
 int zxh_k_set_timing(int on);
 int zxh_k_get_timing(int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
 int zxh_k_reset_timing(void);

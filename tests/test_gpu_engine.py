@@ -18,20 +18,24 @@ MANIFEST = json.load(open(os.path.join(GOLDEN, "manifest.json")))
 TOL128 = 1e-12
 
 
-def make_sim(Z, n, dx, eager, gates, prec="c128"):
-    s = Z.ZXH(n, dx, eager, precision=prec)
+POLICIES = ["fused", "reference"]
+
+
+def make_sim(Z, n, dx, eager, gates, prec="c128", policy="fused"):
+    s = Z.ZXH(n, dx, eager, precision=prec, policy=policy)
     s.gates(gates)
     s.execute()
     return s
 
 
+@pytest.mark.parametrize("policy", POLICIES)
 @pytest.mark.parametrize("name", MANIFEST["circuits"])
-def test_golden_circuit(gpu_engine, name):
+def test_golden_circuit(gpu_engine, name, policy):
     Z = gpu_engine
     d = np.load(os.path.join(GOLDEN, f"circ_{name}.npz"))
     n, dx, eager, shots, seed, M = (int(v) for v in d["meta"])
     gates = [(int(r[0]), int(r[1]), int(r[2]), r[3], r[4], r[5]) for r in d["gates"]]
-    s = make_sim(Z, n, bool(dx), bool(eager), gates)
+    s = make_sim(Z, n, bool(dx), bool(eager), gates, policy=policy)
     assert s.required_M() == M
     st = s.get_state_array()
     if "state" in d:
@@ -42,7 +46,6 @@ def test_golden_circuit(gpu_engine, name):
     # sampling: bit-exact against the oracle's sampler on the GPU's own state
     # (identical probability vector and uniform stream) ...
     s.set_seed(seed)
-    got = s.results_bits() if False else None
     real = s.sample_real(shots)
     phys = s.phys_state()
     seg = min(M, 18)
@@ -54,14 +57,15 @@ def test_golden_circuit(gpu_engine, name):
     assert mismatch <= 0.01, mismatch
 
 
+@pytest.mark.parametrize("policy", POLICIES)
 @pytest.mark.parametrize("seed", range(12))
-def test_random_circuits_vs_oracle(gpu_engine, seed):
+def test_random_circuits_vs_oracle(gpu_engine, seed, policy):
     Z = gpu_engine
     n = 4 + seed % 11
     eager = seed % 2 == 1
     dx = seed % 3 == 0
     gs = W.random_mixed(n, 40 + 8 * seed, 5000 + seed)
-    s = make_sim(Z, n, dx, eager, gs)
+    s = make_sim(Z, n, dx, eager, gs, policy=policy)
     o = O.OracleSim(n, dx, eager)
     o.gates(gs)
     o.execute()
@@ -69,12 +73,13 @@ def test_random_circuits_vs_oracle(gpu_engine, seed):
     assert np.max(np.abs(s.get_state_array() - o.get_state())) < TOL128
 
 
-def test_layers_20q_plain_full_config(gpu_engine):
+@pytest.mark.parametrize("policy", POLICIES)
+def test_layers_20q_plain_full_config(gpu_engine, policy):
     """20 qubits, 20 layers of U3-all / CP ladder / P-all, 1024 shots."""
     Z = gpu_engine
     d = np.load(os.path.join(GOLDEN, "circ_layers_n20_plain.npz"))
     gates = [(int(r[0]), int(r[1]), int(r[2]), r[3], r[4], r[5]) for r in d["gates"]]
-    s = make_sim(Z, 20, False, True, gates)
+    s = make_sim(Z, 20, False, True, gates, policy=policy)
     st = s.get_state_array()
     idx = d["probe_idx"].astype(np.int64)
     assert np.max(np.abs(st[idx] - d["probe_val"])) < TOL128
@@ -134,3 +139,39 @@ def test_large_u3_round_trip(gpu_engine):
     s = make_sim(Z, n, False, True, base + [(W.BARRIER, 0, 0, 0, 0, 0)] + fwd + [(W.P, 0, 0, 0.0, 0, 0)] + inv)
     b = s.phys_state()
     assert np.max(np.abs(a - b)) < 1e-12
+
+
+@pytest.mark.parametrize("n,gates_fn", [(16, lambda n: W.qft(n)), (14, lambda n: W.random_layers(n, 5, 3)),
+                                       (13, lambda n: W.random_mixed(n, 150, 77, kinds=(W.H, W.U3, W.P, W.CP, W.RZ, W.Z,
+                                                                                       W.BARRIER)))])
+def test_fused_policy_matches_oracle_and_reference_policy(gpu_engine, n, gates_fn):
+    """Plain-method circuits through the fused policy (U3 batches merged into
+    tile passes with diagonal epilogues) agree with the oracle and with the
+    reference-policy run."""
+    Z = gpu_engine
+    gs = gates_fn(n)
+    o = O.OracleSim(n, False, True)
+    o.gates(gs)
+    o.execute()
+    want = o.get_state()
+    for prec, tol in (("c128", TOL128), ("c64", 2e-5)):
+        a = make_sim(Z, n, False, True, gs, prec, "fused").get_state_array()
+        b = make_sim(Z, n, False, True, gs, prec, "reference").get_state_array()
+        assert np.max(np.abs(a - want)) < tol
+        assert np.max(np.abs(b - want)) < tol
+
+
+def test_fused_policy_composes_across_commuting_windows(gpu_engine):
+    """U3(q); P(q + 6); U3(q)^-1: the P commutes with both U3 gates, so the two
+    U3 matrices meet in one pass (composed exactly, no identity skip) and must
+    agree with the oracle, which applies them in separate sweeps."""
+    Z = gpu_engine
+    n = 12
+    gs = W.hadamard_all(n) + [(W.BARRIER, 0, 0, 0, 0, 0)]
+    for q in range(6):
+        gs += [(W.U3, 0, q, 0.4, 0.3, 0.2), (W.P, 0, q + 6, 0.7, 0, 0), (W.U3, 0, q, -0.4, -0.2, -0.3)]
+    o = O.OracleSim(n, False, True)
+    o.gates(gs)
+    o.execute()
+    s = make_sim(Z, n, False, True, gs, "c128", "fused")
+    assert np.max(np.abs(s.get_state_array() - o.get_state())) < TOL128

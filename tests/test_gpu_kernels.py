@@ -265,3 +265,31 @@ def test_mcx_is_exact_permutation(gpu_engine, n, ctrls, target):
     d = dev(gpu_engine, psi)
     d.mcx(mask, target)
     assert np.array_equal(d.download(), psi[perm])
+
+
+# ---------------- fused tile pass (U3 batch then diagonal window) ----------------
+
+@pytest.mark.parametrize("n,low,high,ndiag", [(12, [0, 2], [5, 11], 30), (13, [0, 1, 2, 3], list(range(4, 12)), 100),
+                                             (16, [], [15, 14, 4, 9], 64), (20, [1], [19, 12, 7], 256),
+                                             (14, [3], [], 10), (15, [], [], 40)])
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_fused_pass_matches_u3_then_diag(gpu_engine, n, low, high, ndiag, prec):
+    rng = np.random.default_rng(n * 13 + ndiag)
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in low + high]
+    s0, s1, th = diag_window(rng, n, ndiag)
+    want = oracle_diag(oracle_u3(psi, n, descs) if descs else psi, n, s0, s1, th)
+    d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    d.fused_pass(descs, s0, s1, th)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
+def test_fused_pass_rejects_multibit_and_overfull(gpu_engine):
+    from zxh_adae_amd import ZXHError
+    rng = np.random.default_rng(1)
+    d = dev(gpu_engine, rand_state(rng, 13))
+    with pytest.raises(ZXHError):
+        d.fused_pass([(pack(3, False), 1, rand_u(rng))], [], [], [])
+    with pytest.raises(ZXHError):
+        d.fused_pass([(pack(1 << q, False), 1 << q, rand_u(rng)) for q in range(4, 13)], [], [], [])

@@ -8,7 +8,8 @@ through the C ABI of libzxh_amd.so to the HIP engine; the reference aborts the
 process on errors, this module raises ``ZXHError`` instead.
 
 Extensions (no reference counterpart) are keyword-only or separately named:
-``precision="c64"``, ``get_state_array``, ``sample_real``, ``phys_state``,
+``precision="c64"``, ``policy="reference"`` (one HBM pass per flushed batch
+instead of the default fused passes), ``get_state_array``, ``sample_real``, ``phys_state``,
 ``set_phys_state``, timing accessors.
 """
 from __future__ import annotations
@@ -26,7 +27,9 @@ __all__ = ["init", "finalize", "active", "rank", "nprocs", "PrintRes", "ZXH", "Z
 _PREC = {"c128": _lib.ZXH_C128, "complex128": _lib.ZXH_C128, "c64": _lib.ZXH_C64, "complex64": _lib.ZXH_C64}
 
 # kernel family ids of zxh_get_timing
-TIMING_FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6}
+TIMING_FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6, "copy": 7,
+                   "scale": 8, "fused": 9}
+_POLICY = {"fused": _lib.ZXH_POLICY_FUSED, "reference": _lib.ZXH_POLICY_REFERENCE}
 
 
 def init() -> None:
@@ -61,9 +64,11 @@ def PrintRes(res: Sequence[Sequence[bool]], n: int) -> None:
 
 class ZXH:
     def __init__(self, n: int, disable_x: bool = False, eager_expand_all: bool = False, *,
-                 precision: str = "c128"):
+                 precision: str = "c128", policy: str = "fused"):
         if precision not in _PREC:
             raise ValueError(f"unknown precision {precision!r}")
+        if policy not in _POLICY:
+            raise ValueError(f"unknown policy {policy!r}")
         self._lib = _lib.load()
         self._n = int(n)
         self._prec = _PREC[precision]
@@ -71,6 +76,7 @@ class ZXH:
         check(self._lib.zxh_create(self._n, int(bool(disable_x)), int(bool(eager_expand_all)), self._prec,
                                    C.byref(h)))
         self._h = h
+        check(self._lib.zxh_set_policy(self._h, _POLICY[policy]))
 
     def __del__(self):
         h = getattr(self, "_h", None)

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(HERE, "lib", "libzxh_amd.so")
 ZXH_OK = 0
 ZXH_C128 = 0
 ZXH_C64 = 1
+ZXH_POLICY_REFERENCE = 0
+ZXH_POLICY_FUSED = 1
 
 _sz = C.c_size_t
 _u64 = C.c_uint64
@@ -69,6 +71,7 @@ _SIGS = [
     ("zxh_dev_sync", C.c_int, []),
     ("zxh_k_u3_batch", C.c_int, [_vp, C.c_int, _sz, _vp, _sz]),
     ("zxh_k_diag", C.c_int, [_vp, C.c_int, _sz, _u64p, _u64p, _dp, _sz]),
+    ("zxh_k_fused_pass", C.c_int, [_vp, C.c_int, _sz, _vp, _sz, _u64p, _u64p, _dp, _sz]),
     ("zxh_k_x", C.c_int, [_vp, C.c_int, _sz, _u64, _u64]),
     ("zxh_k_cx", C.c_int, [_vp, C.c_int, _sz, _u64, _u64, _u64]),
     ("zxh_k_cp", C.c_int, [_vp, C.c_int, _sz, _u64, _u64, C.c_double]),
@@ -76,6 +79,8 @@ _SIGS = [
     ("zxh_k_sample", C.c_int, [_vp, C.c_int, _sz, _sz, _u64, _sz, _u64p]),
     ("zxh_k_sample_thresholds", C.c_int, [_vp, C.c_int, _sz, _sz, _dp, _sz, _u64p]),
     ("zxh_k_prob_total", C.c_int, [_vp, C.c_int, _sz, _dp]),
+    ("zxh_k_copy", C.c_int, [_vp, _vp, _sz]),
+    ("zxh_k_scale", C.c_int, [_vp, C.c_int, _sz, C.c_double, C.c_double]),
     ("zxh_k_set_timing", C.c_int, [C.c_int]),
     ("zxh_k_get_timing", C.c_int, [C.c_int, _u64p, _dp, _u64p]),
     ("zxh_k_reset_timing", C.c_int, []),

@@ -228,8 +228,9 @@ int zxh_num_qubits(const zxh_sim *s, size_t *out)
 int zxh_set_policy(zxh_sim *s, int policy)
 {
     CHECK_SIM(s);
-    if (policy != ZXH_POLICY_REFERENCE)
+    if (policy != ZXH_POLICY_REFERENCE && policy != ZXH_POLICY_FUSED)
         return fail(ZXH_EINVAL, "unknown batching policy");
+    s->sim->set_policy(policy);
     return ZXH_OK;
 }
 
@@ -368,6 +369,20 @@ int zxh_k_u3_batch(void *state, int precision, size_t local_bits, const zxh_u3_d
                                    reinterpret_cast<const zxa::U3Gate *>(descs), ndesc));
 }
 
+int zxh_k_fused_pass(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc,
+                     const uint64_t *sel0, const uint64_t *sel1, const double *theta, size_t ndiag)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    std::vector<zxa::DiagGate> g(ndiag);
+    for (size_t i = 0; i < ndiag; i++)
+        g[i] = {sel0[i], sel1[i], theta[i]};
+    return finish(c, zxa::fused_pass_apply(*c, state, precision, unsigned(local_bits),
+                                           reinterpret_cast<const zxa::U3Gate *>(descs), ndesc, g.data(), ndiag));
+}
+
 int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
                const double *theta, size_t ngates)
 {
@@ -456,6 +471,23 @@ int zxh_k_prob_total(const void *state, int precision, size_t m, double *out)
         return rc;
     WITH_CTX(c);
     return zxa::prob_total(*c, state, precision, unsigned(m), out);
+}
+
+int zxh_k_copy(void *dst, const void *src, size_t bytes)
+{
+    if (!dst || !src)
+        return fail(ZXH_EINVAL, "null pointer");
+    WITH_CTX(c);
+    return finish(c, zxa::copy_apply(*c, dst, src, bytes));
+}
+
+int zxh_k_scale(void *state, int precision, size_t local_bits, double re, double im)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return finish(c, zxa::scale_apply(*c, state, precision, unsigned(local_bits), re, im));
 }
 
 int zxh_k_set_timing(int on)

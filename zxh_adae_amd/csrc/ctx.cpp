@@ -104,6 +104,12 @@ int ParamRing::stage(const void *src, size_t bytes, hipStream_t stream, void **d
         return EINVAL_;
     }
     const int s = next_;
+    for (int o : open_)
+        if (o == s)
+        {
+            set_error("parameter ring exhausted before done()");
+            return EINVAL_;
+        }
     next_ = (next_ + 1) % kSlots;
     if (used_[s])
         ZXA_HIP(hipEventSynchronize(ev_[s]));
@@ -111,18 +117,19 @@ int ParamRing::stage(const void *src, size_t bytes, hipStream_t stream, void **d
     char *d = static_cast<char *>(dev_) + size_t(s) * kSlotBytes;
     std::memcpy(h, src, bytes);
     ZXA_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
-    cur_ = s;
+    open_.push_back(s);
     *dev = d;
     return OK;
 }
 
 int ParamRing::done(hipStream_t stream)
 {
-    if (cur_ < 0)
-        return OK;
-    ZXA_HIP(hipEventRecord(ev_[cur_], stream));
-    used_[cur_] = true;
-    cur_ = -1;
+    for (int s : open_)
+    {
+        ZXA_HIP(hipEventRecord(ev_[s], stream));
+        used_[s] = true;
+    }
+    open_.clear();
     return OK;
 }
 

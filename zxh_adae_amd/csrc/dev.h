@@ -83,7 +83,10 @@ enum Family
     F_CP = 4,
     F_SAMPLE = 5,
     F_RESET = 6,
-    F_COUNT = 8
+    F_COPY = 7,
+    F_SCALE = 8,
+    F_FUSED = 9,
+    F_COUNT = 10
 };
 
 struct FamilyTiming
@@ -113,7 +116,7 @@ class ParamRing
     hipEvent_t ev_[kSlots] = {};
     bool used_[kSlots] = {};
     int next_ = 0;
-    int cur_ = -1;
+    std::vector<int> open_; // staged since the last done()
 };
 
 struct Ctx
@@ -189,6 +192,11 @@ struct U3Gate
 
 // Applies commuting gates on distinct targets (a fused U3 cluster).
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates);
+// One tile pass applying commuting single-bit U3 gates (alpha == delta, at
+// most 4 targets below bit 4 and 8 above), then a diagonal window whose
+// selectors have at most one alpha bit, in one sweep.
+int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
+                     const DiagGate *diag, size_t ndiag);
 // One general pair sweep (any delta / selector).
 int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
 
@@ -199,6 +207,9 @@ int mcx_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t ctrl_mas
 
 // State helpers.
 int state_reset(Ctx &ctx, void *state, int prec, size_t alloc_elems);
+// Streaming helpers: device copy (bandwidth calibration) and in-place scale.
+int copy_apply(Ctx &ctx, void *dst, const void *src, size_t bytes);
+int scale_apply(Ctx &ctx, void *state, int prec, unsigned nbits, double re, double im);
 
 // Sampling in two steps: prepare builds the probability tree of 2^m
 // amplitudes and returns the total (the reference's local_prob); search

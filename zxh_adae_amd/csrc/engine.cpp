@@ -7,6 +7,7 @@
 // src/zxh/src/measure/measure_common.cpp (measure 73-92).  The affine map is
 // held as N row masks of at most 63 physical columns.
 #include "engine.h"
+#include "poly.h"
 
 #include <algorithm>
 #include <atomic>
@@ -311,9 +312,109 @@ int Sim::flush_diag(std::vector<DiagGate> &window)
 {
     if (window.empty())
         return OK;
-    int rc = diag_apply(ctx_, dstate_, prec_, active_bits(), window.data(), window.size());
+    int rc = OK;
+    if (fusable() && window_is_polynomial(active_bits(), window.data(), window.size()))
+    {
+        // the window follows every U3 of the pending pass: it becomes (part
+        // of) the pass's diagonal epilogue
+        const uint64_t amask = (uint64_t(1) << 63) - 1;
+        for (const DiagGate &g : window)
+        {
+            pass_diag_.push_back(g);
+            pass_support_ |= g.s0 & amask;
+            if (g.s1 != kNoSel)
+                pass_support_ |= g.s1 & amask;
+        }
+    }
+    else
+    {
+        rc = flush_pass();
+        if (!rc)
+            rc = diag_apply(ctx_, dstate_, prec_, active_bits(), window.data(), window.size());
+    }
     window.clear();
     return rc;
+}
+
+bool Sim::fusable() const
+{
+    return policy_ == POLICY_FUSED && used_bits_ >= 12;
+}
+
+// Runs the pending fused pass: its U3 gates, then its diagonal window.
+int Sim::flush_pass()
+{
+    if (pass_u3_.empty() && pass_diag_.empty())
+        return OK;
+    int rc = fused_pass_apply(ctx_, dstate_, prec_, active_bits(), pass_u3_.data(), pass_u3_.size(),
+                              pass_diag_.data(), pass_diag_.size());
+    pass_u3_.clear();
+    pass_diag_.clear();
+    pass_support_ = 0;
+    pass_high_ = 0;
+    return rc;
+}
+
+// Adds single-bit U3 batches to the pending pass when exact: a gate joins
+// only if it commutes with the pass's diagonal window (its target is outside
+// the window's support), and the tile has room (targets below bit 4 always
+// fit, at most 8 above).  A second gate on a pass target is composed with
+// the first (applied after it).
+int Sim::emit_u3(std::vector<U3Gate> &descs)
+{
+    bool simple = fusable();
+    for (const U3Gate &g : descs)
+    {
+        const uint64_t alpha = g.packed_sel & ((uint64_t(1) << 63) - 1);
+        simple = simple && g.delta != 0 && (g.delta & (g.delta - 1)) == 0 && alpha == g.delta;
+    }
+    if (!simple)
+    {
+        ZXA_TRY(flush_pass());
+        return u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size());
+    }
+    for (const U3Gate &g : descs)
+        if (g.delta & pass_support_)
+        {
+            ZXA_TRY(flush_pass());
+            break;
+        }
+    for (const U3Gate &g0 : descs)
+    {
+        U3Gate g = g0;
+        if (g.packed_sel >> 63)
+        {
+            // beta = 1: the even side is the target-bit-one element
+            const double sw[8] = {g0.u[6], g0.u[7], g0.u[4], g0.u[5], g0.u[2], g0.u[3], g0.u[0], g0.u[1]};
+            std::memcpy(g.u, sw, sizeof(sw));
+            g.packed_sel = g.delta;
+        }
+        U3Gate *same = nullptr;
+        for (U3Gate &o : pass_u3_)
+            if (o.delta == g.delta)
+                same = &o;
+        if (same)
+        {
+            const std::complex<double> a[4] = {{g.u[0], g.u[1]}, {g.u[2], g.u[3]}, {g.u[4], g.u[5]}, {g.u[6], g.u[7]}};
+            const std::complex<double> b[4] = {{same->u[0], same->u[1]}, {same->u[2], same->u[3]},
+                                               {same->u[4], same->u[5]}, {same->u[6], same->u[7]}};
+            const std::complex<double> c[4] = {a[0] * b[0] + a[1] * b[2], a[0] * b[1] + a[1] * b[3],
+                                               a[2] * b[0] + a[3] * b[2], a[2] * b[1] + a[3] * b[3]};
+            for (int i = 0; i < 4; i++)
+            {
+                same->u[2 * i] = c[i].real();
+                same->u[2 * i + 1] = c[i].imag();
+            }
+            continue;
+        }
+        const bool high = g.delta >= 16;
+        if (high && pass_high_ >= 8)
+            ZXA_TRY(flush_pass());
+        pass_u3_.push_back(g);
+        if (high)
+            pass_high_++;
+    }
+    return OK;
 }
 
 // flush_u3_batch (zxh.cpp:390-428): one descriptor per active accumulator,
@@ -351,7 +452,7 @@ int Sim::flush_u3(std::vector<Acc> &pend, size_t &count)
     count = 0;
     if (descs.empty())
         return OK;
-    return u3_apply(ctx_, dstate_, prec_, lb, descs.data(), descs.size());
+    return emit_u3(descs);
 }
 
 int Sim::execute()
@@ -372,6 +473,10 @@ int Sim::execute()
         m_ = N_;
     }
 
+    pass_u3_.clear();
+    pass_diag_.clear();
+    pass_support_ = 0;
+    pass_high_ = 0;
     std::vector<DiagGate> window;
     std::vector<Acc> pend(N_);
     size_t pend_count = 0;
@@ -397,6 +502,7 @@ int Sim::execute()
                 {
                     // flush the batch, then the eager, expanding apply_u3
                     ZXA_TRY(flush_u3(pend, pend_count));
+                    ZXA_TRY(flush_pass());
                     int rc = OK;
                     const uint64_t d = solve_expand(ug.q, &rc);
                     if (rc)
@@ -457,6 +563,7 @@ int Sim::execute()
             if (disable_x_)
             {
                 ZXA_TRY(flush_diag(window));
+                ZXA_TRY(flush_pass());
                 int rc = OK;
                 const uint64_t d = solve_expand(g.q, &rc);
                 if (rc)
@@ -471,6 +578,7 @@ int Sim::execute()
             if (disable_x_)
             {
                 ZXA_TRY(flush_diag(window));
+                ZXA_TRY(flush_pass());
                 int rc = OK;
                 const uint64_t d = solve_expand(g.q, &rc);
                 if (rc)
@@ -489,6 +597,7 @@ int Sim::execute()
             if (sel_class(scq) == 2 || sel_class(sq) == 2)
             {
                 ZXA_TRY(flush_diag(window));
+                ZXA_TRY(flush_pass());
                 ZXA_TRY(cp_apply(ctx_, dstate_, prec_, active_bits(), scq, sq, g.theta));
             }
             else
@@ -501,6 +610,7 @@ int Sim::execute()
         gi++;
     }
     ZXA_TRY(flush_diag(window));
+    ZXA_TRY(flush_pass());
     ZXA_HIP(hipStreamSynchronize(ctx_.stream));
     executed_ = true;
     return OK;

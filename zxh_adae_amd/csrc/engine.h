@@ -34,6 +34,12 @@ enum GateCode
     G_RX = 11
 };
 
+enum Policy
+{
+    POLICY_REFERENCE = 0, // every flushed batch is its own pass, as in the reference
+    POLICY_FUSED = 1      // flushed batches merge into tile passes when exact
+};
+
 struct GateRec
 {
     int type;
@@ -93,6 +99,14 @@ class Sim
     {
         return prec_;
     }
+    void set_policy(int p)
+    {
+        policy_ = p;
+    }
+    int policy() const
+    {
+        return policy_;
+    }
 
   private:
     struct Acc
@@ -111,6 +125,9 @@ class Sim
     uint64_t solve_expand(size_t q, int *rc);
     int flush_diag(std::vector<DiagGate> &window);
     int flush_u3(std::vector<Acc> &pend, size_t &count);
+    int emit_u3(std::vector<U3Gate> &descs);
+    int flush_pass();
+    bool fusable() const;
     int resize_reset(size_t M);
 
     size_t N_;
@@ -135,6 +152,12 @@ class Sim
     std::vector<uint64_t> real_results_;
     std::vector<uint64_t> packed_;
     size_t result_count_ = 0;
+    // pending fused pass: single-bit U3 gates, then a diagonal window
+    int policy_ = POLICY_FUSED;
+    std::vector<U3Gate> pass_u3_;
+    std::vector<DiagGate> pass_diag_;
+    uint64_t pass_support_ = 0;
+    int pass_high_ = 0;
     Ctx ctx_;
 };
 

@@ -23,7 +23,9 @@
 // General path (multi-bit selectors after CX transport, or states narrower
 // than one chunk): per-amplitude evaluation of every gate, like the reference.
 #include "dev.h"
+#include "poly.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -301,56 +303,18 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t work_items)
     return static_cast<int>(work_items < cap ? work_items : cap);
 }
 
-// Adds theta * [S0][S1] to the polynomial (K, L, Q) for selectors with at
-// most one alpha bit: [x_j ^ beta] = beta + (1 - 2 beta) x_j.
-struct Lin
-{
-    int bit;  // -1: constant
-    double c; // constant part
-    double s; // slope of x_bit
-};
-Lin lin_of(uint64_t packed)
-{
-    const uint64_t alpha = packed & ((uint64_t(1) << 63) - 1);
-    const bool beta = (packed >> 63) & 1u;
-    if (alpha == 0)
-        return {-1, beta ? 1.0 : 0.0, 0.0};
-    return {__builtin_ctzll(alpha), beta ? 1.0 : 0.0, beta ? -1.0 : 1.0};
-}
-
 int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
 {
     const bool c64 = prec == C64;
     const int LB = c64 ? Geo<true>::LB : Geo<false>::LB;
     const int NCO = LB + kRowBits;
     const int nh = static_cast<int>(n) - NCO;
-    std::vector<double> L(n, 0.0), Q(size_t(n) * n, 0.0);
-    double K = 0.0;
-    for (size_t i = 0; i < ngates; i++)
-    {
-        const DiagGate &g = gates[i];
-        const Lin a = lin_of(g.s0);
-        const Lin b = g.s1 == kNoSel ? Lin{-1, 1.0, 0.0} : lin_of(g.s1);
-        const double th = g.theta;
-        K += th * a.c * b.c;
-        if (b.bit >= 0)
-            L[b.bit] += th * a.c * b.s;
-        if (a.bit >= 0)
-            L[a.bit] += th * b.c * a.s;
-        if (a.bit >= 0 && b.bit >= 0)
-        {
-            const double q = th * a.s * b.s;
-            if (a.bit == b.bit)
-                L[a.bit] += q;
-            else
-                Q[size_t(std::min(a.bit, b.bit)) * n + std::max(a.bit, b.bit)] += q;
-        }
-    }
-    auto q_at = [&](int j, int k) { return Q[size_t(j) * n + k]; };
+    Poly P;
+    window_polynomial(n, gates, ngates, P);
 
     const int nlane = 1 << LB;
     std::vector<double> blk;
-    blk.reserve(nlane * (1 + kRowBits) + 64 + NCO * nh + nh + nh * nh + 1);
+    blk.reserve(nlane * (1 + kRowBits) + 64);
     // lane_tab
     for (int xa = 0; xa < nlane; xa++)
     {
@@ -359,10 +323,10 @@ int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates
         {
             if (!((xa >> j) & 1))
                 continue;
-            s += L[j];
+            s += P.L[j];
             for (int k = j + 1; k < LB; k++)
                 if ((xa >> k) & 1)
-                    s += q_at(j, k);
+                    s += P.q(j, k);
         }
         blk.push_back(s);
     }
@@ -373,7 +337,7 @@ int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates
             double s = 0.0;
             for (int j = 0; j < LB; j++)
                 if ((xa >> j) & 1)
-                    s += q_at(j, LB + k);
+                    s += P.q(j, LB + k);
             blk.push_back(s);
         }
     // row_tab
@@ -384,55 +348,39 @@ int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates
         {
             if (!((xw >> k) & 1))
                 continue;
-            s += L[LB + k];
+            s += P.L[LB + k];
             for (int l = k + 1; l < kRowBits; l++)
                 if ((xw >> l) & 1)
-                    s += q_at(LB + k, LB + l);
+                    s += P.q(LB + k, LB + l);
         }
         blk.push_back(s);
     }
-    const size_t tabs_len = blk.size();
-    // cq, hL, hQ
-    for (int j = 0; j < NCO; j++)
-        for (int i = 0; i < nh; i++)
-            blk.push_back(q_at(j, NCO + i));
-    for (int i = 0; i < nh; i++)
-        blk.push_back(L[NCO + i]);
-    for (int i = 0; i < nh; i++)
-        for (int l = 0; l < nh; l++)
-            blk.push_back(l > i ? q_at(NCO + i, NCO + l) : 0.0);
-
     void *dblk = nullptr;
     ZXA_TRY(ctx.ring.stage(blk.data(), blk.size() * sizeof(double), ctx.stream, &dblk));
-    const double *dtabs = static_cast<const double *>(dblk);
-    const double *dcq = dtabs + tabs_len;
-    const double *dhL = dcq + size_t(NCO) * nh;
-    const double *dhQ = dhL + nh;
 
+    std::vector<int> inner(NCO), outer(nh);
+    for (int j = 0; j < NCO; j++)
+        inner[j] = j;
+    for (int i = 0; i < nh; i++)
+        outer[i] = NCO + i;
     const uint64_t nchunks = uint64_t(1) << nh;
     ZXA_TRY(ctx.scratch_a.ensure(nchunks * kCoefStride * sizeof(double)));
     double *coef = static_cast<double *>(ctx.scratch_a.ptr);
-    {
-        const unsigned grid = static_cast<unsigned>((nchunks + kThreads - 1) / kThreads);
-        hipLaunchKernelGGL(diag_chunk_coef_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, dcq, dhL, dhQ, K, NCO,
-                           nh, nchunks, coef);
-        ZXA_HIP(hipGetLastError());
-    }
+    ZXA_TRY(launch_block_coef(ctx, P, inner.data(), NCO, outer.data(), nh, coef));
+
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
     if (c64)
     {
-        static int grid_cap = -1;
         const int grid = persistent_grid(ctx, diag_poly_kernel<true>, nchunks);
-        (void)grid_cap;
-        hipLaunchKernelGGL(diag_poly_kernel<true>, dim3(grid), dim3(kThreads), 0, ctx.stream, state, dtabs,
-                           static_cast<const double *>(coef), nchunks);
+        hipLaunchKernelGGL(diag_poly_kernel<true>, dim3(grid), dim3(kThreads), 0, ctx.stream, state,
+                           static_cast<const double *>(dblk), static_cast<const double *>(coef), nchunks);
     }
     else
     {
         const int grid = persistent_grid(ctx, diag_poly_kernel<false>, nchunks);
-        hipLaunchKernelGGL(diag_poly_kernel<false>, dim3(grid), dim3(kThreads), 0, ctx.stream, state, dtabs,
-                           static_cast<const double *>(coef), nchunks);
+        hipLaunchKernelGGL(diag_poly_kernel<false>, dim3(grid), dim3(kThreads), 0, ctx.stream, state,
+                           static_cast<const double *>(dblk), static_cast<const double *>(coef), nchunks);
     }
     ZXA_HIP(hipGetLastError());
     ZXA_TRY(ctx.toc(F_DIAG, bytes));
@@ -461,6 +409,97 @@ int diag_general(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *ga
 
 } // namespace
 
+bool window_is_polynomial(unsigned n, const DiagGate *gates, size_t ngates)
+{
+    const uint64_t amask = (uint64_t(1) << 63) - 1;
+    for (size_t i = 0; i < ngates; i++)
+    {
+        const uint64_t a0 = gates[i].s0 & amask;
+        if (__builtin_popcountll(a0) > 1 || (n < 64 && (a0 >> n) != 0))
+            return false;
+        if (gates[i].s1 != kNoSel)
+        {
+            const uint64_t a1 = gates[i].s1 & amask;
+            if (__builtin_popcountll(a1) > 1 || (n < 64 && (a1 >> n) != 0))
+                return false;
+        }
+    }
+    return true;
+}
+
+// theta * [S0][S1] with [x_j ^ beta] = beta + (1 - 2 beta) x_j.
+void window_polynomial(unsigned n, const DiagGate *gates, size_t ngates, Poly &P)
+{
+    struct Lin
+    {
+        int bit;  // -1: constant selector
+        double c; // constant part
+        double s; // slope of x_bit
+    };
+    auto lin_of = [](uint64_t packed) -> Lin {
+        const uint64_t alpha = packed & ((uint64_t(1) << 63) - 1);
+        const bool beta = (packed >> 63) & 1u;
+        if (alpha == 0)
+            return {-1, beta ? 1.0 : 0.0, 0.0};
+        return {__builtin_ctzll(alpha), beta ? 1.0 : 0.0, beta ? -1.0 : 1.0};
+    };
+    P.n = n;
+    P.K = 0.0;
+    P.L.assign(n, 0.0);
+    P.Q.assign(size_t(n) * n, 0.0);
+    for (size_t i = 0; i < ngates; i++)
+    {
+        const DiagGate &g = gates[i];
+        const Lin a = lin_of(g.s0);
+        const Lin b = g.s1 == kNoSel ? Lin{-1, 1.0, 0.0} : lin_of(g.s1);
+        const double th = g.theta;
+        P.K += th * a.c * b.c;
+        if (b.bit >= 0)
+            P.L[b.bit] += th * a.c * b.s;
+        if (a.bit >= 0)
+            P.L[a.bit] += th * b.c * a.s;
+        if (a.bit >= 0 && b.bit >= 0)
+        {
+            const double q = th * a.s * b.s;
+            if (a.bit == b.bit)
+                P.L[a.bit] += q;
+            else
+                P.Q[size_t(std::min(a.bit, b.bit)) * n + std::max(a.bit, b.bit)] += q;
+        }
+    }
+}
+
+int launch_block_coef(Ctx &ctx, const Poly &P, const int *inner, int nco, const int *outer, int nh, double *out)
+{
+    if (nco + 1 > kCoefStride)
+    {
+        set_error("block coefficients: too many inner bits");
+        return EINVAL_;
+    }
+    std::vector<double> blk;
+    blk.reserve(size_t(nco) * nh + nh + size_t(nh) * nh);
+    for (int j = 0; j < nco; j++)
+        for (int i = 0; i < nh; i++)
+            blk.push_back(P.q(inner[j], outer[i]));
+    for (int i = 0; i < nh; i++)
+        blk.push_back(P.L[outer[i]]);
+    for (int i = 0; i < nh; i++)
+        for (int l = 0; l < nh; l++)
+            blk.push_back(l > i ? P.q(outer[i], outer[l]) : 0.0);
+    if (blk.empty())
+        blk.push_back(0.0);
+    void *d = nullptr;
+    ZXA_TRY(ctx.ring.stage(blk.data(), blk.size() * sizeof(double), ctx.stream, &d));
+    const double *dcq = static_cast<const double *>(d);
+    const double *dhL = dcq + size_t(nco) * nh;
+    const double *dhQ = dhL + nh;
+    const uint64_t nblocks = uint64_t(1) << nh;
+    const unsigned grid = static_cast<unsigned>((nblocks + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(diag_chunk_coef_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, dcq, dhL, dhQ, P.K, nco, nh,
+                       nblocks, out);
+    return hip_check(hipGetLastError(), "block coefficient kernel");
+}
+
 int diag_apply(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
 {
     if (ngates == 0)
@@ -470,19 +509,7 @@ int diag_apply(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gate
         set_error("diag_apply: state wider than 40 bits");
         return EINVAL_;
     }
-    bool simple = true;
-    for (size_t i = 0; i < ngates && simple; i++)
-    {
-        const uint64_t a0 = gates[i].s0 & ((uint64_t(1) << 63) - 1);
-        if (__builtin_popcountll(a0) > 1 || (a0 >> n) != 0)
-            simple = false;
-        if (gates[i].s1 != kNoSel)
-        {
-            const uint64_t a1 = gates[i].s1 & ((uint64_t(1) << 63) - 1);
-            if (__builtin_popcountll(a1) > 1 || (a1 >> n) != 0)
-                simple = false;
-        }
-    }
+    const bool simple = window_is_polynomial(n, gates, ngates);
     const unsigned need = (prec == C64 ? Geo<true>::CB : Geo<false>::CB);
     if (simple && n >= need)
         return diag_fast(ctx, state, prec, n, gates, ngates);

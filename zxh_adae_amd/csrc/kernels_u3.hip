@@ -19,6 +19,7 @@
 // affine mode), and states narrower than a tile, use the pair kernel: one
 // sweep per gate, pairs enumerated by inserting a zero at the lowest delta bit.
 #include "dev.h"
+#include "poly.h"
 
 #include <algorithm>
 #include <cstring>
@@ -227,8 +228,130 @@ __device__ __forceinline__ void transpose(typename Elem<kC64>::T (&v)[kRegs], ty
         v[r] = lds[swz<kC64>(tr | reg_tau<kC64, LR>(r))];
 }
 
-template <bool kC64, bool kL1, bool kL2>
-__global__ __launch_bounds__(kThreads) void u3_tile_kernel(void *__restrict__ state, const TilePass P)
+__device__ __forceinline__ cd cmul(cd a, cd b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+__device__ __forceinline__ cd cis(double a)
+{
+    double s, c;
+    sincos(a, &s, &c);
+    return {c, s};
+}
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    const unsigned long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffu), lane);
+    const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+    return __longlong_as_double((static_cast<unsigned long long>(static_cast<unsigned>(hi)) << 32) |
+                                static_cast<unsigned>(lo));
+}
+
+// Diagonal epilogue of a fused pass, applied in the load/store layout after
+// every gate of the pass.  With theta the window's phase polynomial and the
+// tile's element index split into thread bits T and register bits R,
+//     theta = A(t) + B(r) + sum_{j in R} r_j d_j(t)
+// where A is per thread and tile, B uniform per tile (computed by lane r),
+// and d_j tile-independent.  The 16 registers are visited in Gray-code order
+// so that exp(i sum r_j d_j) costs one complex product per element.
+// dtab: Lt[12] | Qt[12 * 12] (linear and pairwise coefficients of the tile
+// bits); coef: per tile, c_i (i < 12) and C from the non-tile bits.
+template <bool kC64>
+struct DiagEpi
+{
+    double thr_const = 0.0; // sum over set thread bits of Lt and pairwise Qt
+    double reg_const = 0.0; // same over the register bits of lane r = lane & 15
+    cd g[4];                // exp(i d_j(t)) for the four register bits
+
+    __device__ __forceinline__ void init(const double *__restrict__ dtab, int t)
+    {
+        const double *Lt = dtab;
+        const double *Qt = dtab + kTileBits;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+        {
+            if (!((t >> j) & 1))
+                continue;
+            const int i = thread_tile_bit<kC64, 0>(j);
+            thr_const += Lt[i];
+#pragma unroll
+            for (int k = j + 1; k < 8; k++)
+                if ((t >> k) & 1)
+                    thr_const += Qt[i * kTileBits + thread_tile_bit<kC64, 0>(k)];
+        }
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++)
+        {
+            const int R = Lay<kC64>::R[0][jj];
+            double d = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+            {
+                if (!((t >> j) & 1))
+                    continue;
+                const int i = thread_tile_bit<kC64, 0>(j);
+                d += Qt[(i < R ? i : R) * kTileBits + (i < R ? R : i)];
+            }
+            g[jj] = cis(d);
+        }
+        const int r = t & 15;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++)
+        {
+            if (!((r >> jj) & 1))
+                continue;
+            reg_const += Lt[Lay<kC64>::R[0][jj]];
+#pragma unroll
+            for (int kk = jj + 1; kk < 4; kk++)
+                if ((r >> kk) & 1)
+                    reg_const += Qt[Lay<kC64>::R[0][jj] * kTileBits + Lay<kC64>::R[0][kk]];
+        }
+    }
+
+    __device__ __forceinline__ void apply(typename Elem<kC64>::T (&v)[kRegs], const double *__restrict__ cc, int t)
+    {
+        double A = thr_const + cc[kTileBits];
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if ((t >> j) & 1)
+                A += cc[thread_tile_bit<kC64, 0>(j)];
+        double B = reg_const;
+#pragma unroll
+        for (int jj = 0; jj < 4; jj++)
+            if (((t & 15) >> jj) & 1)
+                B += cc[Lay<kC64>::R[0][jj]];
+        const cd bph = cis(B);
+        cd w = cis(A);
+#pragma unroll
+        for (int k = 0; k < kRegs; k++)
+        {
+            const int r = k ^ (k >> 1);
+            if (k > 0)
+            {
+                const int j = __builtin_ctz(k);
+                const cd gj = ((r >> j) & 1) ? g[j] : cd{g[j].x, -g[j].y};
+                w = cmul(w, gj);
+            }
+            const cd ph = cmul(w, {readlane_d(bph.x, r), readlane_d(bph.y, r)});
+            if constexpr (kC64)
+            {
+                const float px = static_cast<float>(ph.x), py = static_cast<float>(ph.y);
+                const float2 a = v[r];
+                v[r] = make_float2(a.x * px - a.y * py, a.x * py + a.y * px);
+            }
+            else
+            {
+                const double2 a = v[r];
+                v[r] = make_double2(a.x * ph.x - a.y * ph.y, a.x * ph.y + a.y * ph.x);
+            }
+        }
+    }
+};
+
+template <bool kC64, bool kL1, bool kL2, bool kDiag>
+__global__ __launch_bounds__(kThreads) void u3_tile_kernel(void *__restrict__ state, const TilePass P,
+                                                           const double *__restrict__ dtab,
+                                                           const double *__restrict__ dcoef)
 {
     using T = typename Elem<kC64>::T;
     __shared__ T lds[(kL1 || kL2) ? (1 << kTileBits) : 1];
@@ -237,6 +360,9 @@ __global__ __launch_bounds__(kThreads) void u3_tile_kernel(void *__restrict__ st
     const uint64_t g1 = thread_goff<kC64, 1>(t, P.tb);
     const uint64_t g2 = thread_goff<kC64, 2>(t, P.tb);
     T *p = static_cast<T *>(state);
+    DiagEpi<kC64> epi;
+    if constexpr (kDiag)
+        epi.init(dtab, t);
 
     for (uint64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x)
     {
@@ -287,6 +413,8 @@ __global__ __launch_bounds__(kThreads) void u3_tile_kernel(void *__restrict__ st
             apply_layout<kC64, 2>(v, P, base, g2);
             transpose<kC64, 2, 0>(v, lds, t);
         }
+        if constexpr (kDiag)
+            epi.apply(v, dcoef + tile * 16, t);
 
         if constexpr (!kC64)
         {
@@ -426,26 +554,35 @@ int tile_grid(Ctx &ctx, K kernel, uint64_t tiles)
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
 }
 
-template <bool kC64>
-int launch_tile(Ctx &ctx, void *state, const TilePass &P, bool l1, bool l2)
+template <bool kC64, bool kL1, bool kL2, bool kDiag>
+int launch_tile_t(Ctx &ctx, void *state, const TilePass &P, const double *dtab, const double *dcoef)
 {
-    const uint64_t tiles = P.ntiles;
-    if (l1 && l2)
-        hipLaunchKernelGGL((u3_tile_kernel<kC64, true, true>), dim3(tile_grid(ctx, u3_tile_kernel<kC64, true, true>, tiles)),
-                           dim3(kThreads), 0, ctx.stream, state, P);
-    else if (l1)
-        hipLaunchKernelGGL((u3_tile_kernel<kC64, true, false>),
-                           dim3(tile_grid(ctx, u3_tile_kernel<kC64, true, false>, tiles)), dim3(kThreads), 0,
-                           ctx.stream, state, P);
-    else if (l2)
-        hipLaunchKernelGGL((u3_tile_kernel<kC64, false, true>),
-                           dim3(tile_grid(ctx, u3_tile_kernel<kC64, false, true>, tiles)), dim3(kThreads), 0,
-                           ctx.stream, state, P);
-    else
-        hipLaunchKernelGGL((u3_tile_kernel<kC64, false, false>),
-                           dim3(tile_grid(ctx, u3_tile_kernel<kC64, false, false>, tiles)), dim3(kThreads), 0,
-                           ctx.stream, state, P);
+    auto k = u3_tile_kernel<kC64, kL1, kL2, kDiag>;
+    hipLaunchKernelGGL(k, dim3(tile_grid(ctx, k, P.ntiles)), dim3(kThreads), 0, ctx.stream, state, P, dtab, dcoef);
     return hip_check(hipGetLastError(), "u3_tile_kernel launch");
+}
+
+template <bool kC64, bool kDiag>
+int launch_tile_d(Ctx &ctx, void *state, const TilePass &P, bool l1, bool l2, const double *dtab, const double *dcoef)
+{
+    if (l1 && l2)
+        return launch_tile_t<kC64, true, true, kDiag>(ctx, state, P, dtab, dcoef);
+    if (l1)
+        return launch_tile_t<kC64, true, false, kDiag>(ctx, state, P, dtab, dcoef);
+    if (l2)
+        return launch_tile_t<kC64, false, true, kDiag>(ctx, state, P, dtab, dcoef);
+    return launch_tile_t<kC64, false, false, kDiag>(ctx, state, P, dtab, dcoef);
+}
+
+int launch_tile(Ctx &ctx, int prec, void *state, const TilePass &P, bool l1, bool l2, const double *dtab,
+                const double *dcoef)
+{
+    const bool diag = dtab != nullptr;
+    if (prec == C64)
+        return diag ? launch_tile_d<true, true>(ctx, state, P, l1, l2, dtab, dcoef)
+                    : launch_tile_d<true, false>(ctx, state, P, l1, l2, dtab, dcoef);
+    return diag ? launch_tile_d<false, true>(ctx, state, P, l1, l2, dtab, dcoef)
+                : launch_tile_d<false, false>(ctx, state, P, l1, l2, dtab, dcoef);
 }
 
 int layout_of(int prec, int tile_idx)
@@ -467,7 +604,8 @@ struct SingleGate
 };
 
 // One tile pass over the given targets (at most 4 below bit 4 and 8 above).
-int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs)
+int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
+                  const DiagGate *diag = nullptr, size_t ndiag = 0)
 {
     std::vector<unsigned> pos = {0, 1, 2, 3};
     std::vector<bool> is_target(64, false);
@@ -509,13 +647,39 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         }
         use[layout_of(prec, idx)] = true;
     }
+    const double *dtab = nullptr;
+    const double *dcoef = nullptr;
+    if (ndiag > 0)
+    {
+        // phase polynomial of the window, split into tile bits and the rest
+        Poly poly;
+        window_polynomial(n, diag, ndiag, poly);
+        std::vector<double> tab(kTileBits + kTileBits * kTileBits, 0.0);
+        int inner[kTileBits];
+        std::vector<int> outer;
+        for (int i = 0; i < kTileBits; i++)
+        {
+            inner[i] = int(pos[i]);
+            tab[i] = poly.L[pos[i]];
+            for (int j = i + 1; j < kTileBits; j++)
+                tab[kTileBits + i * kTileBits + j] = poly.q(int(pos[i]), int(pos[j]));
+        }
+        for (unsigned b = 0; b < n; b++)
+            if (std::find(pos.begin(), pos.end(), b) == pos.end())
+                outer.push_back(int(b));
+        void *d = nullptr;
+        ZXA_TRY(ctx.ring.stage(tab.data(), tab.size() * sizeof(double), ctx.stream, &d));
+        dtab = static_cast<const double *>(d);
+        ZXA_TRY(ctx.scratch_a.ensure(P.ntiles * 16 * sizeof(double)));
+        dcoef = static_cast<const double *>(ctx.scratch_a.ptr);
+        ZXA_TRY(launch_block_coef(ctx, poly, inner, kTileBits, outer.data(), int(outer.size()),
+                                  static_cast<double *>(ctx.scratch_a.ptr)));
+    }
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
-    if (prec == C64)
-        ZXA_TRY(launch_tile<true>(ctx, state, P, use[1], use[2]));
-    else
-        ZXA_TRY(launch_tile<false>(ctx, state, P, use[1], use[2]));
-    return ctx.toc(F_U3TILE, bytes);
+    ZXA_TRY(launch_tile(ctx, prec, state, P, use[1], use[2], dtab, dcoef));
+    ZXA_TRY(ctx.toc(ndiag > 0 ? F_FUSED : F_U3TILE, bytes));
+    return ctx.ring.done(ctx.stream);
 }
 
 } // namespace
@@ -587,6 +751,50 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
             ZXA_TRY(run_tile_pass(ctx, state, prec, n, pass));
     }
     return OK;
+}
+
+int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates,
+                     const DiagGate *diag, size_t ndiag)
+{
+    if (n < unsigned(kTileBits))
+    {
+        set_error("fused pass: state narrower than a tile");
+        return EINVAL_;
+    }
+    if (ndiag > 0 && !window_is_polynomial(n, diag, ndiag))
+    {
+        set_error("fused pass: diagonal window is not a phase polynomial");
+        return EINVAL_;
+    }
+    std::vector<SingleGate> gs;
+    int low = 0, high = 0;
+    for (size_t i = 0; i < ngates; i++)
+    {
+        const U3Gate &g = gates[i];
+        const uint64_t alpha = g.packed_sel & ((uint64_t(1) << 63) - 1);
+        if (g.delta == 0 || (g.delta & (g.delta - 1)) != 0 || alpha != g.delta || (g.delta >> n) != 0)
+        {
+            set_error("fused pass: gate is not a single-bit U3");
+            return EINVAL_;
+        }
+        const unsigned t = unsigned(__builtin_ctzll(g.delta));
+        for (const SingleGate &o : gs)
+            if (o.target == t)
+            {
+                set_error("fused pass: two gates on one target");
+                return EINVAL_;
+            }
+        (t < 4 ? low : high)++;
+        gs.push_back({t, &g});
+    }
+    if (high > 8)
+    {
+        set_error("fused pass: more than 8 targets above bit 3");
+        return EINVAL_;
+    }
+    if (gs.empty())
+        return diag_apply(ctx, state, prec, n, diag, ndiag);
+    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag);
 }
 
 int x_apply(Ctx &ctx, void *state, int prec, unsigned n, uint64_t delta, uint64_t sel)

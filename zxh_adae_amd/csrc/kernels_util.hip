@@ -1,0 +1,140 @@
+// kernels_util.hip -- streaming helpers: a device copy (the bandwidth and
+// counter calibration reference) and an in-place complex scale (renormalise /
+// apply a global phase).
+#include "dev.h"
+
+#include <algorithm>
+
+namespace zxa
+{
+namespace
+{
+
+constexpr int kThreads = 256;
+constexpr int kUnroll = 8;
+
+// 16-byte vectors, kUnroll independent loads in flight per thread.
+__global__ __launch_bounds__(kThreads) void copy_kernel(const float4 *__restrict__ src, float4 *__restrict__ dst,
+                                                        uint64_t n16)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kThreads * kUnroll;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kThreads * kUnroll + threadIdx.x; base < n16;
+         base += stride)
+    {
+        float4 v[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++)
+        {
+            const uint64_t i = base + uint64_t(u) * kThreads;
+            if (i < n16)
+                v[u] = src[i];
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++)
+        {
+            const uint64_t i = base + uint64_t(u) * kThreads;
+            if (i < n16)
+                dst[i] = v[u];
+        }
+    }
+}
+
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void scale_kernel(void *__restrict__ state, uint64_t n16, double re, double im)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kThreads * kUnroll;
+    for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * kThreads * kUnroll + threadIdx.x; base < n16;
+         base += stride)
+    {
+        if constexpr (!kC64)
+        {
+            double2 *p = static_cast<double2 *>(state);
+            double2 v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+            {
+                const uint64_t i = base + uint64_t(u) * kThreads;
+                if (i < n16)
+                    v[u] = p[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+            {
+                const uint64_t i = base + uint64_t(u) * kThreads;
+                if (i < n16)
+                    p[i] = make_double2(v[u].x * re - v[u].y * im, v[u].x * im + v[u].y * re);
+            }
+        }
+        else
+        {
+            float4 *p = static_cast<float4 *>(state);
+            const float r = static_cast<float>(re), m = static_cast<float>(im);
+            float4 v[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+            {
+                const uint64_t i = base + uint64_t(u) * kThreads;
+                if (i < n16)
+                    v[u] = p[i];
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++)
+            {
+                const uint64_t i = base + uint64_t(u) * kThreads;
+                if (i < n16)
+                    p[i] = make_float4(v[u].x * r - v[u].y * m, v[u].x * m + v[u].y * r, v[u].z * r - v[u].w * m,
+                                       v[u].z * m + v[u].w * r);
+            }
+        }
+    }
+}
+
+template <typename K>
+unsigned stream_grid(Ctx &ctx, K kernel, uint64_t n16)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 4;
+    const uint64_t cap = uint64_t(per_cu) * uint64_t(ctx.num_cus > 0 ? ctx.num_cus : 256);
+    const uint64_t need = (n16 + uint64_t(kThreads) * kUnroll - 1) / (uint64_t(kThreads) * kUnroll);
+    return static_cast<unsigned>(std::max<uint64_t>(1, std::min(cap, need)));
+}
+
+} // namespace
+
+int copy_apply(Ctx &ctx, void *dst, const void *src, size_t bytes)
+{
+    if (bytes % 16 != 0)
+    {
+        set_error("copy: size must be a multiple of 16 bytes");
+        return EINVAL_;
+    }
+    const uint64_t n16 = bytes / 16;
+    ZXA_TRY(ctx.tic());
+    hipLaunchKernelGGL(copy_kernel, dim3(stream_grid(ctx, copy_kernel, n16)), dim3(kThreads), 0, ctx.stream,
+                       static_cast<const float4 *>(src), static_cast<float4 *>(dst), n16);
+    ZXA_HIP(hipGetLastError());
+    return ctx.toc(F_COPY, 2ull * bytes);
+}
+
+int scale_apply(Ctx &ctx, void *state, int prec, unsigned n, double re, double im)
+{
+    const uint64_t bytes = (uint64_t(1) << n) * elem_bytes(prec);
+    if (bytes < 16)
+    {
+        set_error("scale: state smaller than 16 bytes");
+        return EINVAL_;
+    }
+    const uint64_t n16 = bytes / 16;
+    ZXA_TRY(ctx.tic());
+    if (prec == C64)
+        hipLaunchKernelGGL(scale_kernel<true>, dim3(stream_grid(ctx, scale_kernel<true>, n16)), dim3(kThreads), 0,
+                           ctx.stream, state, n16, re, im);
+    else
+        hipLaunchKernelGGL(scale_kernel<false>, dim3(stream_grid(ctx, scale_kernel<false>, n16)), dim3(kThreads), 0,
+                           ctx.stream, state, n16, re, im);
+    ZXA_HIP(hipGetLastError());
+    return ctx.toc(F_SCALE, 2ull * bytes);
+}
+
+} // namespace zxa

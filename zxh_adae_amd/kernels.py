@@ -69,6 +69,28 @@ class DeviceState:
                 arr[i].u[2 * k + 1] = flat[k].imag
         check(self.lib.zxh_k_u3_batch(self.ptr, self.prec, self.nbits, C.cast(arr, C.c_void_p), len(descs)))
 
+    @staticmethod
+    def _descs(descs: Sequence[tuple]):
+        arr = (U3Desc * max(len(descs), 1))()
+        for i, (sel, delta, u) in enumerate(descs):
+            arr[i].packed_sel = sel
+            arr[i].delta = delta
+            flat = np.asarray(u, dtype=np.complex128).reshape(4)
+            for k in range(4):
+                arr[i].u[2 * k] = flat[k].real
+                arr[i].u[2 * k + 1] = flat[k].imag
+        return arr
+
+    def fused_pass(self, descs: Sequence[tuple], sel0: Sequence[int], sel1: Sequence[int], theta: Sequence[float]):
+        """U3 descriptors then a diagonal window, in one tile pass."""
+        arr = self._descs(descs)
+        s0 = np.ascontiguousarray(sel0, dtype=np.uint64)
+        s1 = np.ascontiguousarray(sel1, dtype=np.uint64)
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        check(self.lib.zxh_k_fused_pass(self.ptr, self.prec, self.nbits, C.cast(arr, C.c_void_p), len(descs),
+                                        s0.ctypes.data_as(_lib._u64p), s1.ctypes.data_as(_lib._u64p),
+                                        th.ctypes.data_as(_lib._dp), th.size))
+
     def diag(self, sel0: Sequence[int], sel1: Sequence[int], theta: Sequence[float]):
         s0 = np.ascontiguousarray(sel0, dtype=np.uint64)
         s1 = np.ascontiguousarray(sel1, dtype=np.uint64)
@@ -102,6 +124,13 @@ class DeviceState:
                                                out.ctypes.data_as(_lib._u64p)))
         return out[: thr.size]
 
+    def scale(self, re: float, im: float = 0.0):
+        check(self.lib.zxh_k_scale(self.ptr, self.prec, self.nbits, re, im))
+
+    def copy_from(self, other: "DeviceState"):
+        assert other.nbytes == self.nbytes
+        check(self.lib.zxh_k_copy(self.ptr, other.ptr, self.nbytes))
+
     def prob_total(self) -> float:
         out = C.c_double()
         check(self.lib.zxh_k_prob_total(self.ptr, self.prec, self.nbits, C.byref(out)))
@@ -112,7 +141,8 @@ def sync():
     check(_lib.load().zxh_dev_sync())
 
 
-FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6}
+FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6, "copy": 7,
+            "scale": 8, "fused": 9}
 
 
 def set_timing(on: bool):
