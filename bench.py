@@ -2,14 +2,18 @@
 
 Step: one execute() of the 30-qubit QFT-structured circuit (H on every
 target followed by CP(pi/2^k) to all lower qubits, no swaps: 30 H + 435 CP)
-in complex128 through the engine's drop-in ZXH object, with the reference's
-flush rules (a U3 pass per H, one diagonal sweep per CP window).  The state is
-resident in HBM; execute() includes the reset of the 16 GiB state, as the
-reference's execute() does.  value = gates / second of the timed steps.
+in complex128 through the engine's drop-in ZXH object with the default fused
+policy: the reference's own U3 clusters and diagonal windows, merged into
+tile passes where they commute.  The state is resident in HBM; execute()
+includes the reset of the 16 GiB state, as the reference's execute() does.
+value = gates / second of the timed steps.
 
 Reported beside it (same JSON line):
   roofline      dominant kernel of the step: algorithmic bytes per launch /
                 average launch time (HIP events on the launch stream)
+  reference_policy  the same step with one pass per flushed batch, as the
+                reference runs it (30 U3 passes + 29 diagonal sweeps)
+  autotune      the diagonal-sweep variant chosen on the resident state
   diag_batches  one fused sweep per batch of B random P/CP gates, B in
                 {1,4,16,64,256}, uniform-superposition input, c128 and c64
   u3_all        U3 on all 30 qubits as tile passes
@@ -39,7 +43,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def family_stats(sim, families=("diag", "u3_tile", "u3_pair", "reset", "cp", "perm")):
+def family_stats(sim, families=("diag", "u3_tile", "fused", "u3_pair", "reset", "cp", "perm")):
     out = {}
     for f in families:
         t = sim.timing(f)
@@ -60,9 +64,11 @@ def roofline_of(stats):
                  "launches": t["launches"]}
 
 
-def qft_step(Z, W, prec, steps, warmup):
+def qft_step(Z, W, prec, steps, warmup, policy="fused", sim=None):
     gates = W.qft(NQ)
-    sim = Z.ZXH(NQ, False, True, precision=prec)
+    if sim is None:
+        sim = Z.ZXH(NQ, False, True, precision=prec, policy=policy)
+    sim.clear_gates()
     sim.gates(gates)
     for i in range(warmup):
         sim.execute()
@@ -80,17 +86,16 @@ def qft_step(Z, W, prec, steps, warmup):
     return sim, len(gates), dt / steps, stats
 
 
-def diag_batches(Z, W, K, sim_state, prec, repeats=5):
+def wrap_state(sim_state, prec):
+    from zxh_adae_amd.kernels import DeviceState
     ptr, m = sim_state
-    import numpy as np
-    from zxh_adae_amd.kernels import DeviceState, NO_SEL
-    st = DeviceState.__new__(DeviceState)
-    st.lib = Z._core._lib.load()
-    st.nbits = m
-    st.prec = 0 if prec == "c128" else 1
-    st.ptr = __import__("ctypes").c_void_p(ptr)
-    st.dtype = np.complex128 if prec == "c128" else np.complex64
-    st.nbytes = (1 << m) * (16 if prec == "c128" else 8)
+    return DeviceState.wrap(ptr, m, prec)
+
+
+def diag_batches(Z, W, K, sim_state, prec, repeats=5):
+    from zxh_adae_amd.kernels import NO_SEL
+    st = wrap_state(sim_state, prec)
+    m = st.nbits
     out = {}
     for B in (1, 4, 16, 64, 256):
         gs = W.diag_batch(m, B, 100 + B)
@@ -108,20 +113,12 @@ def diag_batches(Z, W, K, sim_state, prec, repeats=5):
         gbps = t["bytes"] / t["launches"] / (avg_ms * 1e-3) / 1e9
         out[str(B)] = {"ms": round(avg_ms, 4), "GB/s": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBPS, 4),
                        "gates/s": round(B / (avg_ms * 1e-3), 1)}
-    st.ptr = None  # owned by the simulator
     return out
 
 
 def u3_all(Z, W, K, sim_state, prec, repeats=3):
-    import ctypes
-    import numpy as np
-    from zxh_adae_amd.kernels import DeviceState
-    ptr, m = sim_state
-    st = DeviceState.__new__(DeviceState)
-    st.lib = Z._core._lib.load()
-    st.nbits = m
-    st.prec = 0 if prec == "c128" else 1
-    st.ptr = ctypes.c_void_p(ptr)
+    st = wrap_state(sim_state, prec)
+    m = st.nbits
     descs = [((1 << g[2]), (1 << g[2]), W.u3_matrix(g[3], g[4], g[5])) for g in W.u3_all(m, 77)]
     st.u3_batch(descs)
     K.reset_timing()
@@ -133,7 +130,6 @@ def u3_all(Z, W, K, sim_state, prec, repeats=3):
     wall = (time.perf_counter() - t0) / repeats
     t = K.timing("u3_tile")
     K.set_timing(False)
-    st.ptr = None
     passes = t["launches"] / repeats
     kern_ms = t["ms"] / repeats
     gbps = t["bytes"] / t["launches"] / (t["ms"] / t["launches"] * 1e-3) / 1e9
@@ -223,8 +219,17 @@ def main():
         print(json.dumps({"kernels": kernels_only(Z, K, W, args.steps, args.kernels), "precision": args.kernels,
                           "qubits": NQ}), flush=True)
         return
-    log(f"QFT-{NQ} c128: warmup {args.warmup}, steps {args.steps}")
-    sim, ngates, t_step, stats = qft_step(Z, W, "c128", args.steps, args.warmup)
+    from zxh_adae_amd import autotune
+
+    # the resident 30-qubit state; the autotuner picks the diagonal-sweep variant on it
+    sim = Z.ZXH(NQ, False, True, precision="c128", policy="fused")
+    sim.gates(W.hadamard_all(NQ))
+    sim.execute()
+    log("autotune diag c128")
+    tuned = {"c128": autotune.tune_diag(wrap_state(sim.device_state(), "c128"))}
+
+    log(f"QFT-{NQ} c128 fused: warmup {args.warmup}, steps {args.steps}")
+    sim, ngates, t_step, stats = qft_step(Z, W, "c128", args.steps, args.warmup, sim=sim)
     fam, roof = roofline_of(stats)
     log(f"step {t_step * 1e3:.2f} ms, {ngates / t_step:.1f} gates/s; dominant {fam}: "
         f"{roof['achieved']:.0f} GB/s ({roof['frac']:.1%})")
@@ -233,12 +238,19 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(t_step * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"qft{NQ}_c128_execute", "qubits": NQ, "gates": ngates, "state": "complex128",
-                   "method": "plain (eager_expand_all)", "policy": "reference flush rules"},
+                   "method": "plain (eager_expand_all)", "policy": "fused"},
         "roofline": roof,
         "kernels": {k: {"launches": v["launches"], "avg_ms": round(v["ms"] / v["launches"], 4),
                         "GB/s": round(v["bytes"] / v["launches"] / (v["ms"] / v["launches"] * 1e-3) / 1e9, 1)}
                     for k, v in stats.items()},
+        "autotune": tuned,
     }
+    log(f"QFT-{NQ} c128 reference policy")
+    simr = Z.ZXH(NQ, False, True, precision="c128", policy="reference")
+    simr, _, t_ref, stats_ref = qft_step(Z, W, "c128", max(1, args.steps // 2), 1, sim=simr)
+    result["reference_policy"] = {"gates/s": round(ngates / t_ref, 3), "ms_per_step": round(t_ref * 1e3, 3),
+                                  "launches": {k: v["launches"] for k, v in stats_ref.items()}}
+    del simr
 
     if not args.no_extras:
         # uniform superposition input for the diagonal batches and the U3 pass
@@ -251,13 +263,19 @@ def main():
         result["u3_all"] = {"c128": u3_all(Z, W, K, sim.device_state(), "c128")}
         log("sampling 1e6 shots")
         sim.set_seed(5)
+        sim.set_timing(True)
         t0 = time.perf_counter()
         sim.measure(1000000)
         result["sampling"] = {"shots": 1000000, "qubits": NQ, "seconds": round(time.perf_counter() - t0, 4),
-                              "kernel_ms": round(sim.timing("sample")["ms"], 3)}
+                              "search_kernel_ms": round(sim.timing("sample")["ms"], 3)}
+        sim.set_timing(False)
         del sim
         log(f"QFT-{NQ} c64")
-        sim64, _, t64, stats64 = qft_step(Z, W, "c64", args.steps, 1)
+        sim64 = Z.ZXH(NQ, False, True, precision="c64", policy="fused")
+        sim64.gates(W.hadamard_all(NQ))
+        sim64.execute()
+        tuned["c64"] = autotune.tune_diag(wrap_state(sim64.device_state(), "c64"))
+        sim64, _, t64, stats64 = qft_step(Z, W, "c64", args.steps, 1, sim=sim64)
         result["c64"] = {"gates/s": round(ngates / t64, 3), "ms_per_step": round(t64 * 1e3, 3)}
         sim64.clear_gates()
         sim64.gates(W.hadamard_all(NQ))

@@ -194,6 +194,13 @@ int zxh_k_copy(void *dst, const void *src, size_t bytes);
 /* state *= (re + i im) in place: renormalisation or a global phase (no
  * reference counterpart) */
 int zxh_k_scale(void *state, int precision, size_t local_bits, double re, double im);
+/* tuning knobs (process-wide), set by the autotuner (zxh_adae_amd.autotune):
+ *   "diag.ept.c128", "diag.ept.c64": 0 = persistent diagonal sweep,
+ *                                    1, 2, 4 = flat grid, 16-byte units per thread
+ *   "diag.nt.c128", "diag.nt.c64": nontemporal loads/stores in the flat
+ *                                  sweep (0/1) */
+int zxh_tune_set(const char *name, int value);
+int zxh_tune_get(const char *name, int *value);
 /* HIP-event timing of the kernel-level calls (same families as zxh_get_timing,
  * plus 7 copy and 8 scale) */
 int zxh_k_set_timing(int on);

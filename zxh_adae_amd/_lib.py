@@ -81,6 +81,8 @@ _SIGS = [
     ("zxh_k_prob_total", C.c_int, [_vp, C.c_int, _sz, _dp]),
     ("zxh_k_copy", C.c_int, [_vp, _vp, _sz]),
     ("zxh_k_scale", C.c_int, [_vp, C.c_int, _sz, C.c_double, C.c_double]),
+    ("zxh_tune_set", C.c_int, [C.c_char_p, C.c_int]),
+    ("zxh_tune_get", C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
     ("zxh_k_set_timing", C.c_int, [C.c_int]),
     ("zxh_k_get_timing", C.c_int, [C.c_int, _u64p, _dp, _u64p]),
     ("zxh_k_reset_timing", C.c_int, []),

@@ -490,6 +490,24 @@ int zxh_k_scale(void *state, int precision, size_t local_bits, double re, double
     return finish(c, zxa::scale_apply(*c, state, precision, unsigned(local_bits), re, im));
 }
 
+int zxh_tune_set(const char *name, int value)
+{
+    if (!name)
+        return fail(ZXH_EINVAL, "null knob name");
+    return zxa::tune_set(name, value);
+}
+
+int zxh_tune_get(const char *name, int *value)
+{
+    if (!name || !value)
+        return fail(ZXH_EINVAL, "null argument");
+    const int v = zxa::tune_get(name);
+    if (v < 0)
+        return fail(ZXH_EINVAL, "unknown tuning knob");
+    *value = v;
+    return ZXH_OK;
+}
+
 int zxh_k_set_timing(int on)
 {
     WITH_CTX(c);

@@ -240,6 +240,51 @@ Ctx *default_ctx(int *rc)
     return ctx;
 }
 
+namespace
+{
+struct Knob
+{
+    const char *name;
+    int value;
+    int lo, hi;
+};
+Knob g_knobs[] = {
+    {"diag.ept.c128", 2, 0, 4},
+    {"diag.ept.c64", 2, 0, 4},
+    {"diag.nt.c128", 1, 0, 1},
+    {"diag.nt.c64", 1, 0, 1},
+};
+std::mutex g_knob_mu;
+} // namespace
+
+int tune_get(const char *name)
+{
+    std::lock_guard<std::mutex> lk(g_knob_mu);
+    for (const Knob &k : g_knobs)
+        if (std::strcmp(k.name, name) == 0)
+            return k.value;
+    return -1;
+}
+
+int tune_set(const char *name, int value)
+{
+    std::lock_guard<std::mutex> lk(g_knob_mu);
+    for (Knob &k : g_knobs)
+        if (std::strcmp(k.name, name) == 0)
+        {
+            const bool ept_knob = std::strncmp(name, "diag.ept.", 9) == 0;
+            if (value < k.lo || value > k.hi || (ept_knob && value == 3))
+            {
+                set_error(std::string("tuning value out of range for ") + name);
+                return EINVAL_;
+            }
+            k.value = value;
+            return OK;
+        }
+    set_error(std::string("unknown tuning knob ") + name);
+    return EINVAL_;
+}
+
 // SplitMix64 finaliser (measure_general.cpp:16-22).
 uint64_t mix_seed(uint64_t base, uint64_t salt)
 {

@@ -167,6 +167,13 @@ struct Ctx
     int take(hipEvent_t *e);
 };
 
+// Tuning knobs (process-wide): variant choices the autotuner sweeps.
+//   diag.ept.c128 / diag.ept.c64  0 = persistent sweep, 1/2/4 = flat grid with
+//                                 that many 16-byte units per thread
+//   diag.nt.c128 / diag.nt.c64     nontemporal loads/stores in the flat sweep
+int tune_get(const char *name);
+int tune_set(const char *name, int value);
+
 // Process-wide context for the kernel-level entry points.
 Ctx *default_ctx(int *rc);
 

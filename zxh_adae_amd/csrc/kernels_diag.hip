@@ -251,6 +251,138 @@ __global__ __launch_bounds__(kThreads) void diag_poly_kernel(void *__restrict__ 
     }
 }
 
+// Flat-grid variant: one workgroup per chunk of 256*EPT sixteen-byte units
+// (4 waves x EPT rows of 1 KiB), launched as one workgroup per chunk so that
+// the hardware dispatcher walks the state in address order.  Same phase
+// factorisation: lane phase (per lane and chunk), row phase (per row, one
+// sincos by lane t, broadcast), and a chunk-independent lane x row factor
+// read from a small table (L1/L2-resident).
+// tabs: av[4][2^LB] | brow[4*EPT] | g[2^LB][EPT] (complex, re/im pairs)
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// 16-byte loads/stores, optionally with the nontemporal hint.
+template <bool kNT>
+__device__ __forceinline__ double2 ld16(const double2 *p)
+{
+    const dvec2 v = kNT ? __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(p))
+                        : *reinterpret_cast<const dvec2 *>(p);
+    return make_double2(v.x, v.y);
+}
+template <bool kNT>
+__device__ __forceinline__ void st16(double2 a, double2 *p)
+{
+    dvec2 v;
+    v.x = a.x;
+    v.y = a.y;
+    if (kNT)
+        __builtin_nontemporal_store(v, reinterpret_cast<dvec2 *>(p));
+    else
+        *reinterpret_cast<dvec2 *>(p) = v;
+}
+template <bool kNT>
+__device__ __forceinline__ float4 ld16(const float4 *p)
+{
+    const fvec4 v = kNT ? __builtin_nontemporal_load(reinterpret_cast<const fvec4 *>(p))
+                        : *reinterpret_cast<const fvec4 *>(p);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <bool kNT>
+__device__ __forceinline__ void st16(float4 a, float4 *p)
+{
+    fvec4 v;
+    v.x = a.x;
+    v.y = a.y;
+    v.z = a.z;
+    v.w = a.w;
+    if (kNT)
+        __builtin_nontemporal_store(v, reinterpret_cast<fvec4 *>(p));
+    else
+        *reinterpret_cast<fvec4 *>(p) = v;
+}
+
+template <bool kC64, int kEPT, bool kNT>
+__global__ __launch_bounds__(kThreads) void diag_flat_kernel(void *__restrict__ state, const double *__restrict__ tabs,
+                                                             const double *__restrict__ coef)
+{
+    constexpr int EPL = kC64 ? 2 : 1;
+    constexpr int LB = 6 + (kC64 ? 1 : 0);
+    constexpr int RB = 2 + (kEPT == 1 ? 0 : kEPT == 2 ? 1 : kEPT == 4 ? 2 : 3);
+    constexpr int NCO = LB + RB;
+    const double *av = tabs;
+    const double *brow = av + 4 * (1 << LB);
+    const double *gt = brow + 4 * kEPT;
+
+    const int lane = threadIdx.x & 63;
+    const int v = threadIdx.x >> 6;
+    const uint64_t chunk = blockIdx.x;
+    const size_t base = static_cast<size_t>(chunk) << (LB + RB);
+    const double *cc = coef + chunk * kCoefStride;
+
+    // loads first: they do not depend on the phases
+    float4 d4[kEPT];
+    double2 d2[kEPT];
+#pragma unroll
+    for (int t = 0; t < kEPT; t++)
+    {
+        const size_t row = static_cast<size_t>(v * kEPT + t);
+        if constexpr (kC64)
+            d4[t] = ld16<kNT>(static_cast<const float4 *>(state) + ((base + (row << LB)) >> 1) + lane);
+        else
+            d2[t] = ld16<kNT>(static_cast<const double2 *>(state) + base + (row << LB) + lane);
+    }
+    // row phase of row v*EPT + t, computed by lane t of the wave
+    const int xw_own = v * kEPT + (lane & (kEPT - 1));
+    double rang = brow[xw_own] + cc[NCO];
+#pragma unroll
+    for (int k = 0; k < RB; k++)
+        if ((xw_own >> k) & 1)
+            rang += cc[LB + k];
+    const cd R = cis(rang);
+#pragma unroll
+    for (int e = 0; e < EPL; e++)
+    {
+        const int xa = lane * EPL + e;
+        double ang = av[v * (1 << LB) + xa];
+#pragma unroll
+        for (int j = 0; j < LB; j++)
+            if ((xa >> j) & 1)
+                ang += cc[j];
+        const cd E = cis(ang);
+#pragma unroll
+        for (int t = 0; t < kEPT; t++)
+        {
+            const double *g = gt + 2 * (xa * kEPT + t);
+            const cd rt = {readlane_d(R.x, t), readlane_d(R.y, t)};
+            const cd ph = cmul(cmul(E, {g[0], g[1]}), rt);
+            if constexpr (kC64)
+            {
+                const float px = static_cast<float>(ph.x), py = static_cast<float>(ph.y);
+                float &re = e == 0 ? d4[t].x : d4[t].z;
+                float &im = e == 0 ? d4[t].y : d4[t].w;
+                const float a = re, b = im;
+                re = a * px - b * py;
+                im = a * py + b * px;
+            }
+            else
+            {
+                const double a = d2[t].x, b = d2[t].y;
+                d2[t].x = a * ph.x - b * ph.y;
+                d2[t].y = a * ph.y + b * ph.x;
+            }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < kEPT; t++)
+    {
+        const size_t row = static_cast<size_t>(v * kEPT + t);
+        if constexpr (kC64)
+            st16<kNT>(d4[t], static_cast<float4 *>(state) + ((base + (row << LB)) >> 1) + lane);
+        else
+            st16<kNT>(d2[t], static_cast<double2 *>(state) + base + (row << LB) + lane);
+    }
+}
+
 __device__ __forceinline__ bool sel_eval(uint64_t packed, uint64_t x)
 {
     const uint64_t alpha = packed & ((uint64_t(1) << 63) - 1);
@@ -301,6 +433,106 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t work_items)
         per_cu = 2;
     const uint64_t cap = static_cast<uint64_t>(per_cu) * static_cast<uint64_t>(ctx.num_cus > 0 ? ctx.num_cus : 256);
     return static_cast<int>(work_items < cap ? work_items : cap);
+}
+
+template <bool kC64, int kEPT>
+int launch_flat(Ctx &ctx, void *state, const double *tabs, const double *coef, uint64_t nchunks, bool nt)
+{
+    if (nt)
+        hipLaunchKernelGGL((diag_flat_kernel<kC64, kEPT, true>), dim3(static_cast<unsigned>(nchunks)), dim3(kThreads),
+                           0, ctx.stream, state, tabs, coef);
+    else
+        hipLaunchKernelGGL((diag_flat_kernel<kC64, kEPT, false>), dim3(static_cast<unsigned>(nchunks)),
+                           dim3(kThreads), 0, ctx.stream, state, tabs, coef);
+    return hip_check(hipGetLastError(), "diag_flat_kernel launch");
+}
+
+// Flat-grid diagonal sweep with EPT sixteen-byte units per thread.
+int diag_flat(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates, int ept, bool nt)
+{
+    const bool c64 = prec == C64;
+    const int LB = c64 ? 7 : 6;
+    const int RB = 2 + (ept == 1 ? 0 : ept == 2 ? 1 : 2);
+    const int NCO = LB + RB;
+    const int nh = static_cast<int>(n) - NCO;
+    Poly P;
+    window_polynomial(n, gates, ngates, P);
+    const int nlane = 1 << LB;
+    std::vector<double> blk;
+    // av[v][xa]: lane part incl. the two wave-select row bits (row bits RB-2, RB-1)
+    const int tb = RB - 2;
+    for (int vv = 0; vv < 4; vv++)
+        for (int xa = 0; xa < nlane; xa++)
+        {
+            double s = 0.0;
+            for (int j = 0; j < LB; j++)
+            {
+                if (!((xa >> j) & 1))
+                    continue;
+                s += P.L[j];
+                for (int k = j + 1; k < LB; k++)
+                    if ((xa >> k) & 1)
+                        s += P.q(j, k);
+                for (int w = 0; w < 2; w++)
+                    if ((vv >> w) & 1)
+                        s += P.q(j, LB + tb + w);
+            }
+            blk.push_back(s);
+        }
+    // brow[xw]: row part within the chunk
+    for (int xw = 0; xw < (1 << RB); xw++)
+    {
+        double s = 0.0;
+        for (int k = 0; k < RB; k++)
+        {
+            if (!((xw >> k) & 1))
+                continue;
+            s += P.L[LB + k];
+            for (int l = k + 1; l < RB; l++)
+                if ((xw >> l) & 1)
+                    s += P.q(LB + k, LB + l);
+        }
+        blk.push_back(s);
+    }
+    // g[xa][t]: exp(i sum_{k < tb} t_k Q(xa bits, row bit k)), chunk-independent
+    for (int xa = 0; xa < nlane; xa++)
+        for (int t = 0; t < ept; t++)
+        {
+            double a = 0.0;
+            for (int j = 0; j < LB; j++)
+                if ((xa >> j) & 1)
+                    for (int k = 0; k < tb; k++)
+                        if ((t >> k) & 1)
+                            a += P.q(j, LB + k);
+            blk.push_back(std::cos(a));
+            blk.push_back(std::sin(a));
+        }
+    void *dblk = nullptr;
+    ZXA_TRY(ctx.ring.stage(blk.data(), blk.size() * sizeof(double), ctx.stream, &dblk));
+    std::vector<int> inner(NCO), outer(nh);
+    for (int j = 0; j < NCO; j++)
+        inner[j] = j;
+    for (int i = 0; i < nh; i++)
+        outer[i] = NCO + i;
+    const uint64_t nchunks = uint64_t(1) << nh;
+    ZXA_TRY(ctx.scratch_a.ensure(nchunks * kCoefStride * sizeof(double)));
+    double *coef = static_cast<double *>(ctx.scratch_a.ptr);
+    ZXA_TRY(launch_block_coef(ctx, P, inner.data(), NCO, outer.data(), nh, coef));
+    const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
+    const double *tabs = static_cast<const double *>(dblk);
+    ZXA_TRY(ctx.tic());
+    int rc;
+    if (c64)
+        rc = ept == 1 ? launch_flat<true, 1>(ctx, state, tabs, coef, nchunks, nt)
+             : ept == 2 ? launch_flat<true, 2>(ctx, state, tabs, coef, nchunks, nt)
+                        : launch_flat<true, 4>(ctx, state, tabs, coef, nchunks, nt);
+    else
+        rc = ept == 1 ? launch_flat<false, 1>(ctx, state, tabs, coef, nchunks, nt)
+             : ept == 2 ? launch_flat<false, 2>(ctx, state, tabs, coef, nchunks, nt)
+                        : launch_flat<false, 4>(ctx, state, tabs, coef, nchunks, nt);
+    ZXA_TRY(rc);
+    ZXA_TRY(ctx.toc(F_DIAG, bytes));
+    return ctx.ring.done(ctx.stream);
 }
 
 int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
@@ -510,6 +742,14 @@ int diag_apply(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gate
         return EINVAL_;
     }
     const bool simple = window_is_polynomial(n, gates, ngates);
+    const int ept = tune_get(prec == C64 ? "diag.ept.c64" : "diag.ept.c128");
+    if (simple && ept > 0)
+    {
+        const unsigned need = (prec == C64 ? 7u : 6u) + 2u + (ept == 1 ? 0u : ept == 2 ? 1u : 2u);
+        if (n >= need && (n - need) < 32)
+            return diag_flat(ctx, state, prec, n, gates, ngates, ept,
+                             tune_get(prec == C64 ? "diag.nt.c64" : "diag.nt.c128") != 0);
+    }
     const unsigned need = (prec == C64 ? Geo<true>::CB : Geo<false>::CB);
     if (simple && n >= need)
         return diag_fast(ctx, state, prec, n, gates, ngates);

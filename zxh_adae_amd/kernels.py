@@ -38,10 +38,23 @@ class DeviceState:
         check(self.lib.zxh_dev_alloc(self.nbytes, C.byref(p)))
         self.ptr = p
 
+    @classmethod
+    def wrap(cls, ptr: int, nbits: int, precision: str = "c128") -> "DeviceState":
+        """A non-owning view of an existing device state (e.g. a simulator's)."""
+        self = cls.__new__(cls)
+        self.lib = _lib.load()
+        self.nbits = int(nbits)
+        self.prec = _lib.ZXH_C128 if precision in ("c128", "complex128") else _lib.ZXH_C64
+        self.dtype = np.complex128 if self.prec == _lib.ZXH_C128 else np.complex64
+        self.nbytes = (1 << self.nbits) * np.dtype(self.dtype).itemsize
+        self.ptr = C.c_void_p(ptr)
+        self._owned = False
+        return self
+
     def free(self):
-        if getattr(self, "ptr", None):
+        if getattr(self, "ptr", None) and getattr(self, "_owned", True):
             self.lib.zxh_dev_free(self.ptr)
-            self.ptr = None
+        self.ptr = None
 
     def __del__(self):
         self.free()
