@@ -270,16 +270,25 @@ def test_mcx_is_exact_permutation(gpu_engine, n, ctrls, target):
 # ---------------- fused tile pass (U3 batch then diagonal window) ----------------
 
 @pytest.mark.parametrize("n,low,high,ndiag", [(12, [0, 2], [5, 11], 30), (13, [0, 1, 2, 3], list(range(4, 12)), 100),
+                                             (13, [0, 2, 4], [5, 12], 30), (14, [], list(range(6, 14)), 50),
                                              (16, [], [15, 14, 4, 9], 64), (20, [1], [19, 12, 7], 256),
-                                             (14, [3], [], 10), (15, [], [], 40)])
+                                             (14, [3], [], 10), (15, [], [], 40), (18, [4], [17], 0)])
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 def test_fused_pass_matches_u3_then_diag(gpu_engine, n, low, high, ndiag, prec):
+    """Targets below the tile's low bits (4 for complex128, 5 for complex64)
+    ride free; at most 8 others; the tile needs 12 / 13 address bits."""
     rng = np.random.default_rng(n * 13 + ndiag)
     psi = rand_state(rng, n)
     descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in low + high]
-    s0, s1, th = diag_window(rng, n, ndiag)
-    want = oracle_diag(oracle_u3(psi, n, descs) if descs else psi, n, s0, s1, th)
+    s0, s1, th = diag_window(rng, n, ndiag) if ndiag else ([], [], [])
+    want = oracle_diag(oracle_u3(psi, n, descs) if descs else psi, n, s0, s1, th) if ndiag else oracle_u3(psi, n, descs)
     d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    lowbits = 5 if prec == "c64" else 4
+    if n < lowbits + 8 or sum(q >= lowbits for q in low + high) > 8:
+        from zxh_adae_amd import ZXHError
+        with pytest.raises(ZXHError):
+            d.fused_pass(descs, s0, s1, th)
+        return
     d.fused_pass(descs, s0, s1, th)
     got = d.download().astype(np.complex128)
     assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)

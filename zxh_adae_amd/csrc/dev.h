@@ -197,11 +197,15 @@ struct U3Gate
     double u[8];
 };
 
+// Tile geometry: a tile pass reaches every target below tile_low_bits(prec)
+// and up to eight more; it needs a state of at least tile_min_bits(prec).
+unsigned tile_low_bits(int prec);
+unsigned tile_min_bits(int prec);
 // Applies commuting gates on distinct targets (a fused U3 cluster).
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates);
-// One tile pass applying commuting single-bit U3 gates (alpha == delta, at
-// most 4 targets below bit 4 and 8 above), then a diagonal window whose
-// selectors have at most one alpha bit, in one sweep.
+// One tile pass applying commuting single-bit U3 gates (alpha == delta, any
+// number of targets below tile_low_bits and at most 8 at or above), then a
+// diagonal window whose selectors have at most one alpha bit, in one sweep.
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
                      const DiagGate *diag, size_t ndiag);
 // One general pair sweep (any delta / selector).

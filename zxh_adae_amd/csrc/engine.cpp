@@ -338,7 +338,7 @@ int Sim::flush_diag(std::vector<DiagGate> &window)
 
 bool Sim::fusable() const
 {
-    return policy_ == POLICY_FUSED && used_bits_ >= 12;
+    return policy_ == POLICY_FUSED && used_bits_ >= tile_min_bits(prec_);
 }
 
 // Runs the pending fused pass: its U3 gates, then its diagonal window.
@@ -357,8 +357,8 @@ int Sim::flush_pass()
 
 // Adds single-bit U3 batches to the pending pass when exact: a gate joins
 // only if it commutes with the pass's diagonal window (its target is outside
-// the window's support), and the tile has room (targets below bit 4 always
-// fit, at most 8 above).  A second gate on a pass target is composed with
+// the window's support), and the tile has room (targets below the tile's low
+// bits always fit, at most 8 others).  A second gate on a pass target is composed with
 // the first (applied after it).
 int Sim::emit_u3(std::vector<U3Gate> &descs)
 {
@@ -407,7 +407,7 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
             }
             continue;
         }
-        const bool high = g.delta >= 16;
+        const bool high = g.delta >> tile_low_bits(prec_) != 0;
         if (high && pass_high_ >= 8)
             ZXA_TRY(flush_pass());
         pass_u3_.push_back(g);

@@ -1,0 +1,1009 @@
+// kernels_tile.hip -- tile passes: fused single-qubit batches, optionally
+// followed by a diagonal window, in one read and one write of the state.
+//
+// The reference applies a fused U3 cluster as one full-state sweep per
+// descriptor (u3_block_batch_kernel_cuda, src/zxh/src/kernel/kernel_cuda.cu:390-423;
+// CPU kernel_general.cpp:173-202) and a diagonal window as a further sweep
+// (diag_pending_flush, diag_cuda.cu:260-281).  Here a pass works on tiles.
+//
+// A tile is 4096 16-byte units (complex128: one amplitude; complex64: two
+// amplitudes adjacent in address bit 0) whose unit addresses differ in 12
+// chosen "unit bits": the four lowest unit-address bits, so that each 16-lane
+// group of a wave moves 256 contiguous bytes, plus up to eight target bits.
+// A complex64 tile therefore spans address bits 0..4 and eight more.  Each
+// workgroup streams through tiles; each tile is loaded once, every gate whose
+// target lies in the tile is applied, the diagonal epilogue multiplies each
+// amplitude by its phase, and the tile is written back.
+//
+// Inside a tile each of the 256 threads holds 16 units.  Gates act on
+// register pairs; tile bits that are thread bits in the load layout are
+// reached by transposing the tile through LDS (XOR-swizzled: conflict-free
+// for the three layouts used) into a layout whose register bits they are.
+//
+// Streaming: the grid is persistent (two workgroups per CU, LDS-bound); a
+// workgroup visits tiles blockIdx, blockIdx + grid, ... so the tiles in
+// flight at any moment are consecutive, and it issues the loads of its next
+// tile before working on the current one, which keeps HBM busy while the
+// gates, transposes and phases run.  Loads and stores are nontemporal.
+#include "dev.h"
+#include "poly.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+namespace zxa
+{
+namespace
+{
+
+constexpr int kThreads = 256;
+constexpr int kUnitBits = 12;
+constexpr int kUnits = 1 << kUnitBits;
+constexpr int kRegs = 16;
+constexpr int kIntra = kUnitBits; // gate slot of the complex64 in-unit bit
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// Read-only tables are read through the constant address space: the loads
+// become scalar (s_load, counted by lgkmcnt) instead of uniform vector loads,
+// which would share vmcnt with the in-flight tile loads and force waits on
+// them.
+#define ZXA_CONST const __attribute__((address_space(4)))
+
+template <bool kC64>
+struct UnitT;
+template <>
+struct UnitT<false>
+{
+    using T = dvec2;
+};
+template <>
+struct UnitT<true>
+{
+    using T = fvec4;
+};
+
+struct TGate
+{
+    double u[8];    // m00 m01 m10 m11 as (re, im)
+    float uf[8];    // the same in float for complex64 states
+    uint64_t amask; // selector bits besides the target, unit-address space
+    uint32_t a0;    // complex64: selector bit 0 (differs between a unit's two amplitudes)
+    uint32_t beta;
+    uint32_t rpar;  // bit r: parity of amask over register r's offset in the gate's layout
+    uint32_t pad;
+};
+
+// Kernel arguments of a pass.  Unit bits 0..3 are unit-address bits 0..3;
+// the host assigns the other eight freely (targets first into the register
+// bits of layout 0, then of layout 1), so ub is not sorted; sorted holds the
+// same positions ascending.
+struct TPass
+{
+    uint32_t ub[kUnitBits];     // unit-address bit of each unit bit
+    uint32_t sorted[kUnitBits]; // the same, ascending
+    uint64_t ntiles;
+    uint64_t tmask; // unit-address mask of the 12 unit bits
+    uint64_t gstep; // the grid size deposited into the non-tile bits
+};
+
+// Per-pass tables in device memory.  The kernel re-reads them through the
+// scalar cache at every tile instead of holding them in SGPRs across the
+// tile loop (12 matrices and 16 offsets would not fit).  Every slot of a
+// layout the pass uses holds a matrix (the identity where there is no gate):
+// the kernel applies all of them without branches.
+struct TTab
+{
+    uint64_t roff[kRegs]; // layout-0 register offsets (unit addresses)
+    TGate g[kUnitBits + 1];
+};
+
+// Register bits of the three layouts (unit bits); the thread bits are the
+// remaining eight in ascending order.  Layout 0 is the load/store layout: its
+// thread bits start with unit bits 0..3, the contiguous ones.
+__host__ __device__ constexpr int reg_bit(int L, int i)
+{
+    return L == 0 ? 8 + i : (L == 1 ? 4 + i : i);
+}
+__host__ __device__ constexpr int thr_bit(int L, int j)
+{
+    return L == 0 ? j : (L == 1 ? (j < 4 ? j : j + 4) : j + 4);
+}
+int layout_of_unit_bit(int k)
+{
+    return k >= 8 ? 0 : (k >= 4 ? 1 : 2);
+}
+
+template <int L>
+__device__ __forceinline__ int thr_tau(int t)
+{
+    int tau = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        tau |= ((t >> j) & 1) << thr_bit(L, j);
+    return tau;
+}
+template <int L>
+__device__ __forceinline__ constexpr int reg_tau(int r)
+{
+    int tau = 0;
+    for (int i = 0; i < 4; i++)
+        tau |= ((r >> i) & 1) << reg_bit(L, i);
+    return tau;
+}
+template <int L>
+__device__ __forceinline__ uint64_t thr_off(int t, const TPass &P)
+{
+    uint64_t o = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        o |= uint64_t((t >> j) & 1) << P.ub[thr_bit(L, j)];
+    return o;
+}
+// Unit address of the first unit of a tile: the tile index deposited into
+// the non-tile bits.
+__host__ __device__ __forceinline__ uint64_t deposit(uint64_t tile, const uint32_t *ub)
+{
+    for (int k = 0; k < kUnitBits; k++)
+    {
+        const unsigned p = ub[k];
+        tile = ((tile >> p) << (p + 1)) | (tile & ((uint64_t(1) << p) - 1));
+    }
+    return tile;
+}
+
+// deposit(tile + grid) from deposit(tile): an addition inside the non-tile
+// bits, the tile bits set to one so that carries run through them.
+__device__ __forceinline__ uint64_t next_base(uint64_t base, const TPass &P)
+{
+    return ((base | P.tmask) + P.gstep) & ~P.tmask;
+}
+
+// LDS swizzle: within every group of 16 lanes the 16 units of an access land
+// in 16 distinct 16-byte bank groups, for all three layouts.  It is linear
+// over XOR: swz(a | b) = swz(a) ^ swz(b) for disjoint a, b.
+__host__ __device__ constexpr int swz(int tau)
+{
+    return tau ^ ((tau >> 4) & 15);
+}
+
+__device__ __forceinline__ uint32_t par(uint64_t x)
+{
+    return __popcll(x) & 1u;
+}
+
+template <typename R>
+struct Pair
+{
+    R ax, ay, bx, by;
+};
+
+// (a, b) -> (m00 a + m01 b, m10 a + m11 b); with flip the roles of a and b
+// swap (a is then the odd side of the pair).  m: the matrix, loaded once
+// per gate.
+template <typename R>
+__device__ __forceinline__ Pair<R> pair2(R ax, R ay, R bx, R by, const R (&m)[8], bool flip)
+{
+    const R m00x = flip ? m[6] : m[0], m00y = flip ? m[7] : m[1];
+    const R m01x = flip ? m[4] : m[2], m01y = flip ? m[5] : m[3];
+    const R m10x = flip ? m[2] : m[4], m10y = flip ? m[3] : m[5];
+    const R m11x = flip ? m[0] : m[6], m11y = flip ? m[1] : m[7];
+    Pair<R> o;
+    o.ax = m00x * ax - m00y * ay + m01x * bx - m01y * by;
+    o.ay = m00x * ay + m00y * ax + m01x * by + m01y * bx;
+    o.bx = m10x * ax - m10y * ay + m11x * bx - m11y * by;
+    o.by = m10x * ay + m10y * ax + m11x * by + m11y * bx;
+    return o;
+}
+
+template <bool kC64>
+using Real = typename std::conditional<kC64, float, double>::type;
+
+template <bool kC64>
+__device__ __forceinline__ void load_matrix(Real<kC64> (&m)[8], ZXA_CONST TGate &g)
+{
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+    {
+        if constexpr (kC64)
+            m[i] = g.uf[i];
+        else
+            m[i] = g.u[i];
+    }
+}
+
+template <bool kC64, bool kFlip>
+__device__ __forceinline__ void unit_pair(typename UnitT<kC64>::T &a, typename UnitT<kC64>::T &b,
+                                          const Real<kC64> (&m)[8], bool flip, bool a0)
+{
+    if constexpr (kC64)
+    {
+        const Pair<float> lo = pair2<float>(a.x, a.y, b.x, b.y, m, kFlip && flip);
+        const Pair<float> hi = pair2<float>(a.z, a.w, b.z, b.w, m, kFlip && (flip ^ a0));
+        a = fvec4{lo.ax, lo.ay, hi.ax, hi.ay};
+        b = fvec4{lo.bx, lo.by, hi.bx, hi.by};
+    }
+    else
+    {
+        const Pair<double> o = pair2<double>(a.x, a.y, b.x, b.y, m, kFlip && flip);
+        a = dvec2{o.ax, o.ay};
+        b = dvec2{o.bx, o.by};
+    }
+}
+
+// Gates on the register bits of layout L (and, in layout 0, the complex64
+// in-unit gate).  base | toff is the unit address of the thread's register 0.
+template <bool kC64, bool kFlip, int L>
+__device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs], const TPass &P, ZXA_CONST TTab *G,
+                                             uint64_t base, uint64_t toff)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        const int k = reg_bit(L, i);
+        ZXA_CONST TGate &g = G->g[k];
+        Real<kC64> m[8];
+        load_matrix<kC64>(m, g);
+        const uint64_t amask = kFlip ? g.amask : 0;
+        const uint32_t pt = kFlip ? (par((base | toff) & amask) ^ g.beta) : 0u;
+        const bool a0 = kFlip && g.a0 != 0;
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+        {
+            if ((r >> i) & 1)
+                continue;
+            const bool flip = kFlip && ((pt ^ (g.rpar >> r)) & 1u) != 0u;
+            unit_pair<kC64, kFlip>(v[r], v[r | (1 << i)], m, flip, a0);
+        }
+    }
+    if constexpr (kC64 && L == 0)
+    {
+        ZXA_CONST TGate &g = G->g[kIntra];
+        float m[8];
+        load_matrix<true>(m, g);
+        const uint64_t amask = kFlip ? g.amask : 0;
+        const uint32_t pt = kFlip ? (par((base | toff) & amask) ^ g.beta) : 0u;
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+        {
+            const bool flip = kFlip && ((pt ^ (g.rpar >> r)) & 1u) != 0u;
+            const Pair<float> o = pair2<float>(v[r].x, v[r].y, v[r].z, v[r].w, m, flip);
+            v[r] = fvec4{o.ax, o.ay, o.bx, o.by};
+        }
+    }
+}
+
+template <bool kC64, int LW, int LR>
+__device__ __forceinline__ void transpose(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds, int t)
+{
+    // The LDS addresses are recomputed at every transpose (one XOR each)
+    // rather than hoisted out of the tile loop, which would pin 32 VGPRs.
+    int tw = swz(thr_tau<LW>(t));
+    int tr = swz(thr_tau<LR>(t));
+    asm volatile("" : "+v"(tw), "+v"(tr));
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        lds[tw ^ swz(reg_tau<LW>(r))] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        v[r] = lds[tr ^ swz(reg_tau<LR>(r))];
+}
+
+struct cd
+{
+    double x, y;
+};
+__device__ __forceinline__ cd cmul(cd a, cd b)
+{
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+
+// Diagonal epilogue (layout 0).  For the amplitude of thread t at register
+// element rho (complex128: rho = r; complex64: rho = 2 r + in-unit bit) the
+// window's phase is
+//     Phi = ST(t) * prod_{j in t} E_j * D[rho] * prod_{b in rho} G_b(t)
+// with ST, G tile-independent (per-thread table) and E, D per tile (written
+// by tile_phase_kernel).  rho is visited in Gray-code order so that the last
+// product costs one complex multiplication per amplitude.
+template <bool kC64>
+struct Epi
+{
+    static constexpr int kNB = kC64 ? 5 : 4;         // register element bits
+    static constexpr int kNR = 1 << kNB;             // register elements
+    static constexpr int kStride = 2 * (8 + kNR);    // doubles per tile in tphase
+    static constexpr int kThrStride = 2 * (1 + kNB); // doubles per thread in ttab
+    cd st;
+    cd g[kNB];
+
+    __device__ __forceinline__ void init(const double *__restrict__ ttab, int t)
+    {
+        const double *a = ttab + t * kThrStride;
+        st = {a[0], a[1]};
+#pragma unroll
+        for (int b = 0; b < kNB; b++)
+            g[b] = {a[2 + 2 * b], a[3 + 2 * b]};
+    }
+
+    __device__ __forceinline__ void apply(typename UnitT<kC64>::T (&v)[kRegs], ZXA_CONST double *tp, int t) const
+    {
+        cd w = st;
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+        {
+            const cd e = {tp[2 * j], tp[2 * j + 1]};
+            const cd we = cmul(w, e);
+            const bool on = (t >> j) & 1;
+            w.x = on ? we.x : w.x;
+            w.y = on ? we.y : w.y;
+        }
+        ZXA_CONST double *D = tp + 16;
+#pragma unroll
+        for (int k = 0; k < kNR; k++)
+        {
+            const int rho = k ^ (k >> 1);
+            if (k > 0)
+            {
+                const int b = __builtin_ctz(k);
+                const cd gb = ((rho >> b) & 1) ? g[b] : cd{g[b].x, -g[b].y};
+                w = cmul(w, gb);
+            }
+            const cd ph = cmul(w, {D[2 * rho], D[2 * rho + 1]});
+            if constexpr (kC64)
+            {
+                const float px = static_cast<float>(ph.x), py = static_cast<float>(ph.y);
+                fvec4 &u = v[rho >> 1];
+                if (rho & 1)
+                {
+                    const float zx = u.z * px - u.w * py, zy = u.z * py + u.w * px;
+                    u.z = zx;
+                    u.w = zy;
+                }
+                else
+                {
+                    const float xx = u.x * px - u.y * py, xy = u.x * py + u.y * px;
+                    u.x = xx;
+                    u.y = xy;
+                }
+            }
+            else
+            {
+                dvec2 &u = v[rho];
+                const double nx = u.x * ph.x - u.y * ph.y, ny = u.x * ph.y + u.y * ph.x;
+                u.x = nx;
+                u.y = ny;
+            }
+        }
+    }
+};
+
+template <bool kC64>
+__device__ __forceinline__ void load_tile(typename UnitT<kC64>::T (&v)[kRegs], const typename UnitT<kC64>::T *p,
+                                          uint64_t at, ZXA_CONST TTab *G)
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        v[r] = __builtin_nontemporal_load(p + (at | G->roff[r]));
+}
+
+template <bool kC64>
+__device__ __forceinline__ void store_tile(const typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *p,
+                                           uint64_t at, ZXA_CONST TTab *G)
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        __builtin_nontemporal_store(v[r], p + (at | G->roff[r]));
+}
+
+// Gates, transposes and the diagonal epilogue of one tile held in v.
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
+__device__ __forceinline__ void tile_work(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds,
+                                          const TPass &P, ZXA_CONST TTab *G, const double *tphase,
+                                          const Epi<kC64> &epi, int t, uint64_t toff, uint64_t tile, uint64_t base)
+{
+    apply_layout<kC64, kFlip, 0>(v, P, G, base, toff);
+    if constexpr (kL1)
+    {
+        transpose<kC64, 0, 1>(v, lds, t);
+        apply_layout<kC64, kFlip, 1>(v, P, G, base, thr_off<1>(t, P));
+        if constexpr (kL2)
+        {
+            transpose<kC64, 1, 2>(v, lds, t);
+            apply_layout<kC64, kFlip, 2>(v, P, G, base, thr_off<2>(t, P));
+            transpose<kC64, 2, 0>(v, lds, t);
+        }
+        else
+        {
+            transpose<kC64, 1, 0>(v, lds, t);
+        }
+    }
+    else if constexpr (kL2)
+    {
+        transpose<kC64, 0, 2>(v, lds, t);
+        apply_layout<kC64, kFlip, 2>(v, P, G, base, thr_off<2>(t, P));
+        transpose<kC64, 2, 0>(v, lds, t);
+    }
+    if constexpr (kDiag)
+        epi.apply(v, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), t);
+}
+
+// The tables are re-read at every tile (see TTab): the pointer is laundered
+// so that the loads are not hoisted out of the tile loop.
+__device__ __forceinline__ ZXA_CONST TTab *tables(const TTab *tabp)
+{
+    asm volatile("" : "+s"(tabp));
+    return (ZXA_CONST TTab *)tabp;
+}
+
+// tabp: register offsets and gate matrices; ttab: per-thread ST and G
+// (kDiag only); tphase: per-tile E and D (kDiag only).
+//
+// Software pipeline over the workgroup's tiles t0, t1 = t0 + grid, ... with
+// two register buffers: while tile k is worked on, tile k + 1 is in flight,
+// and the store of tile k - 1 is issued after the loads of tile k + 1, so
+// that waiting for a tile's loads never waits for the previous stores.  The
+// prefetch is unconditional (past the last tile it re-reads the current one):
+// with no branch around it the compiler's wait counts stay exact.
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void tile_kernel(
+    void *__restrict__ state, const TPass P, const TTab *__restrict__ tabp, const double *__restrict__ ttab,
+    const double *__restrict__ tphase)
+{
+    using T = typename UnitT<kC64>::T;
+    __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
+    const int t = threadIdx.x;
+    T *p = static_cast<T *>(state);
+    const uint64_t toff = thr_off<0>(t, P);
+    Epi<kC64> epi;
+    if constexpr (kDiag)
+        epi.init(ttab, t);
+    const uint64_t grid = gridDim.x;
+
+    T a[kRegs], b[kRegs];
+    uint64_t t0 = blockIdx.x;
+    uint64_t b0 = deposit(t0, P.sorted);
+    ZXA_CONST TTab *G = tables(tabp);
+    load_tile<kC64>(a, p, b0 | toff, G);
+    uint64_t t1 = t0 + grid;
+    bool m1 = t1 < P.ntiles;
+    uint64_t b1 = next_base(b0, P);
+    load_tile<kC64>(b, p, (m1 ? b1 : b0) | toff, G);
+    tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t0, b0);
+    for (;;)
+    {
+        G = tables(tabp);
+        store_tile<kC64>(a, p, b0 | toff, G);
+        if (!m1)
+            break;
+        const uint64_t t2 = t1 + grid;
+        const bool m2 = t2 < P.ntiles;
+        const uint64_t b2 = next_base(b1, P);
+        load_tile<kC64>(a, p, (m2 ? b2 : b1) | toff, G);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip>(b, lds, P, G, tphase, epi, t, toff, t1, b1);
+
+        G = tables(tabp);
+        store_tile<kC64>(b, p, b1 | toff, G);
+        if (!m2)
+            break;
+        const uint64_t t3 = t2 + grid;
+        const bool m3 = t3 < P.ntiles;
+        const uint64_t b3 = next_base(b2, P);
+        load_tile<kC64>(b, p, (m3 ? b3 : b2) | toff, G);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t2, b2);
+
+        t0 = t2;
+        b0 = b2;
+        t1 = t3;
+        b1 = b3;
+        m1 = m3;
+    }
+}
+
+__device__ __forceinline__ cd cis(double a)
+{
+    double s, c;
+    sincos(a, &s, &c);
+    return {c, s};
+}
+
+// Per-tile phase factors of the diagonal epilogue, one thread per tile.
+// Tile index bit i is address bit outer[i].  tab (nE tile element bits, nh
+// outer bits, nR register elements):
+//     cq[k * nh + i] = Q(tile bit k, outer i)   k < nE
+//     hL[i] = L(outer i), hQ[i * nh + l] = Q(outer i, outer l)  (i < l)
+//     rs[rho]        = static phase of register element rho
+//     map[0..8)      = tile element index of thread bit j
+//     map[8..8+nb)   = tile element index of register element bit b
+// Output per tile: E_j = exp(i c(thr j)) (8), D[rho] = exp(i (C + rs[rho] +
+// sum_{b in rho} c(reg b))) (nR), as complex pairs.
+__global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__restrict__ tab, int nE, int nh, int nb,
+                                                              double K, uint64_t ntiles, double *__restrict__ out)
+{
+    extern __shared__ double sh[];
+    const int nR = 1 << nb;
+    const int ntab = nE * nh + nh + nh * nh + nR + 8 + nb;
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x)
+        sh[i] = tab[i];
+    __syncthreads();
+    const uint64_t tile = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (tile >= ntiles)
+        return;
+    const double *cq = sh;
+    const double *hL = cq + nE * nh;
+    const double *hQ = hL + nh;
+    const double *rs = hQ + nh * nh;
+    const double *mp = rs + nR;
+    double c[13];
+    for (int k = 0; k < nE; k++)
+    {
+        double s = 0.0;
+        for (int i = 0; i < nh; i++)
+            if ((tile >> i) & 1u)
+                s += cq[k * nh + i];
+        c[k] = s;
+    }
+    double C = K;
+    for (int i = 0; i < nh; i++)
+    {
+        if (!((tile >> i) & 1u))
+            continue;
+        double s = hL[i];
+        for (int l = i + 1; l < nh; l++)
+            if ((tile >> l) & 1u)
+                s += hQ[i * nh + l];
+        C += s;
+    }
+    const int stride = 2 * (8 + nR);
+    double *o = out + tile * stride;
+    for (int j = 0; j < 8; j++)
+    {
+        const cd e = cis(c[static_cast<int>(mp[j])]);
+        o[2 * j] = e.x;
+        o[2 * j + 1] = e.y;
+    }
+    for (int rho = 0; rho < nR; rho++)
+    {
+        double a = C + rs[rho];
+        for (int b = 0; b < nb; b++)
+            if ((rho >> b) & 1)
+                a += c[static_cast<int>(mp[8 + b])];
+        const cd d = cis(a);
+        o[16 + 2 * rho] = d.x;
+        o[16 + 2 * rho + 1] = d.y;
+    }
+}
+
+template <typename K>
+int persistent_grid(Ctx &ctx, K kernel, uint64_t tiles)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    const uint64_t cap = uint64_t(per_cu) * uint64_t(ctx.num_cus > 0 ? ctx.num_cus : 256);
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
+}
+
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
+int launch_t(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const double *ttab, const double *tphase)
+{
+    auto k = tile_kernel<kC64, kL1, kL2, kDiag, kFlip>;
+    const int grid = persistent_grid(ctx, k, P0.ntiles);
+    TPass P = P0;
+    P.gstep = deposit(uint64_t(grid), P.sorted);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, state, P, tab, ttab, tphase);
+    return hip_check(hipGetLastError(), "tile_kernel launch");
+}
+
+template <bool kC64, bool kDiag, bool kFlip>
+int launch_l(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, const TTab *tab, const double *ttab,
+             const double *tphase)
+{
+    if (l1 && l2)
+        return launch_t<kC64, true, true, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
+    if (l1)
+        return launch_t<kC64, true, false, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
+    if (l2)
+        return launch_t<kC64, false, true, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
+    return launch_t<kC64, false, false, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
+}
+
+template <bool kC64>
+int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip, const TTab *tab,
+             const double *ttab, const double *tphase)
+{
+    if (ttab)
+        return launch_l<kC64, true, false>(ctx, state, P, l1, l2, tab, ttab, tphase);
+    if (flip)
+        return launch_l<kC64, false, true>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
+    return launch_l<kC64, false, false>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
+}
+
+struct SingleGate
+{
+    unsigned target;
+    const U3Gate *g;
+};
+
+// Tile plan: the address bit of every tile element bit e (complex128: e is
+// the unit bit; complex64: e = 0 is address bit 0 inside a unit and e = k + 1
+// is unit bit k), and the unit bit of every target.  Unit bits 0..3 are the
+// lowest address bits; targets above them go to the register bits of
+// layout 0 (unit bits 8..11), then of layout 1 (4..7); the remaining slots
+// take the lowest other bits, those of layout 1 first (so that, in the load
+// layout, lanes 16..63 continue the contiguous run of lanes 0..15).
+struct Plan
+{
+    std::vector<unsigned> ebit; // tile element bit -> address bit
+    std::vector<int> slot;      // per gate: unit bit, or kIntra
+};
+
+bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl)
+{
+    const bool c64 = prec == C64;
+    const unsigned low = tile_low_bits(prec);
+    const unsigned shift = c64 ? 1u : 0u;
+    std::vector<int> unit_addr(kUnitBits, -1); // unit bit -> address bit
+    for (int k = 0; k < 4; k++)
+        unit_addr[k] = int(k + shift);
+    std::vector<bool> used(64, false);
+    for (unsigned b = 0; b < low; b++)
+        used[b] = true;
+    std::vector<unsigned> upper;
+    for (const SingleGate &s : gs)
+        if (s.target >= low)
+            upper.push_back(s.target);
+    std::sort(upper.begin(), upper.end());
+    if (upper.size() > 8 || n < tile_min_bits(prec))
+        return false;
+    const int order[8] = {8, 9, 10, 11, 4, 5, 6, 7};
+    size_t u = 0;
+    for (; u < upper.size(); u++)
+    {
+        unit_addr[order[u]] = int(upper[u]);
+        used[upper[u]] = true;
+    }
+    const int fill[8] = {4, 5, 6, 7, 8, 9, 10, 11};
+    unsigned next = low;
+    for (int f : fill)
+    {
+        if (unit_addr[f] >= 0)
+            continue;
+        while (used[next])
+            next++;
+        unit_addr[f] = int(next);
+        used[next] = true;
+    }
+    pl.ebit.clear();
+    if (c64)
+        pl.ebit.push_back(0);
+    for (int k = 0; k < kUnitBits; k++)
+        pl.ebit.push_back(unsigned(unit_addr[k]));
+    for (unsigned e : pl.ebit)
+        if (e >= n)
+            return false;
+    pl.slot.clear();
+    for (const SingleGate &s : gs)
+    {
+        const int e = int(std::find(pl.ebit.begin(), pl.ebit.end(), s.target) - pl.ebit.begin());
+        pl.slot.push_back(c64 ? (e == 0 ? kIntra : e - 1) : e);
+    }
+    return true;
+}
+
+// Host tables of the diagonal epilogue: the per-thread table (ST, G) and the
+// packed input of tile_phase_kernel.
+void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, const Poly &poly,
+                     std::vector<double> &ttab, std::vector<double> &ptab, int *nh_out, int *nb_out)
+// pos: address bit of each tile element bit (Plan::ebit)
+{
+    const bool c64 = prec == C64;
+    const int nE = int(pos.size());
+    const int nb = c64 ? 5 : 4;
+    const int nR = 1 << nb;
+    int thr_k[8], reg_k[5];
+    for (int j = 0; j < 8; j++)
+        thr_k[j] = c64 ? j + 1 : j;
+    for (int b = 0; b < nb; b++)
+        reg_k[b] = c64 ? (b == 0 ? 0 : 8 + b) : 8 + b;
+    auto Lt = [&](int k) { return poly.L[pos[k]]; };
+    auto Qt = [&](int k, int l) { return poly.q(int(pos[k]), int(pos[l])); };
+
+    ttab.assign(size_t(kThreads) * 2 * (1 + nb), 0.0);
+    for (int t = 0; t < kThreads; t++)
+    {
+        double s = 0.0;
+        for (int j = 0; j < 8; j++)
+        {
+            if (!((t >> j) & 1))
+                continue;
+            s += Lt(thr_k[j]);
+            for (int l = j + 1; l < 8; l++)
+                if ((t >> l) & 1)
+                    s += Qt(thr_k[j], thr_k[l]);
+        }
+        double *a = ttab.data() + size_t(t) * 2 * (1 + nb);
+        a[0] = std::cos(s);
+        a[1] = std::sin(s);
+        for (int b = 0; b < nb; b++)
+        {
+            double d = 0.0;
+            for (int j = 0; j < 8; j++)
+                if ((t >> j) & 1)
+                    d += Qt(thr_k[j], reg_k[b]);
+            a[2 + 2 * b] = std::cos(d);
+            a[3 + 2 * b] = std::sin(d);
+        }
+    }
+
+    std::vector<int> outer;
+    for (unsigned b = 0; b < n; b++)
+        if (std::find(pos.begin(), pos.end(), b) == pos.end())
+            outer.push_back(int(b));
+    const int nh = int(outer.size());
+    ptab.assign(size_t(nE) * nh + nh + size_t(nh) * nh + nR + 8 + nb, 0.0);
+    double *cq = ptab.data();
+    double *hL = cq + nE * nh;
+    double *hQ = hL + nh;
+    double *rs = hQ + nh * nh;
+    double *mp = rs + nR;
+    for (int k = 0; k < nE; k++)
+        for (int i = 0; i < nh; i++)
+            cq[k * nh + i] = poly.q(int(pos[k]), outer[i]);
+    for (int i = 0; i < nh; i++)
+    {
+        hL[i] = poly.L[outer[i]];
+        for (int l = i + 1; l < nh; l++)
+            hQ[i * nh + l] = poly.q(outer[i], outer[l]);
+    }
+    for (int rho = 0; rho < nR; rho++)
+    {
+        double s = 0.0;
+        for (int b = 0; b < nb; b++)
+        {
+            if (!((rho >> b) & 1))
+                continue;
+            s += Lt(reg_k[b]);
+            for (int c = b + 1; c < nb; c++)
+                if ((rho >> c) & 1)
+                    s += Qt(reg_k[b], reg_k[c]);
+        }
+        rs[rho] = s;
+    }
+    for (int j = 0; j < 8; j++)
+        mp[j] = thr_k[j];
+    for (int b = 0; b < nb; b++)
+        mp[8 + b] = reg_k[b];
+    *nh_out = nh;
+    *nb_out = nb;
+}
+
+// One tile pass over the given targets, then the diagonal window (if any).
+int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
+                  const DiagGate *diag = nullptr, size_t ndiag = 0)
+{
+    const bool c64 = prec == C64;
+    Plan pl;
+    if (!plan_tile(prec, n, gs, pl))
+    {
+        set_error("tile pass: state narrower than a tile or too many targets");
+        return EINVAL_;
+    }
+    const std::vector<unsigned> &pos = pl.ebit;
+    TPass P;
+    TTab G;
+    std::memset(&P, 0, sizeof(P));
+    std::memset(&G, 0, sizeof(G));
+    const unsigned shift = c64 ? 1u : 0u; // element bit -> unit-address bit
+    for (int k = 0; k < kUnitBits; k++)
+    {
+        P.ub[k] = pos[k + shift] - shift;
+        P.sorted[k] = P.ub[k];
+        P.tmask |= uint64_t(1) << P.ub[k];
+    }
+    std::sort(P.sorted, P.sorted + kUnitBits);
+    for (int r = 0; r < kRegs; r++)
+        for (int i = 0; i < 4; i++)
+            if ((r >> i) & 1)
+                G.roff[r] |= uint64_t(1) << P.ub[reg_bit(0, i)];
+    P.ntiles = uint64_t(1) << (n - shift - kUnitBits);
+    // every slot starts as the identity
+    for (TGate &tg : G.g)
+    {
+        tg.u[0] = tg.u[6] = 1.0;
+        tg.uf[0] = tg.uf[6] = 1.0f;
+    }
+    bool use[3] = {false, false, false};
+    bool flip = false;
+    for (size_t gi = 0; gi < gs.size(); gi++)
+    {
+        const SingleGate &s = gs[gi];
+        const int slot = pl.slot[gi];
+        TGate &tg = G.g[slot];
+        const uint64_t alpha = s.g->packed_sel & ((uint64_t(1) << 63) - 1);
+        const uint64_t rest = alpha & ~s.g->delta;
+        uint32_t beta = uint32_t(s.g->packed_sel >> 63);
+        const double *u = s.g->u;
+        double m[8];
+        if (rest == 0 && beta)
+        {
+            // the even side is always the target-bit-one amplitude: swap roles
+            const double sw[8] = {u[6], u[7], u[4], u[5], u[2], u[3], u[0], u[1]};
+            std::memcpy(m, sw, sizeof(m));
+            beta = 0;
+        }
+        else
+        {
+            std::memcpy(m, u, sizeof(m));
+        }
+        for (int i = 0; i < 8; i++)
+        {
+            tg.u[i] = m[i];
+            tg.uf[i] = float(m[i]);
+        }
+        tg.amask = rest >> shift;
+        tg.a0 = c64 ? uint32_t(rest & 1u) : 0u;
+        tg.beta = beta;
+        const int lay = slot == kIntra ? 0 : layout_of_unit_bit(slot);
+        for (int r = 0; r < kRegs; r++)
+        {
+            uint64_t off = 0;
+            for (int i = 0; i < 4; i++)
+                if ((r >> i) & 1)
+                    off |= uint64_t(1) << P.ub[reg_bit(lay, i)];
+            tg.rpar |= uint32_t(__builtin_popcountll(off & tg.amask) & 1) << r;
+        }
+        if (rest != 0)
+            flip = true;
+        use[slot == kIntra ? 0 : layout_of_unit_bit(slot)] = true;
+    }
+    const double *ttab = nullptr;
+    const double *tphase = nullptr;
+    if (ndiag > 0)
+    {
+        if (flip)
+        {
+            set_error("tile pass: diagonal epilogue with per-pair selectors");
+            return EINVAL_;
+        }
+        Poly poly;
+        window_polynomial(n, diag, ndiag, poly);
+        std::vector<double> tt, pt;
+        int nh = 0, nb = 0;
+        epilogue_tables(prec, n, pos, poly, tt, pt, &nh, &nb);
+        void *dtt = nullptr, *dpt = nullptr;
+        ZXA_TRY(ctx.ring.stage(tt.data(), tt.size() * sizeof(double), ctx.stream, &dtt));
+        ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
+        const size_t stride = 2 * (8 + (size_t(1) << nb));
+        ZXA_TRY(ctx.scratch_a.ensure(P.ntiles * stride * sizeof(double)));
+        const unsigned grid = unsigned((P.ntiles + kThreads - 1) / kThreads);
+        hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), pt.size() * sizeof(double), ctx.stream,
+                           static_cast<const double *>(dpt), int(pos.size()), nh, nb, poly.K, P.ntiles,
+                           static_cast<double *>(ctx.scratch_a.ptr));
+        ZXA_HIP(hipGetLastError());
+        ttab = static_cast<const double *>(dtt);
+        tphase = static_cast<const double *>(ctx.scratch_a.ptr);
+    }
+    void *dtab = nullptr;
+    ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
+    const TTab *tab = static_cast<const TTab *>(dtab);
+    const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
+    ZXA_TRY(ctx.tic());
+    if (c64)
+        ZXA_TRY(launch_p<true>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
+    else
+        ZXA_TRY(launch_p<false>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
+    ZXA_TRY(ctx.toc(ndiag > 0 ? F_FUSED : F_U3TILE, bytes));
+    return ctx.ring.done(ctx.stream);
+}
+
+} // namespace
+
+unsigned tile_low_bits(int prec)
+{
+    return prec == C64 ? 5u : 4u;
+}
+
+unsigned tile_min_bits(int prec)
+{
+    return tile_low_bits(prec) + 8u;
+}
+
+int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates)
+{
+    const unsigned low = tile_low_bits(prec);
+    std::vector<SingleGate> singles;
+    for (size_t i = 0; i < ngates; i++)
+    {
+        const U3Gate &g = gates[i];
+        const uint64_t alpha = g.packed_sel & ((uint64_t(1) << 63) - 1);
+        const bool one_bit = g.delta != 0 && (g.delta & (g.delta - 1)) == 0 && (g.delta >> n) == 0;
+        const bool paired = (__builtin_popcountll(alpha & g.delta) & 1) == 1;
+        if (n >= tile_min_bits(prec) && n <= 40 && one_bit && paired && (alpha >> n) == 0)
+            singles.push_back({unsigned(__builtin_ctzll(g.delta)), &g});
+        else
+            ZXA_TRY(u3_pair_apply(ctx, state, prec, n, g));
+    }
+    if (singles.empty())
+        return OK;
+    std::sort(singles.begin(), singles.end(),
+              [](const SingleGate &a, const SingleGate &b) { return a.target < b.target; });
+    for (size_t i = 1; i < singles.size(); i++)
+        if (singles[i].target == singles[i - 1].target)
+        {
+            set_error("u3_apply: two gates on one target in one batch");
+            return EINVAL_;
+        }
+    // the low targets ride with the first pass; at most 8 others per pass
+    std::vector<SingleGate> lows, highs;
+    for (const SingleGate &s : singles)
+        (s.target < low ? lows : highs).push_back(s);
+    size_t hi = 0;
+    bool first = true;
+    while (first || hi < highs.size())
+    {
+        std::vector<SingleGate> pass;
+        if (first)
+            pass = lows;
+        const size_t take = std::min<size_t>(8, highs.size() - hi);
+        for (size_t k = 0; k < take; k++)
+            pass.push_back(highs[hi + k]);
+        hi += take;
+        first = false;
+        if (!pass.empty())
+            ZXA_TRY(run_tile_pass(ctx, state, prec, n, pass));
+    }
+    return OK;
+}
+
+int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates,
+                     const DiagGate *diag, size_t ndiag)
+{
+    if (n < tile_min_bits(prec) || n > 40)
+    {
+        set_error("fused pass: state narrower than a tile");
+        return EINVAL_;
+    }
+    if (ndiag > 0 && !window_is_polynomial(n, diag, ndiag))
+    {
+        set_error("fused pass: diagonal window is not a phase polynomial");
+        return EINVAL_;
+    }
+    const unsigned low = tile_low_bits(prec);
+    std::vector<SingleGate> gs;
+    int high = 0;
+    for (size_t i = 0; i < ngates; i++)
+    {
+        const U3Gate &g = gates[i];
+        const uint64_t alpha = g.packed_sel & ((uint64_t(1) << 63) - 1);
+        if (g.delta == 0 || (g.delta & (g.delta - 1)) != 0 || alpha != g.delta || (g.delta >> n) != 0)
+        {
+            set_error("fused pass: gate is not a single-bit U3");
+            return EINVAL_;
+        }
+        const unsigned t = unsigned(__builtin_ctzll(g.delta));
+        for (const SingleGate &o : gs)
+            if (o.target == t)
+            {
+                set_error("fused pass: two gates on one target");
+                return EINVAL_;
+            }
+        if (t >= low)
+            high++;
+        gs.push_back({t, &g});
+    }
+    if (high > 8)
+    {
+        set_error("fused pass: more than 8 targets above the tile's low bits");
+        return EINVAL_;
+    }
+    if (gs.empty())
+        return diag_apply(ctx, state, prec, n, diag, ndiag);
+    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag);
+}
+
+} // namespace zxa
