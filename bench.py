@@ -13,7 +13,8 @@ Reported beside it (same JSON line):
                 average launch time (HIP events on the launch stream)
   reference_policy  the same step with one pass per flushed batch, as the
                 reference runs it (30 U3 passes + 29 diagonal sweeps)
-  autotune      the diagonal-sweep variant chosen on the resident state
+  autotune      the diagonal-sweep variant and the tile geometry chosen on the
+                resident state before the timed steps
   diag_batches  one fused sweep per batch of B random P/CP gates, B in
                 {1,4,16,64,256}, uniform-superposition input, c128 and c64
   u3_all        U3 on all 30 qubits as tile passes
@@ -225,8 +226,9 @@ def main():
     sim = Z.ZXH(NQ, False, True, precision="c128", policy="fused")
     sim.gates(W.hadamard_all(NQ))
     sim.execute()
-    log("autotune diag c128")
-    tuned = {"c128": autotune.tune_diag(wrap_state(sim.device_state(), "c128"))}
+    log("autotune diag and tile c128")
+    tuned = {"c128": autotune.tune_diag(wrap_state(sim.device_state(), "c128")),
+             "tile_c128": autotune.tune_tile(wrap_state(sim.device_state(), "c128"))}
 
     log(f"QFT-{NQ} c128 fused: warmup {args.warmup}, steps {args.steps}")
     sim, ngates, t_step, stats = qft_step(Z, W, "c128", args.steps, args.warmup, sim=sim)
@@ -275,6 +277,7 @@ def main():
         sim64.gates(W.hadamard_all(NQ))
         sim64.execute()
         tuned["c64"] = autotune.tune_diag(wrap_state(sim64.device_state(), "c64"))
+        tuned["tile_c64"] = autotune.tune_tile(wrap_state(sim64.device_state(), "c64"))
         sim64, _, t64, stats64 = qft_step(Z, W, "c64", args.steps, 1, sim=sim64)
         result["c64"] = {"gates/s": round(ngates / t64, 3), "ms_per_step": round(t64 * 1e3, 3)}
         sim64.clear_gates()

@@ -35,3 +35,18 @@ def gpu_engine():
     import zxh_adae_amd as Z
     Z.init()
     return Z
+
+
+@pytest.fixture(params=[4, 5, 6], ids=lambda s: f"seg{s}")
+def tile_seg(request, gpu_engine):
+    """Runs a test under each tile geometry (knobs tile.seg.c128 and
+    tile.seg.c64: 4, 5 or 6 contiguous unit-address bits per tile) and
+    restores the knobs."""
+    from zxh_adae_amd import autotune
+    knobs = ("tile.seg.c128", "tile.seg.c64")
+    old = [autotune.tune_get(k) for k in knobs]
+    for k in knobs:
+        autotune.tune_set(k, request.param)
+    yield request.param
+    for k, v in zip(knobs, old):
+        autotune.tune_set(k, v)

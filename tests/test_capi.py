@@ -33,3 +33,25 @@ def test_library_exports_every_declared_symbol():
 @pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="native library not built")
 def test_python_signatures_cover_header():
     assert set(declared_functions()) == set(_lib.EXPORTED)
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="native library not built")
+def test_tuning_knobs_ranges():
+    """The knobs the autotuner sweeps exist, accept their documented values,
+    reject the others, and are host state only (no device needed)."""
+    from zxh_adae_amd import autotune
+    from zxh_adae_amd import ZXHError
+    cases = {"diag.ept.c128": ([0, 1, 2, 4], [3, 5, -1]), "diag.ept.c64": ([0, 1, 2, 4], [3, 8]),
+             "diag.nt.c128": ([0, 1], [2]), "diag.nt.c64": ([0, 1], [2]),
+             "tile.seg.c128": ([4, 5, 6], [3, 7]), "tile.seg.c64": ([4, 5, 6], [0, 12])}
+    for knob, (good, bad) in cases.items():
+        old = autotune.tune_get(knob)
+        for v in good:
+            autotune.tune_set(knob, v)
+            assert autotune.tune_get(knob) == v
+        for v in bad:
+            with pytest.raises(ZXHError):
+                autotune.tune_set(knob, v)
+        autotune.tune_set(knob, old)
+    with pytest.raises(ZXHError):
+        autotune.tune_set("no.such.knob", 1)

@@ -144,7 +144,7 @@ def test_large_u3_round_trip(gpu_engine):
 @pytest.mark.parametrize("n,gates_fn", [(16, lambda n: W.qft(n)), (14, lambda n: W.random_layers(n, 5, 3)),
                                        (13, lambda n: W.random_mixed(n, 150, 77, kinds=(W.H, W.U3, W.P, W.CP, W.RZ, W.Z,
                                                                                        W.BARRIER)))])
-def test_fused_policy_matches_oracle_and_reference_policy(gpu_engine, n, gates_fn):
+def test_fused_policy_matches_oracle_and_reference_policy(gpu_engine, tile_seg, n, gates_fn):
     """Plain-method circuits through the fused policy (U3 batches merged into
     tile passes with diagonal epilogues) agree with the oracle and with the
     reference-policy run."""

@@ -274,22 +274,36 @@ def test_mcx_is_exact_permutation(gpu_engine, n, ctrls, target):
                                              (16, [], [15, 14, 4, 9], 64), (20, [1], [19, 12, 7], 256),
                                              (14, [3], [], 10), (15, [], [], 40), (18, [4], [17], 0)])
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-def test_fused_pass_matches_u3_then_diag(gpu_engine, n, low, high, ndiag, prec):
-    """Targets below the tile's low bits (4 for complex128, 5 for complex64)
-    ride free; at most 8 others; the tile needs 12 / 13 address bits."""
+def test_fused_pass_matches_u3_then_diag(gpu_engine, tile_seg, n, low, high, ndiag, prec):
+    """Targets below the tile's low bits (seg for complex128, seg + 1 for
+    complex64) ride free; at most 12 - seg others; the tile needs 12 / 13
+    address bits."""
     rng = np.random.default_rng(n * 13 + ndiag)
     psi = rand_state(rng, n)
     descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in low + high]
     s0, s1, th = diag_window(rng, n, ndiag) if ndiag else ([], [], [])
     want = oracle_diag(oracle_u3(psi, n, descs) if descs else psi, n, s0, s1, th) if ndiag else oracle_u3(psi, n, descs)
     d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
-    lowbits = 5 if prec == "c64" else 4
-    if n < lowbits + 8 or sum(q >= lowbits for q in low + high) > 8:
+    lowbits = tile_seg + (1 if prec == "c64" else 0)
+    if n < lowbits + 12 - tile_seg or sum(q >= lowbits for q in low + high) > 12 - tile_seg:
         from zxh_adae_amd import ZXHError
         with pytest.raises(ZXHError):
             d.fused_pass(descs, s0, s1, th)
         return
     d.fused_pass(descs, s0, s1, th)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
+@pytest.mark.parametrize("n,targets", [(13, list(range(13))), (20, list(range(20))), (17, [0, 5, 6, 9, 16])])
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_u3_batch_under_each_tile_geometry(gpu_engine, tile_seg, n, targets, prec):
+    rng = np.random.default_rng(n * 5 + tile_seg)
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in targets]
+    want = oracle_u3(psi, n, descs)
+    d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    d.u3_batch(descs)
     got = d.download().astype(np.complex128)
     assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
 

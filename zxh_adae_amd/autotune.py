@@ -8,6 +8,9 @@ Knobs swept:
   diag.ept.<prec>  0 = persistent diagonal sweep; 1, 2, 4 = flat grid with that
                    many 16-byte units per thread
   diag.nt.<prec>   nontemporal loads/stores in the flat sweep
+  tile.seg.<prec>  contiguous unit-address bits of a tile pass (4, 5, 6: every
+                   wave access moves 256 B, 512 B or 1 KiB contiguous; a pass
+                   then reaches 8, 7 or 6 targets above them)
 """
 from __future__ import annotations
 
@@ -62,3 +65,39 @@ def tune_diag(state: "K.DeviceState", repeats: int = 3, batch: int = 64) -> Dict
     tune_set(ntk, best[1])
     return {"knob": knob, "best": {"ept": best[0], "nt": best[1]}, "previous": {"ept": old[0], "nt": old[1]},
             "ms": {f"ept{e}_nt{n}": round(v, 4) for (e, n), v in results.items()}}
+
+
+TILE_SEGS = (4, 5, 6)
+
+
+def tune_tile(state: "K.DeviceState", repeats: int = 3, window: int = 32) -> Dict[str, object]:
+    """Times one fused tile pass per tile geometry on state (U3 on every low
+    target and on the highest targets the geometry allows, then a window of
+    P/CP gates) and sets the geometry with the shortest pass.  The state is
+    changed by the passes."""
+    c64 = state.prec == _lib.ZXH_C64
+    knob = "tile.seg.c64" if c64 else "tile.seg.c128"
+    n = state.nbits
+    old = tune_get(knob)
+    gs = W.diag_batch(n, window, 4343)
+    s0 = [(1 << g[2]) for g in gs]
+    s1 = [K.NO_SEL if g[0] == W.P else (1 << g[1]) for g in gs]
+    th = [g[3] for g in gs]
+    results: Dict[int, float] = {}
+    for seg in TILE_SEGS:
+        tune_set(knob, seg)
+        low = seg + (1 if c64 else 0)
+        targets = list(range(low)) + list(range(n - (12 - seg), n))
+        descs = [((1 << q), (1 << q), W.u3_matrix(0.3 + 0.01 * q, 0.2, 0.1)) for q in targets]
+        state.fused_pass(descs, s0, s1, th)  # warm
+        K.reset_timing()
+        K.set_timing(True)
+        for _ in range(repeats):
+            state.fused_pass(descs, s0, s1, th)
+        t = K.timing("fused")
+        K.set_timing(False)
+        results[seg] = t["ms"] / max(t["launches"], 1)
+    best = min(results, key=results.get)
+    tune_set(knob, best)
+    return {"knob": knob, "best": best, "previous": old,
+            "ms": {f"seg{k}": round(v, 4) for k, v in results.items()}}

@@ -171,6 +171,7 @@ struct Ctx
 //   diag.ept.c128 / diag.ept.c64  0 = persistent sweep, 1/2/4 = flat grid with
 //                                 that many 16-byte units per thread
 //   diag.nt.c128 / diag.nt.c64     nontemporal loads/stores in the flat sweep
+//   tile.seg.c128 / tile.seg.c64   contiguous unit-address bits of a tile (4..6)
 int tune_get(const char *name);
 int tune_set(const char *name, int value);
 
@@ -197,15 +198,20 @@ struct U3Gate
     double u[8];
 };
 
-// Tile geometry: a tile pass reaches every target below tile_low_bits(prec)
-// and up to eight more; it needs a state of at least tile_min_bits(prec).
+// Tile geometry (tuning knob tile.seg.<prec> = 4, 5 or 6 contiguous
+// 16-byte-unit address bits): a tile pass reaches every target below tile_low_bits(prec)
+// and up to tile_upper_slots(prec) more; it needs a state of at least
+// tile_min_bits(prec) bits (12 for complex128, 13 for complex64).
+unsigned tile_seg(int prec);
 unsigned tile_low_bits(int prec);
+unsigned tile_upper_slots(int prec);
 unsigned tile_min_bits(int prec);
 // Applies commuting gates on distinct targets (a fused U3 cluster).
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates);
 // One tile pass applying commuting single-bit U3 gates (alpha == delta, any
-// number of targets below tile_low_bits and at most 8 at or above), then a
-// diagonal window whose selectors have at most one alpha bit, in one sweep.
+// number of targets below tile_low_bits and at most tile_upper_slots at or
+// above), then a diagonal window whose selectors have at most one alpha bit,
+// in one sweep.
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
                      const DiagGate *diag, size_t ndiag);
 // One general pair sweep (any delta / selector).

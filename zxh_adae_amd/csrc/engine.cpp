@@ -358,7 +358,7 @@ int Sim::flush_pass()
 // Adds single-bit U3 batches to the pending pass when exact: a gate joins
 // only if it commutes with the pass's diagonal window (its target is outside
 // the window's support), and the tile has room (targets below the tile's low
-// bits always fit, at most 8 others).  A second gate on a pass target is composed with
+// bits always fit, at most tile_upper_slots others).  A second gate on a pass target is composed with
 // the first (applied after it).
 int Sim::emit_u3(std::vector<U3Gate> &descs)
 {
@@ -408,7 +408,7 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
             continue;
         }
         const bool high = g.delta >> tile_low_bits(prec_) != 0;
-        if (high && pass_high_ >= 8)
+        if (high && unsigned(pass_high_) >= tile_upper_slots(prec_))
             ZXA_TRY(flush_pass());
         pass_u3_.push_back(g);
         if (high)

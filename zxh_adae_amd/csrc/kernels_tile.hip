@@ -631,11 +631,12 @@ struct SingleGate
 
 // Tile plan: the address bit of every tile element bit e (complex128: e is
 // the unit bit; complex64: e = 0 is address bit 0 inside a unit and e = k + 1
-// is unit bit k), and the unit bit of every target.  Unit bits 0..3 are the
-// lowest address bits; targets above them go to the register bits of
-// layout 0 (unit bits 8..11), then of layout 1 (4..7); the remaining slots
-// take the lowest other bits, those of layout 1 first (so that, in the load
-// layout, lanes 16..63 continue the contiguous run of lanes 0..15).
+// is unit bit k), and the unit bit of every target.  Unit bits 0..seg-1 are
+// the lowest unit-address bits (seg = 4, 5 or 6: 256 B, 512 B or 1 KiB
+// contiguous per wave access); targets above them go to the register bits of
+// layout 0 (unit bits 8..11), then of layout 1 from unit bit 7 down; the
+// remaining slots take the lowest other bits in ascending order (so that in
+// the load layout the contiguous run continues across lanes where it can).
 struct Plan
 {
     std::vector<unsigned> ebit; // tile element bit -> address bit
@@ -645,10 +646,11 @@ struct Plan
 bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl)
 {
     const bool c64 = prec == C64;
+    const int seg = int(tile_seg(prec));
     const unsigned low = tile_low_bits(prec);
     const unsigned shift = c64 ? 1u : 0u;
     std::vector<int> unit_addr(kUnitBits, -1); // unit bit -> address bit
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < seg; k++)
         unit_addr[k] = int(k + shift);
     std::vector<bool> used(64, false);
     for (unsigned b = 0; b < low; b++)
@@ -658,18 +660,20 @@ bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl
         if (s.target >= low)
             upper.push_back(s.target);
     std::sort(upper.begin(), upper.end());
-    if (upper.size() > 8 || n < tile_min_bits(prec))
+    if (upper.size() > tile_upper_slots(prec) || n < tile_min_bits(prec))
         return false;
-    const int order[8] = {8, 9, 10, 11, 4, 5, 6, 7};
-    size_t u = 0;
-    for (; u < upper.size(); u++)
+    // targets: layout-0 register bits, then layout-1 register bits from the top
+    std::vector<int> order;
+    for (int k : {8, 9, 10, 11, 7, 6, 5, 4})
+        if (k >= seg)
+            order.push_back(k);
+    for (size_t u = 0; u < upper.size(); u++)
     {
         unit_addr[order[u]] = int(upper[u]);
         used[upper[u]] = true;
     }
-    const int fill[8] = {4, 5, 6, 7, 8, 9, 10, 11};
     unsigned next = low;
-    for (int f : fill)
+    for (int f = seg; f < kUnitBits; f++)
     {
         if (unit_addr[f] >= 0)
             continue;
@@ -903,14 +907,25 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
 
 } // namespace
 
+unsigned tile_seg(int prec)
+{
+    const int s = tune_get(prec == C64 ? "tile.seg.c64" : "tile.seg.c128");
+    return s >= 4 && s <= 6 ? unsigned(s) : 4u;
+}
+
 unsigned tile_low_bits(int prec)
 {
-    return prec == C64 ? 5u : 4u;
+    return tile_seg(prec) + (prec == C64 ? 1u : 0u);
+}
+
+unsigned tile_upper_slots(int prec)
+{
+    return unsigned(kUnitBits) - tile_seg(prec);
 }
 
 unsigned tile_min_bits(int prec)
 {
-    return tile_low_bits(prec) + 8u;
+    return tile_low_bits(prec) + tile_upper_slots(prec);
 }
 
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates)
@@ -938,7 +953,7 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
             set_error("u3_apply: two gates on one target in one batch");
             return EINVAL_;
         }
-    // the low targets ride with the first pass; at most 8 others per pass
+    // the low targets ride with the first pass; the others fill the slots
     std::vector<SingleGate> lows, highs;
     for (const SingleGate &s : singles)
         (s.target < low ? lows : highs).push_back(s);
@@ -949,7 +964,7 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
         std::vector<SingleGate> pass;
         if (first)
             pass = lows;
-        const size_t take = std::min<size_t>(8, highs.size() - hi);
+        const size_t take = std::min<size_t>(tile_upper_slots(prec), highs.size() - hi);
         for (size_t k = 0; k < take; k++)
             pass.push_back(highs[hi + k]);
         hi += take;
@@ -996,9 +1011,9 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
             high++;
         gs.push_back({t, &g});
     }
-    if (high > 8)
+    if (unsigned(high) > tile_upper_slots(prec))
     {
-        set_error("fused pass: more than 8 targets above the tile's low bits");
+        set_error("fused pass: more targets above the tile's low bits than the tile has slots");
         return EINVAL_;
     }
     if (gs.empty())
