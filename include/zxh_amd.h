@@ -194,15 +194,74 @@ int zxh_k_copy(void *dst, const void *src, size_t bytes);
 /* state *= (re + i im) in place: renormalisation or a global phase (no
  * reference counterpart) */
 int zxh_k_scale(void *state, int precision, size_t local_bits, double re, double im);
+/* Expectation of a diagonal observable H = sum_k coef[k] Z_qi[k] Z_qj[k]
+ * (qj[k] < 0 for a single Z) on a plain-addressed state (qubit q = bit q):
+ * out[0] = <psi|H|psi>, out[1] = <psi|psi>, accumulated in FP64 with Kahan
+ * compensation and a fixed reduction order (deterministic).  No reference
+ * counterpart: the reference only samples. */
+int zxh_k_expect_z(const void *state, int precision, size_t local_bits, const int32_t *qi, const int32_t *qj,
+                   const double *coef, size_t nterms, double *out);
+/* dst = H src for the same observable (dst may not alias src); out (nullable)
+ * receives the same two sums as zxh_k_expect_z.  The adjoint method's first
+ * step.  No reference counterpart. */
+int zxh_k_apply_z(const void *src, void *dst, int precision, size_t local_bits, const int32_t *qi,
+                  const int32_t *qj, const double *coef, size_t nterms, double *out);
+/* ---- parameterized circuits: expectation values and adjoint gradients ----
+ * No reference counterpart (the reference has no gradient path).  Operations
+ * act on qubit q = address bit q; param >= 0 selects params[param] as the
+ * angle, param < 0 uses value.  RX/RY/RZ(theta) = exp(-i theta G / 2) with
+ * G = X/Y/Z; CZ and CP(theta) are diagonal. */
+enum
+{
+    ZXH_A_RX = 0,
+    ZXH_A_RY = 1,
+    ZXH_A_RZ = 2,
+    ZXH_A_H = 3,
+    ZXH_A_CZ = 4,
+    ZXH_A_CP = 5
+};
+typedef struct
+{
+    int32_t kind;
+    int32_t q0;
+    int32_t q1;
+    int32_t param;
+    double value;
+} zxh_ansatz_op;
+typedef struct zxh_ansatz zxh_ansatz;
+int zxh_ansatz_create(size_t n, int precision, const zxh_ansatz_op *ops, size_t nops, size_t nparams,
+                      zxh_ansatz **out);
+int zxh_ansatz_destroy(zxh_ansatz *a);
+/* observable: sum_k coef[k] Z_qi[k] Z_qj[k] (qj[k] < 0: one Z) */
+int zxh_ansatz_set_observable(zxh_ansatz *a, const int32_t *qi, const int32_t *qj, const double *coef,
+                              size_t nterms);
+/* out[0] = <psi(params)|H|psi(params)>, out[1] = <psi|psi> (FP64 sums); the
+ * final state stays on the device (zxh_ansatz_state) */
+int zxh_ansatz_forward(zxh_ansatz *a, const double *params, double *out);
+/* grads[nparams] = dE/dparams by the adjoint method (no stored intermediate
+ * states); re-runs the forward pass unless the last forward used the same
+ * params; out (nullable) as in forward.  Uncomputes the device state. */
+int zxh_ansatz_backward(zxh_ansatz *a, const double *params, double *grads, double *out);
+/* device address of the state and its width; tile passes per forward */
+int zxh_ansatz_state(zxh_ansatz *a, void **state, size_t *nbits);
+int zxh_ansatz_passes(zxh_ansatz *a, size_t *tile_passes);
+/* HIP-event timing of the ansatz's own stream (families as zxh_get_timing) */
+int zxh_ansatz_set_timing(zxh_ansatz *a, int on);
+int zxh_ansatz_get_timing(zxh_ansatz *a, int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
+int zxh_ansatz_reset_timing(zxh_ansatz *a);
+
 /* tuning knobs (process-wide), set by the autotuner (zxh_adae_amd.autotune):
  *   "diag.ept.c128", "diag.ept.c64": 0 = persistent diagonal sweep,
  *                                    1, 2, 4 = flat grid, 16-byte units per thread
  *   "diag.nt.c128", "diag.nt.c64": nontemporal loads/stores in the flat
- *                                  sweep (0/1) */
+ *                                  sweep (0/1)
+ *   "tile.seg.c128", "tile.seg.c64": 4, 5, 6 contiguous 16-byte-unit address
+ *                                    bits per tile pass (targets per pass: 12
+ *                                    minus this above them) */
 int zxh_tune_set(const char *name, int value);
 int zxh_tune_get(const char *name, int *value);
 /* HIP-event timing of the kernel-level calls (same families as zxh_get_timing,
- * plus 7 copy and 8 scale) */
+ * plus 7 copy, 8 scale, 9 fused pass, 10 observable, 11 adjoint pass) */
 int zxh_k_set_timing(int on);
 int zxh_k_get_timing(int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
 int zxh_k_reset_timing(void);

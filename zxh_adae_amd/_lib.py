@@ -22,6 +22,7 @@ _u64 = C.c_uint64
 _dp = C.POINTER(C.c_double)
 _u64p = C.POINTER(C.c_uint64)
 _vp = C.c_void_p
+_i32p = C.POINTER(C.c_int32)
 
 # (name, restype, argtypes); every function returns int status unless noted
 _SIGS = [
@@ -81,6 +82,18 @@ _SIGS = [
     ("zxh_k_prob_total", C.c_int, [_vp, C.c_int, _sz, _dp]),
     ("zxh_k_copy", C.c_int, [_vp, _vp, _sz]),
     ("zxh_k_scale", C.c_int, [_vp, C.c_int, _sz, C.c_double, C.c_double]),
+    ("zxh_k_expect_z", C.c_int, [_vp, C.c_int, _sz, _i32p, _i32p, _dp, _sz, _dp]),
+    ("zxh_ansatz_create", C.c_int, [_sz, C.c_int, _vp, _sz, _sz, C.POINTER(_vp)]),
+    ("zxh_ansatz_destroy", C.c_int, [_vp]),
+    ("zxh_ansatz_set_observable", C.c_int, [_vp, _i32p, _i32p, _dp, _sz]),
+    ("zxh_ansatz_forward", C.c_int, [_vp, _dp, _dp]),
+    ("zxh_ansatz_backward", C.c_int, [_vp, _dp, _dp, _dp]),
+    ("zxh_ansatz_state", C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_sz)]),
+    ("zxh_ansatz_passes", C.c_int, [_vp, C.POINTER(_sz)]),
+    ("zxh_ansatz_set_timing", C.c_int, [_vp, C.c_int]),
+    ("zxh_ansatz_get_timing", C.c_int, [_vp, C.c_int, C.POINTER(_u64), _dp, C.POINTER(_u64)]),
+    ("zxh_ansatz_reset_timing", C.c_int, [_vp]),
+    ("zxh_k_apply_z", C.c_int, [_vp, _vp, C.c_int, _sz, _i32p, _i32p, _dp, _sz, _dp]),
     ("zxh_tune_set", C.c_int, [C.c_char_p, C.c_int]),
     ("zxh_tune_get", C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),
     ("zxh_k_set_timing", C.c_int, [C.c_int]),

@@ -1,0 +1,132 @@
+"""Parameterized circuits on the MI355X engine: expectation values of
+diagonal Z / ZZ observables and their adjoint gradients (C-ABI
+zxh_ansatz_*; no reference counterpart).
+
+    a = Ansatz(n, ops, nparams, precision="c64")
+    a.set_observable([(0.5, 3), (-1.2, 4, 7)])     # 0.5 Z3 - 1.2 Z4 Z7
+    e, norm = a.forward(params)
+    grads, e = a.backward(params)
+
+ops are (kind, q0, q1, param, value) with kind one of RX, RY, RZ, H, CZ, CP;
+param >= 0 takes params[param] as the angle, otherwise value is used.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Sequence, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+RX, RY, RZ, H, CZ, CP = 0, 1, 2, 3, 4, 5
+
+
+class AnsatzOp(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("q0", C.c_int32), ("q1", C.c_int32), ("param", C.c_int32),
+                ("value", C.c_double)]
+
+
+FAMILIES = {"diag": 0, "u3_tile": 1, "reset": 6, "fused": 9, "obs": 10, "adjoint": 11}
+
+
+class Ansatz:
+    def __init__(self, n: int, ops: Sequence[tuple], nparams: int, precision: str = "c64"):
+        self.lib = _lib.load()
+        self.n = int(n)
+        self.nparams = int(nparams)
+        self.precision = precision
+        arr = (AnsatzOp * max(len(ops), 1))()
+        for i, o in enumerate(ops):
+            kind, q0, q1, param, value = o
+            arr[i].kind, arr[i].q0, arr[i].q1, arr[i].param, arr[i].value = kind, q0, q1, param, value
+        h = C.c_void_p()
+        prec = _lib.ZXH_C64 if precision in ("c64", "complex64") else _lib.ZXH_C128
+        check(self.lib.zxh_ansatz_create(self.n, prec, C.cast(arr, C.c_void_p), len(ops), self.nparams,
+                                         C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.zxh_ansatz_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_observable(self, terms: Sequence[tuple]):
+        """terms: (coef, i) for coef * Z_i, (coef, i, j) for coef * Z_i Z_j."""
+        qi = np.array([int(t[1]) for t in terms], dtype=np.int32)
+        qj = np.array([int(t[2]) if len(t) > 2 else -1 for t in terms], dtype=np.int32)
+        cf = np.array([float(t[0]) for t in terms], dtype=np.float64)
+        self._obs = (qi, qj, cf)
+        check(self.lib.zxh_ansatz_set_observable(self._h, qi.ctypes.data_as(_lib._i32p),
+                                                 qj.ctypes.data_as(_lib._i32p), cf.ctypes.data_as(_lib._dp),
+                                                 len(cf)))
+
+    def _params(self, params) -> np.ndarray:
+        p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        if p.size != self.nparams:
+            raise ValueError(f"expected {self.nparams} parameters, got {p.size}")
+        return p
+
+    def forward(self, params) -> Tuple[float, float]:
+        """(<psi|H|psi>, <psi|psi>); the state stays on the device."""
+        p = self._params(params)
+        out = np.zeros(2)
+        check(self.lib.zxh_ansatz_forward(self._h, p.ctypes.data_as(_lib._dp), out.ctypes.data_as(_lib._dp)))
+        return float(out[0]), float(out[1])
+
+    def backward(self, params) -> Tuple[np.ndarray, float]:
+        """(dE/dparams, E) by the adjoint method."""
+        p = self._params(params)
+        g = np.zeros(max(self.nparams, 1))
+        out = np.zeros(2)
+        check(self.lib.zxh_ansatz_backward(self._h, p.ctypes.data_as(_lib._dp), g.ctypes.data_as(_lib._dp),
+                                           out.ctypes.data_as(_lib._dp)))
+        return g[:self.nparams], float(out[0])
+
+    def state(self) -> np.ndarray:
+        """Host copy of the device state (after forward)."""
+        ptr = C.c_void_p()
+        nb = C.c_size_t()
+        check(self.lib.zxh_ansatz_state(self._h, C.byref(ptr), C.byref(nb)))
+        dt = np.complex64 if self.precision in ("c64", "complex64") else np.complex128
+        out = np.empty(1 << nb.value, dtype=dt)
+        check(self.lib.zxh_dev_download(out.ctypes.data, ptr, out.nbytes))
+        return out
+
+    def tile_passes(self) -> int:
+        v = C.c_size_t()
+        check(self.lib.zxh_ansatz_passes(self._h, C.byref(v)))
+        return v.value
+
+    def set_timing(self, on: bool):
+        check(self.lib.zxh_ansatz_set_timing(self._h, 1 if on else 0))
+
+    def reset_timing(self):
+        check(self.lib.zxh_ansatz_reset_timing(self._h))
+
+    def timing(self, family: str) -> Dict[str, float]:
+        launches = C.c_uint64()
+        ms = C.c_double()
+        nbytes = C.c_uint64()
+        check(self.lib.zxh_ansatz_get_timing(self._h, FAMILIES[family], C.byref(launches), C.byref(ms),
+                                             C.byref(nbytes)))
+        return {"launches": launches.value, "ms": ms.value, "bytes": nbytes.value}
+
+
+def two_local(n: int, layers: int):
+    """The two-local ansatz: per layer RY(theta) then RZ(phi) on every qubit,
+    then CZ(q, q + 1) for q = 0 .. n - 2.  Returns (ops, nparams)."""
+    ops = []
+    k = 0
+    for _ in range(layers):
+        for q in range(n):
+            ops.append((RY, q, 0, k, 0.0))
+            ops.append((RZ, q, 0, k + 1, 0.0))
+            k += 2
+        for q in range(n - 1):
+            ops.append((CZ, q, q + 1, -1, 0.0))
+    return ops, k

@@ -2,7 +2,9 @@
 #include "../../include/zxh_amd.h"
 
 #include "engine.h"
+#include "ansatz.h"
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 
@@ -11,6 +13,11 @@ using zxa::Sim;
 struct zxh_sim
 {
     Sim *sim;
+};
+
+struct zxh_ansatz
+{
+    zxa::Ansatz *a;
 };
 
 namespace
@@ -488,6 +495,152 @@ int zxh_k_scale(void *state, int precision, size_t local_bits, double re, double
         return rc;
     WITH_CTX(c);
     return finish(c, zxa::scale_apply(*c, state, precision, unsigned(local_bits), re, im));
+}
+
+static int observable_call(const void *src, void *dst, int precision, size_t local_bits, const int32_t *qi,
+                           const int32_t *qj, const double *coef, size_t nterms, double *out)
+{
+    int rc = check_kernel_args(src, precision, local_bits);
+    if (rc)
+        return rc;
+    if (nterms > 0 && (!qi || !qj || !coef))
+        return fail(ZXH_EINVAL, "null observable arrays");
+    if (dst && dst == src)
+        return fail(ZXH_EINVAL, "observable: dst aliases src");
+    WITH_CTX(c);
+    return finish(c, zxa::observable_apply(*c, src, dst, precision, unsigned(local_bits), qi, qj, coef, nterms, out));
+}
+
+int zxh_k_expect_z(const void *state, int precision, size_t local_bits, const int32_t *qi, const int32_t *qj,
+                   const double *coef, size_t nterms, double *out)
+{
+    if (!out)
+        return fail(ZXH_EINVAL, "null output");
+    return observable_call(state, nullptr, precision, local_bits, qi, qj, coef, nterms, out);
+}
+
+int zxh_k_apply_z(const void *src, void *dst, int precision, size_t local_bits, const int32_t *qi,
+                  const int32_t *qj, const double *coef, size_t nterms, double *out)
+{
+    if (!dst)
+        return fail(ZXH_EINVAL, "null destination");
+    return observable_call(src, dst, precision, local_bits, qi, qj, coef, nterms, out);
+}
+
+// ---- parameterized circuits ----
+
+#define CHECK_ANS(x)                                                                                                   \
+    if (!(x) || !(x)->a)                                                                                               \
+        return fail(ZXH_EINVAL, "null ansatz handle");
+
+int zxh_ansatz_create(size_t n, int precision, const zxh_ansatz_op *ops, size_t nops, size_t nparams,
+                      zxh_ansatz **out)
+{
+    if (!out)
+        return fail(ZXH_EINVAL, "null output handle");
+    *out = nullptr;
+    if (!zxa::runtime_active())
+        return fail(ZXH_ESTATE, "init must be called before constructing or using ansatz objects");
+    if (precision != ZXH_C128 && precision != ZXH_C64)
+        return fail(ZXH_EINVAL, "precision must be ZXH_C128 or ZXH_C64");
+    if (nops > 0 && !ops)
+        return fail(ZXH_EINVAL, "null operation array");
+    static_assert(sizeof(zxh_ansatz_op) == sizeof(zxa::AnsatzOp), "operation layout");
+    zxa::Ansatz *a = new zxa::Ansatz(unsigned(std::min<size_t>(n, 64)), precision, nparams);
+    int rc = a->init(reinterpret_cast<const zxa::AnsatzOp *>(ops), nops);
+    if (rc)
+    {
+        delete a;
+        return rc;
+    }
+    *out = new zxh_ansatz{a};
+    return ZXH_OK;
+}
+
+int zxh_ansatz_destroy(zxh_ansatz *a)
+{
+    if (!a)
+        return ZXH_OK;
+    delete a->a;
+    delete a;
+    return ZXH_OK;
+}
+
+int zxh_ansatz_set_observable(zxh_ansatz *a, const int32_t *qi, const int32_t *qj, const double *coef,
+                              size_t nterms)
+{
+    CHECK_ANS(a);
+    if (nterms > 0 && (!qi || !qj || !coef))
+        return fail(ZXH_EINVAL, "null observable arrays");
+    return a->a->set_observable(qi, qj, coef, nterms);
+}
+
+int zxh_ansatz_forward(zxh_ansatz *a, const double *params, double *out)
+{
+    CHECK_ANS(a);
+    if (!params || !out)
+        return fail(ZXH_EINVAL, "null params or output");
+    return a->a->forward(params, out);
+}
+
+int zxh_ansatz_backward(zxh_ansatz *a, const double *params, double *grads, double *out)
+{
+    CHECK_ANS(a);
+    if (!params || !grads)
+        return fail(ZXH_EINVAL, "null params or gradient output");
+    return a->a->backward(params, grads, out);
+}
+
+int zxh_ansatz_state(zxh_ansatz *a, void **state, size_t *nbits)
+{
+    CHECK_ANS(a);
+    if (state)
+        *state = a->a->state();
+    if (nbits)
+        *nbits = a->a->nbits();
+    return ZXH_OK;
+}
+
+int zxh_ansatz_passes(zxh_ansatz *a, size_t *tile_passes)
+{
+    CHECK_ANS(a);
+    if (!tile_passes)
+        return fail(ZXH_EINVAL, "null output");
+    *tile_passes = a->a->tile_passes();
+    return ZXH_OK;
+}
+
+int zxh_ansatz_set_timing(zxh_ansatz *a, int on)
+{
+    CHECK_ANS(a);
+    a->a->ctx().timing = on != 0;
+    return ZXH_OK;
+}
+
+int zxh_ansatz_get_timing(zxh_ansatz *a, int family, uint64_t *launches, double *total_ms, uint64_t *bytes)
+{
+    CHECK_ANS(a);
+    if (family < 0 || family >= zxa::F_COUNT)
+        return fail(ZXH_EINVAL, "timing family out of range");
+    if (!launches || !total_ms || !bytes)
+        return fail(ZXH_EINVAL, "null output");
+    int rc = a->a->ctx().collect();
+    if (rc)
+        return rc;
+    const zxa::FamilyTiming &f = a->a->ctx().fam[family];
+    *launches = f.launches;
+    *total_ms = f.ms;
+    *bytes = f.bytes;
+    return ZXH_OK;
+}
+
+int zxh_ansatz_reset_timing(zxh_ansatz *a)
+{
+    CHECK_ANS(a);
+    int rc = a->a->ctx().collect();
+    for (auto &f : a->a->ctx().fam)
+        f = zxa::FamilyTiming();
+    return rc;
 }
 
 int zxh_tune_set(const char *name, int value)

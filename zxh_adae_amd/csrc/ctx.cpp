@@ -159,6 +159,8 @@ void Ctx::release()
     scratch_b.release();
     scratch_c.release();
     scratch_d.release();
+    scratch_e.release();
+    scratch_f.release();
     (void)collect();
     for (hipEvent_t e : pool_)
         (void)hipEventDestroy(e);

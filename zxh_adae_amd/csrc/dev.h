@@ -86,7 +86,9 @@ enum Family
     F_COPY = 7,
     F_SCALE = 8,
     F_FUSED = 9,
-    F_COUNT = 10
+    F_OBS = 10,
+    F_ADJ = 11,
+    F_COUNT = 12
 };
 
 struct FamilyTiming
@@ -129,6 +131,8 @@ struct Ctx
     DevBuf scratch_b; // sampler tree levels
     DevBuf scratch_c; // sampler thresholds / results
     DevBuf scratch_d; // sampler per-segment tails
+    DevBuf scratch_e; // observable per-workgroup partial sums
+    DevBuf scratch_f; // adjoint per-wave transition accumulators
     bool timing = false;
     FamilyTiming fam[F_COUNT];
     // sampler state between prepare and search (tree in scratch_b, tails in scratch_d)
@@ -214,6 +218,23 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gate
 // in one sweep.
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
                      const DiagGate *diag, size_t ndiag);
+// One target of an adjoint pass: the uncompute matrix m (= U^dagger of the
+// forward pass's fused single-qubit gate on this qubit, (re, im) x 4), up to
+// two transition operators a[k] (2x2 Hermitian; zero when unused) whose
+// values Im <lam| a[k] |psi> are added to grads[param[k]] (param < 0: none).
+struct AdjTarget
+{
+    unsigned target;
+    double m[8];
+    double a[2][8];
+    int param[2];
+};
+// The reverse of one forward tile pass on psi and lam together (plain
+// addressing, single-bit targets): transition values, then m on both states,
+// then the diagonal window (given with the angles to apply) on both.
+// dgrads: device array of parameter gradients, accumulated in stream order.
+int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned nbits, const AdjTarget *targets,
+                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads);
 // One general pair sweep (any delta / selector).
 int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
 
@@ -238,6 +259,13 @@ int sample_apply(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg
                  size_t shots, uint64_t *out_real);
 // Root of the pairwise |psi|^2 tree (the reference's total probability).
 int prob_total(Ctx &ctx, const void *state, int prec, unsigned m, double *out);
+
+// Diagonal Z / ZZ observable H = sum_k coef[k] Z_qi[k] Z_qj[k] (qj[k] < 0:
+// one Z) on a state of n address bits (qubit q = address bit q).  out2
+// (host, nullable) receives <psi|H|psi> and <psi|psi>, both accumulated in
+// FP64 with compensation.  With dst non-null also writes dst = H psi.
+int observable_apply(Ctx &ctx, const void *src, void *dst, int prec, unsigned nbits, const int32_t *qi,
+                     const int32_t *qj, const double *coef, size_t nterms, double *out2);
 
 // Host threshold stream (measure_common.cpp:14-71) from an already mixed seed.
 void threshold_stream(uint64_t stream_seed, size_t cnt, double begin, double end, double *out);

@@ -1,0 +1,406 @@
+// kernels_obs.hip -- diagonal Pauli-Z observables: expectation values,
+// compensated norms, and the observable applied to a state.
+//
+// There is no counterpart in the reference (its engine measures by sampling
+// only).  An observable H = sum_k c_k Z_{i_k} [Z_{j_k}] is diagonal in the
+// computational basis; with s_q = 1 - 2 x_q its value on basis state x is a
+// polynomial of degree <= 2 in the address bits,
+//     h(x) = K + sum_b L_b x_b + sum_{b<c} Q_bc x_b x_c .
+// The sweep splits the address like the diagonal phase sweep: lane bits
+// (one 16-byte unit per thread, 256 threads), six row bits iterated by every
+// thread in Gray-code order, and block bits, so that h costs one addition per
+// amplitude.  Reductions accumulate in FP64 with Kahan compensation per
+// thread, then a fixed-order tree per workgroup and a fixed-order final pass:
+// the result is deterministic and, for complex64 states, carries FP64
+// accuracy over the FP32 amplitudes.
+#include "dev.h"
+#include "poly.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+namespace zxa
+{
+namespace
+{
+
+constexpr int kThreads = 256;
+constexpr int kRowBits = 6;
+constexpr int kRows = 1 << kRowBits;
+constexpr int kCoefStride = 16; // matches launch_block_coef's layout
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+#define ZXA_CONST const __attribute__((address_space(4)))
+
+template <bool kC64>
+struct OGeo
+{
+    static constexpr int kEPU = kC64 ? 2 : 1;          // amplitudes per 16-byte unit
+    static constexpr int kLane = 8 + (kC64 ? 1 : 0);  // lane element bits
+    static constexpr int kBlock = kLane + kRowBits;    // block element bits
+};
+
+struct Kahan
+{
+    double s = 0.0, c = 0.0;
+    __device__ __forceinline__ void add(double v)
+    {
+        const double y = v - c;
+        const double t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// Per-thread sums -> per-workgroup partial (fixed-order tree in LDS).
+__device__ __forceinline__ void wg_reduce2(double a, double b, double *sh, double *out2)
+{
+    const int t = threadIdx.x;
+    sh[t] = a;
+    sh[kThreads + t] = b;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1)
+    {
+        if (t < w)
+        {
+            sh[t] += sh[t + w];
+            sh[kThreads + t] += sh[kThreads + t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+    {
+        out2[0] = sh[0];
+        out2[1] = sh[kThreads];
+    }
+}
+
+// ltab layout (host-built, per thread t and lane element e < kEPU):
+//   A[t][e], d[t][e][j] (j < 6)      static lane part and lane-row couplings
+//   then R[r] (r < 64)               static row part
+// coef: per block, c_b for the kBlock inner bits and C at [kBlock]
+// (launch_block_coef).  kApply: also write dst = h * src.
+template <bool kC64, bool kApply>
+__global__ __launch_bounds__(kThreads) void obs_kernel(const void *__restrict__ src, void *__restrict__ dst,
+                                                       const double *__restrict__ ltab,
+                                                       const double *__restrict__ coef, uint64_t nblocks,
+                                                       double *__restrict__ partial)
+{
+    using G = OGeo<kC64>;
+    constexpr int E = G::kEPU;
+    __shared__ double sh[2 * kThreads];
+    const int t = threadIdx.x;
+    double A[E], d[E][kRowBits];
+#pragma unroll
+    for (int e = 0; e < E; e++)
+    {
+        const double *a = ltab + (size_t(t) * E + e) * (1 + kRowBits);
+        A[e] = a[0];
+#pragma unroll
+        for (int j = 0; j < kRowBits; j++)
+            d[e][j] = a[1 + j];
+    }
+    ZXA_CONST double *R = (ZXA_CONST double *)(ltab + size_t(kThreads) * E * (1 + kRowBits));
+    Kahan eh, en;
+    for (uint64_t blk = blockIdx.x; blk < nblocks; blk += gridDim.x)
+    {
+        ZXA_CONST double *c = (ZXA_CONST double *)(coef + blk * kCoefStride);
+        double T[E], D[E][kRowBits];
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+            // lane element bits: complex64 puts the in-unit bit first
+            const int x = kC64 ? (t << 1) | e : t;
+            double s = c[G::kBlock] + A[e];
+#pragma unroll
+            for (int b = 0; b < G::kLane; b++)
+                if ((x >> b) & 1)
+                    s += c[b];
+            T[e] = s;
+#pragma unroll
+            for (int j = 0; j < kRowBits; j++)
+                D[e][j] = d[e][j] + c[G::kLane + j];
+        }
+        const uint64_t base = blk << (G::kBlock - (kC64 ? 1 : 0)); // unit address of row 0
+        double g[E];
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            g[e] = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < kRows; k++)
+        {
+            const int r = k ^ (k >> 1);
+            if (k > 0)
+            {
+                const int j = __builtin_ctz(k);
+                const bool up = (r >> j) & 1;
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                    g[e] += up ? D[e][j] : -D[e][j];
+            }
+            const uint64_t u = base + (uint64_t(r) << 8) + t;
+            if constexpr (kC64)
+            {
+                const fvec4 v = __builtin_nontemporal_load(static_cast<const fvec4 *>(src) + u);
+                const double h0 = T[0] + R[r] + g[0];
+                const double h1 = T[1] + R[r] + g[1];
+                const double p0 = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+                const double p1 = double(v.z) * double(v.z) + double(v.w) * double(v.w);
+                eh.add(p0 * h0);
+                eh.add(p1 * h1);
+                en.add(p0);
+                en.add(p1);
+                if constexpr (kApply)
+                {
+                    const float f0 = float(h0), f1 = float(h1);
+                    __builtin_nontemporal_store(fvec4{v.x * f0, v.y * f0, v.z * f1, v.w * f1},
+                                                static_cast<fvec4 *>(dst) + u);
+                }
+            }
+            else
+            {
+                const dvec2 v = __builtin_nontemporal_load(static_cast<const dvec2 *>(src) + u);
+                const double h = T[0] + R[r] + g[0];
+                const double p = v.x * v.x + v.y * v.y;
+                eh.add(p * h);
+                en.add(p);
+                if constexpr (kApply)
+                    __builtin_nontemporal_store(dvec2{v.x * h, v.y * h}, static_cast<dvec2 *>(dst) + u);
+            }
+        }
+    }
+    wg_reduce2(eh.s, en.s, sh, partial + 2 * blockIdx.x);
+}
+
+// Small states (fewer address bits than one block): h evaluated per
+// amplitude from the polynomial.  poly: K, L[n], Q[n*n].
+template <bool kC64, bool kApply>
+__global__ __launch_bounds__(kThreads) void obs_small_kernel(const void *__restrict__ src, void *__restrict__ dst,
+                                                             const double *__restrict__ poly, int n,
+                                                             double *__restrict__ partial)
+{
+    __shared__ double sh[2 * kThreads];
+    Kahan eh, en;
+    const uint64_t N = uint64_t(1) << n;
+    for (uint64_t x = blockIdx.x * uint64_t(kThreads) + threadIdx.x; x < N; x += uint64_t(gridDim.x) * kThreads)
+    {
+        double h = poly[0];
+        for (int b = 0; b < n; b++)
+        {
+            if (!((x >> b) & 1))
+                continue;
+            h += poly[1 + b];
+            for (int c2 = b + 1; c2 < n; c2++)
+                if ((x >> c2) & 1)
+                    h += poly[1 + n + b * n + c2];
+        }
+        double p;
+        if constexpr (kC64)
+        {
+            const float2 v = static_cast<const float2 *>(src)[x];
+            p = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+            if constexpr (kApply)
+                static_cast<float2 *>(dst)[x] = make_float2(v.x * float(h), v.y * float(h));
+        }
+        else
+        {
+            const double2 v = static_cast<const double2 *>(src)[x];
+            p = v.x * v.x + v.y * v.y;
+            if constexpr (kApply)
+                static_cast<double2 *>(dst)[x] = make_double2(v.x * h, v.y * h);
+        }
+        eh.add(p * h);
+        en.add(p);
+    }
+    wg_reduce2(eh.s, en.s, sh, partial + 2 * blockIdx.x);
+}
+
+// Fixed-order final reduction of the per-workgroup partials (one workgroup).
+__global__ __launch_bounds__(kThreads) void partial_sum_kernel(const double *__restrict__ partial, int count,
+                                                               double *__restrict__ out2)
+{
+    __shared__ double sh[2 * kThreads];
+    Kahan a, b;
+    for (int i = threadIdx.x; i < count; i += kThreads)
+    {
+        a.add(partial[2 * i]);
+        b.add(partial[2 * i + 1]);
+    }
+    wg_reduce2(a.s, b.s, sh, out2);
+}
+
+// Observable polynomial from Z / ZZ terms (j < 0: single Z).
+void observable_poly(unsigned n, const int32_t *qi, const int32_t *qj, const double *coef, size_t nterms,
+                     Poly &P)
+{
+    P.n = n;
+    P.K = 0.0;
+    P.L.assign(n, 0.0);
+    P.Q.assign(size_t(n) * n, 0.0);
+    for (size_t k = 0; k < nterms; k++)
+    {
+        const double c = coef[k];
+        const int i = qi[k], j = qj[k];
+        if (j < 0 || j == i)
+        {
+            // Z_i = 1 - 2 x_i  (Z_i Z_i = 1)
+            if (j == i)
+            {
+                P.K += c;
+                continue;
+            }
+            P.K += c;
+            P.L[i] -= 2.0 * c;
+            continue;
+        }
+        // Z_i Z_j = 1 - 2 x_i - 2 x_j + 4 x_i x_j
+        P.K += c;
+        P.L[i] -= 2.0 * c;
+        P.L[j] -= 2.0 * c;
+        const int a = std::min(i, j), b = std::max(i, j);
+        P.Q[size_t(a) * n + b] += 4.0 * c;
+    }
+}
+
+template <typename K>
+int grid_of(Ctx &ctx, K kernel, uint64_t items)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kThreads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    const uint64_t cap = uint64_t(per_cu) * uint64_t(ctx.num_cus > 0 ? ctx.num_cus : 256);
+    return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(items, cap)));
+}
+
+} // namespace
+
+int observable_apply(Ctx &ctx, const void *src, void *dst, int prec, unsigned n, const int32_t *qi,
+                     const int32_t *qj, const double *coef, size_t nterms, double *out2)
+{
+    const bool c64 = prec == C64;
+    const bool apply = dst != nullptr;
+    for (size_t k = 0; k < nterms; k++)
+        if (qi[k] < 0 || unsigned(qi[k]) >= n || qj[k] >= int(n))
+        {
+            set_error("observable: qubit index outside the state");
+            return EINVAL_;
+        }
+    if (n > 40)
+    {
+        set_error("observable: state wider than 40 bits");
+        return EINVAL_;
+    }
+    Poly P;
+    observable_poly(n, qi, qj, coef, nterms, P);
+    const unsigned bb = c64 ? OGeo<true>::kBlock : OGeo<false>::kBlock;
+    const uint64_t bytes = (apply ? 2ull : 1ull) * (uint64_t(1) << n) * elem_bytes(prec);
+    int grid = 0;
+    if (n < bb)
+    {
+        std::vector<double> pt(1 + n + size_t(n) * n, 0.0);
+        pt[0] = P.K;
+        for (unsigned b = 0; b < n; b++)
+        {
+            pt[1 + b] = P.L[b];
+            for (unsigned c = b + 1; c < n; c++)
+                pt[1 + n + b * n + c] = P.q(int(b), int(c));
+        }
+        void *dpt = nullptr;
+        ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
+        const uint64_t N = uint64_t(1) << n;
+        grid = int(std::max<uint64_t>(1, std::min<uint64_t>((N + kThreads - 1) / kThreads, 1024)));
+        ZXA_TRY(ctx.scratch_e.ensure(size_t(grid) * 2 * sizeof(double) + 2 * sizeof(double)));
+        double *part = static_cast<double *>(ctx.scratch_e.ptr);
+        ZXA_TRY(ctx.tic());
+        auto k = c64 ? (apply ? obs_small_kernel<true, true> : obs_small_kernel<true, false>)
+                     : (apply ? obs_small_kernel<false, true> : obs_small_kernel<false, false>);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, src, dst, static_cast<const double *>(dpt),
+                           int(n), part);
+        ZXA_HIP(hipGetLastError());
+    }
+    else
+    {
+        // lane table (per thread and lane element) and row table
+        const int E = c64 ? 2 : 1;
+        const int LB = c64 ? OGeo<true>::kLane : OGeo<false>::kLane;
+        std::vector<double> lt(size_t(kThreads) * E * (1 + kRowBits) + kRows, 0.0);
+        for (int t = 0; t < kThreads; t++)
+            for (int e = 0; e < E; e++)
+            {
+                const int x = c64 ? (t << 1) | e : t;
+                double *a = lt.data() + (size_t(t) * E + e) * (1 + kRowBits);
+                double s = 0.0;
+                for (int b = 0; b < LB; b++)
+                {
+                    if (!((x >> b) & 1))
+                        continue;
+                    s += P.L[b];
+                    for (int c = b + 1; c < LB; c++)
+                        if ((x >> c) & 1)
+                            s += P.q(b, c);
+                }
+                a[0] = s;
+                for (int j = 0; j < kRowBits; j++)
+                {
+                    double dj = 0.0;
+                    for (int b = 0; b < LB; b++)
+                        if ((x >> b) & 1)
+                            dj += P.q(b, LB + j);
+                    a[1 + j] = dj;
+                }
+            }
+        double *R = lt.data() + size_t(kThreads) * E * (1 + kRowBits);
+        for (int r = 0; r < kRows; r++)
+        {
+            double s = 0.0;
+            for (int j = 0; j < kRowBits; j++)
+            {
+                if (!((r >> j) & 1))
+                    continue;
+                s += P.L[LB + j];
+                for (int k = j + 1; k < kRowBits; k++)
+                    if ((r >> k) & 1)
+                        s += P.q(LB + j, LB + k);
+            }
+            R[r] = s;
+        }
+        void *dlt = nullptr;
+        ZXA_TRY(ctx.ring.stage(lt.data(), lt.size() * sizeof(double), ctx.stream, &dlt));
+        const int nco = int(bb);
+        const int nh = int(n) - nco;
+        std::vector<int> inner(nco), outer(nh);
+        for (int i = 0; i < nco; i++)
+            inner[i] = i;
+        for (int i = 0; i < nh; i++)
+            outer[i] = nco + i;
+        const uint64_t nblocks = uint64_t(1) << nh;
+        ZXA_TRY(ctx.scratch_a.ensure(nblocks * kCoefStride * sizeof(double)));
+        double *coefs = static_cast<double *>(ctx.scratch_a.ptr);
+        ZXA_TRY(launch_block_coef(ctx, P, inner.data(), nco, outer.data(), nh, coefs));
+        auto k = c64 ? (apply ? obs_kernel<true, true> : obs_kernel<true, false>)
+                     : (apply ? obs_kernel<false, true> : obs_kernel<false, false>);
+        grid = grid_of(ctx, k, nblocks);
+        ZXA_TRY(ctx.scratch_e.ensure(size_t(grid) * 2 * sizeof(double) + 2 * sizeof(double)));
+        double *part = static_cast<double *>(ctx.scratch_e.ptr);
+        ZXA_TRY(ctx.tic());
+        hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, src, dst, static_cast<const double *>(dlt),
+                           static_cast<const double *>(coefs), nblocks, part);
+        ZXA_HIP(hipGetLastError());
+    }
+    double *part = static_cast<double *>(ctx.scratch_e.ptr);
+    double *res = part + size_t(grid) * 2;
+    hipLaunchKernelGGL(partial_sum_kernel, dim3(1), dim3(kThreads), 0, ctx.stream, part, grid, res);
+    ZXA_HIP(hipGetLastError());
+    ZXA_TRY(ctx.toc(F_OBS, bytes));
+    if (out2)
+    {
+        ZXA_HIP(hipMemcpyAsync(out2, res, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+        ZXA_HIP(hipStreamSynchronize(ctx.stream));
+    }
+    return ctx.ring.done(ctx.stream);
+}
+
+} // namespace zxa

@@ -330,7 +330,39 @@ struct Epi
             g[b] = {a[2 + 2 * b], a[3 + 2 * b]};
     }
 
-    __device__ __forceinline__ void apply(typename UnitT<kC64>::T (&v)[kRegs], ZXA_CONST double *tp, int t) const
+    // Multiplies element rho of v by ph.
+    static __device__ __forceinline__ void mul(typename UnitT<kC64>::T (&v)[kRegs], int rho, cd ph)
+    {
+        if constexpr (kC64)
+        {
+            const float px = static_cast<float>(ph.x), py = static_cast<float>(ph.y);
+            fvec4 &u = v[rho >> 1];
+            if (rho & 1)
+            {
+                const float zx = u.z * px - u.w * py, zy = u.z * py + u.w * px;
+                u.z = zx;
+                u.w = zy;
+            }
+            else
+            {
+                const float xx = u.x * px - u.y * py, xy = u.x * py + u.y * px;
+                u.x = xx;
+                u.y = xy;
+            }
+        }
+        else
+        {
+            dvec2 &u = v[rho];
+            const double nx = u.x * ph.x - u.y * ph.y, ny = u.x * ph.y + u.y * ph.x;
+            u.x = nx;
+            u.y = ny;
+        }
+    }
+
+    // The window's phases on v and, when kTwo, the same phases on w.
+    template <bool kTwo>
+    __device__ __forceinline__ void apply_n(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w2)[kRegs],
+                                            ZXA_CONST double *tp, int t) const
     {
         cd w = st;
 #pragma unroll
@@ -354,31 +386,15 @@ struct Epi
                 w = cmul(w, gb);
             }
             const cd ph = cmul(w, {D[2 * rho], D[2 * rho + 1]});
-            if constexpr (kC64)
-            {
-                const float px = static_cast<float>(ph.x), py = static_cast<float>(ph.y);
-                fvec4 &u = v[rho >> 1];
-                if (rho & 1)
-                {
-                    const float zx = u.z * px - u.w * py, zy = u.z * py + u.w * px;
-                    u.z = zx;
-                    u.w = zy;
-                }
-                else
-                {
-                    const float xx = u.x * px - u.y * py, xy = u.x * py + u.y * px;
-                    u.x = xx;
-                    u.y = xy;
-                }
-            }
-            else
-            {
-                dvec2 &u = v[rho];
-                const double nx = u.x * ph.x - u.y * ph.y, ny = u.x * ph.y + u.y * ph.x;
-                u.x = nx;
-                u.y = ny;
-            }
+            mul(v, rho, ph);
+            if constexpr (kTwo)
+                mul(w2, rho, ph);
         }
+    }
+
+    __device__ __forceinline__ void apply(typename UnitT<kC64>::T (&v)[kRegs], ZXA_CONST double *tp, int t) const
+    {
+        apply_n<false>(v, v, tp, t);
     }
 };
 
@@ -502,6 +518,226 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         b1 = b3;
         m1 = m3;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint pass: the reverse of one forward tile pass, on two states at once.
+// psi is the ansatz state and lam = (later gates)^dagger H psi, both taken at
+// the point just after the forward pass's single-qubit gates.  For every
+// target slot the pass accumulates up to two transition values
+//     Im <lam| A |psi>      (A: a 2x2 Hermitian operator on the slot's qubit,
+//                            the generator of a rotation conjugated by the
+//                            gates composed after it; the rotation's gradient)
+// and then applies the slot's uncompute matrix M = U^dagger to both states.
+// The diagonal epilogue (if any) is the previous forward pass's window with
+// negated angles, applied to both states.
+// ---------------------------------------------------------------------------
+
+constexpr int kObs = 2;                     // transition operators per slot
+constexpr int kAccs = (kUnitBits + 1) * kObs; // accumulator lanes (< 64)
+
+struct TObs
+{
+    double a[kObs][8]; // A (re, im) per operator; zero when unused
+    float af[kObs][8];
+};
+struct TAdj
+{
+    TObs o[kUnitBits + 1];
+};
+
+// Im(conj(l0) (A00 p0 + A01 p1) + conj(l1) (A10 p0 + A11 p1))
+template <typename R>
+__device__ __forceinline__ R pair_im(R p0x, R p0y, R p1x, R p1y, R l0x, R l0y, R l1x, R l1y, const R (&a)[8])
+{
+    const R a0x = a[0] * p0x - a[1] * p0y + a[2] * p1x - a[3] * p1y;
+    const R a0y = a[0] * p0y + a[1] * p0x + a[2] * p1y + a[3] * p1x;
+    const R a1x = a[4] * p0x - a[5] * p0y + a[6] * p1x - a[7] * p1y;
+    const R a1y = a[4] * p0y + a[5] * p0x + a[6] * p1y + a[7] * p1x;
+    return l0x * a0y - l0y * a0x + l1x * a1y - l1y * a1x;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int m = 32; m > 0; m >>= 1)
+        v += __shfl_xor(v, m);
+    return v;
+}
+
+template <bool kC64>
+__device__ __forceinline__ void load_obs(Real<kC64> (&a)[8], ZXA_CONST TObs &o, int k)
+{
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+    {
+        if constexpr (kC64)
+            a[i] = o.af[k][i];
+        else
+            a[i] = o.a[k][i];
+    }
+}
+
+// Slot s of layout L (register bit i; s = kIntra for the complex64 in-unit
+// bit): transition values into the lane-indexed accumulator, then M on both.
+template <bool kC64, int L>
+__device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
+                                           ZXA_CONST TTab *G, ZXA_CONST TAdj *A, double &acc, int lane)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        const int k = reg_bit(L, i);
+#pragma unroll
+        for (int o = 0; o < kObs; o++)
+        {
+            Real<kC64> a[8];
+            load_obs<kC64>(a, A->o[k], o);
+            double part = 0.0;
+#pragma unroll
+            for (int r = 0; r < kRegs; r++)
+            {
+                if ((r >> i) & 1)
+                    continue;
+                const int r1 = r | (1 << i);
+                if constexpr (kC64)
+                {
+                    part += double(pair_im<float>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x,
+                                                  w[r1].y, a));
+                    part += double(pair_im<float>(v[r].z, v[r].w, v[r1].z, v[r1].w, w[r].z, w[r].w, w[r1].z,
+                                                  w[r1].w, a));
+                }
+                else
+                {
+                    part += pair_im<double>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x, w[r1].y, a);
+                }
+            }
+            const double tot = wave_sum(part);
+            acc += lane == k * kObs + o ? tot : 0.0;
+        }
+        Real<kC64> m[8];
+        load_matrix<kC64>(m, G->g[k]);
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+        {
+            if ((r >> i) & 1)
+                continue;
+            unit_pair<kC64, false>(v[r], v[r | (1 << i)], m, false, false);
+            unit_pair<kC64, false>(w[r], w[r | (1 << i)], m, false, false);
+        }
+    }
+    if constexpr (kC64 && L == 0)
+    {
+#pragma unroll
+        for (int o = 0; o < kObs; o++)
+        {
+            float a[8];
+            load_obs<true>(a, A->o[kIntra], o);
+            double part = 0.0;
+#pragma unroll
+            for (int r = 0; r < kRegs; r++)
+                part += double(pair_im<float>(v[r].x, v[r].y, v[r].z, v[r].w, w[r].x, w[r].y, w[r].z, w[r].w, a));
+            const double tot = wave_sum(part);
+            acc += lane == kIntra * kObs + o ? tot : 0.0;
+        }
+        float m[8];
+        load_matrix<true>(m, G->g[kIntra]);
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+        {
+            const Pair<float> pv = pair2<float>(v[r].x, v[r].y, v[r].z, v[r].w, m, false);
+            v[r] = fvec4{pv.ax, pv.ay, pv.bx, pv.by};
+            const Pair<float> pw = pair2<float>(w[r].x, w[r].y, w[r].z, w[r].w, m, false);
+            w[r] = fvec4{pw.ax, pw.ay, pw.bx, pw.by};
+        }
+    }
+}
+
+// partial: per wave, kAccs doubles (lane-indexed accumulators).
+template <bool kC64, bool kL1, bool kL2, bool kDiag>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void adj_kernel(
+    void *__restrict__ psi, void *__restrict__ lam, const TPass P, const TTab *__restrict__ tabp,
+    const TAdj *__restrict__ adjp, const double *__restrict__ ttab, const double *__restrict__ tphase,
+    double *__restrict__ partial)
+{
+    using T = typename UnitT<kC64>::T;
+    __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    T *pp = static_cast<T *>(psi);
+    T *pl = static_cast<T *>(lam);
+    const uint64_t toff = thr_off<0>(t, P);
+    Epi<kC64> epi;
+    if constexpr (kDiag)
+        epi.init(ttab, t);
+    double acc = 0.0;
+    uint64_t base = deposit(uint64_t(blockIdx.x), P.sorted);
+    for (uint64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x)
+    {
+        ZXA_CONST TTab *G = tables(tabp);
+        const TAdj *A0 = adjp;
+        asm volatile("" : "+s"(A0));
+        ZXA_CONST TAdj *A = (ZXA_CONST TAdj *)A0;
+        T v[kRegs], w[kRegs];
+        load_tile<kC64>(v, pp, base | toff, G);
+        load_tile<kC64>(w, pl, base | toff, G);
+        adj_layout<kC64, 0>(v, w, G, A, acc, lane);
+        if constexpr (kL1)
+        {
+            transpose<kC64, 0, 1>(v, lds, t);
+            transpose<kC64, 0, 1>(w, lds, t);
+            adj_layout<kC64, 1>(v, w, G, A, acc, lane);
+            if constexpr (kL2)
+            {
+                transpose<kC64, 1, 2>(v, lds, t);
+                transpose<kC64, 1, 2>(w, lds, t);
+                adj_layout<kC64, 2>(v, w, G, A, acc, lane);
+                transpose<kC64, 2, 0>(v, lds, t);
+                transpose<kC64, 2, 0>(w, lds, t);
+            }
+            else
+            {
+                transpose<kC64, 1, 0>(v, lds, t);
+                transpose<kC64, 1, 0>(w, lds, t);
+            }
+        }
+        else if constexpr (kL2)
+        {
+            transpose<kC64, 0, 2>(v, lds, t);
+            transpose<kC64, 0, 2>(w, lds, t);
+            adj_layout<kC64, 2>(v, w, G, A, acc, lane);
+            transpose<kC64, 2, 0>(v, lds, t);
+            transpose<kC64, 2, 0>(w, lds, t);
+        }
+        if constexpr (kDiag)
+            epi.template apply_n<true>(v, w, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), t);
+        store_tile<kC64>(v, pp, base | toff, G);
+        store_tile<kC64>(w, pl, base | toff, G);
+        base = next_base(base, P);
+    }
+    if (lane < kAccs)
+        partial[(size_t(blockIdx.x) * (kThreads / 64) + (t >> 6)) * kAccs + lane] = acc;
+}
+
+// Sums the per-wave accumulators in a fixed order and adds slot/operator
+// (s, o)'s total into grads[param[s * kObs + o]] (param < 0: not a
+// gradient).  One thread per accumulator; distinct accumulators of one pass
+// may share a parameter, so the adds into grads run in one thread, in order.
+__global__ __launch_bounds__(64) void adj_collect_kernel(const double *__restrict__ partial, int nwaves,
+                                                         const int *__restrict__ param, double *__restrict__ grads)
+{
+    __shared__ double tot[64];
+    const int j = threadIdx.x;
+    double s = 0.0;
+    if (j < kAccs)
+        for (int w = 0; w < nwaves; w++)
+            s += partial[size_t(w) * kAccs + j];
+    tot[j] = s;
+    __syncthreads();
+    if (j == 0)
+        for (int a = 0; a < kAccs; a++)
+            if (param[a] >= 0)
+                grads[param[a]] += tot[a];
 }
 
 __device__ __forceinline__ cd cis(double a)
@@ -786,6 +1022,62 @@ void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, con
     *nb_out = nb;
 }
 
+// Stages the diagonal epilogue of a pass: the per-thread table and the
+// per-tile phase factors (tile_phase_kernel into scratch_a).
+int setup_epilogue(Ctx &ctx, int prec, unsigned n, const std::vector<unsigned> &pos, const DiagGate *diag,
+                   size_t ndiag, uint64_t ntiles, const double **ttab, const double **tphase)
+{
+    Poly poly;
+    window_polynomial(n, diag, ndiag, poly);
+    std::vector<double> tt, pt;
+    int nh = 0, nb = 0;
+    epilogue_tables(prec, n, pos, poly, tt, pt, &nh, &nb);
+    void *dtt = nullptr, *dpt = nullptr;
+    ZXA_TRY(ctx.ring.stage(tt.data(), tt.size() * sizeof(double), ctx.stream, &dtt));
+    ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
+    const size_t stride = 2 * (8 + (size_t(1) << nb));
+    ZXA_TRY(ctx.scratch_a.ensure(ntiles * stride * sizeof(double)));
+    const unsigned grid = unsigned((ntiles + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), pt.size() * sizeof(double), ctx.stream,
+                       static_cast<const double *>(dpt), int(pos.size()), nh, nb, poly.K, ntiles,
+                       static_cast<double *>(ctx.scratch_a.ptr));
+    ZXA_HIP(hipGetLastError());
+    *ttab = static_cast<const double *>(dtt);
+    *tphase = static_cast<const double *>(ctx.scratch_a.ptr);
+    return OK;
+}
+
+// Kernel arguments and tables of a pass from its plan (targets' matrices in
+// gs order; every other slot the identity).  use[L]: layout L has a target.
+void fill_pass(int prec, unsigned n, const Plan &pl, TPass &P, TTab &G, bool use[3])
+{
+    const bool c64 = prec == C64;
+    const std::vector<unsigned> &pos = pl.ebit;
+    std::memset(&P, 0, sizeof(P));
+    std::memset(&G, 0, sizeof(G));
+    const unsigned shift = c64 ? 1u : 0u;
+    for (int k = 0; k < kUnitBits; k++)
+    {
+        P.ub[k] = pos[k + shift] - shift;
+        P.sorted[k] = P.ub[k];
+        P.tmask |= uint64_t(1) << P.ub[k];
+    }
+    std::sort(P.sorted, P.sorted + kUnitBits);
+    for (int r = 0; r < kRegs; r++)
+        for (int i = 0; i < 4; i++)
+            if ((r >> i) & 1)
+                G.roff[r] |= uint64_t(1) << P.ub[reg_bit(0, i)];
+    P.ntiles = uint64_t(1) << (n - shift - kUnitBits);
+    for (TGate &tg : G.g)
+    {
+        tg.u[0] = tg.u[6] = 1.0;
+        tg.uf[0] = tg.uf[6] = 1.0f;
+    }
+    use[0] = use[1] = use[2] = false;
+    for (int slot : pl.slot)
+        use[slot == kIntra ? 0 : layout_of_unit_bit(slot)] = true;
+}
+
 // One tile pass over the given targets, then the diagonal window (if any).
 int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
                   const DiagGate *diag = nullptr, size_t ndiag = 0)
@@ -800,28 +1092,9 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     const std::vector<unsigned> &pos = pl.ebit;
     TPass P;
     TTab G;
-    std::memset(&P, 0, sizeof(P));
-    std::memset(&G, 0, sizeof(G));
+    bool use[3];
+    fill_pass(prec, n, pl, P, G, use);
     const unsigned shift = c64 ? 1u : 0u; // element bit -> unit-address bit
-    for (int k = 0; k < kUnitBits; k++)
-    {
-        P.ub[k] = pos[k + shift] - shift;
-        P.sorted[k] = P.ub[k];
-        P.tmask |= uint64_t(1) << P.ub[k];
-    }
-    std::sort(P.sorted, P.sorted + kUnitBits);
-    for (int r = 0; r < kRegs; r++)
-        for (int i = 0; i < 4; i++)
-            if ((r >> i) & 1)
-                G.roff[r] |= uint64_t(1) << P.ub[reg_bit(0, i)];
-    P.ntiles = uint64_t(1) << (n - shift - kUnitBits);
-    // every slot starts as the identity
-    for (TGate &tg : G.g)
-    {
-        tg.u[0] = tg.u[6] = 1.0;
-        tg.uf[0] = tg.uf[6] = 1.0f;
-    }
-    bool use[3] = {false, false, false};
     bool flip = false;
     for (size_t gi = 0; gi < gs.size(); gi++)
     {
@@ -863,7 +1136,6 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         }
         if (rest != 0)
             flip = true;
-        use[slot == kIntra ? 0 : layout_of_unit_bit(slot)] = true;
     }
     const double *ttab = nullptr;
     const double *tphase = nullptr;
@@ -874,23 +1146,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
             set_error("tile pass: diagonal epilogue with per-pair selectors");
             return EINVAL_;
         }
-        Poly poly;
-        window_polynomial(n, diag, ndiag, poly);
-        std::vector<double> tt, pt;
-        int nh = 0, nb = 0;
-        epilogue_tables(prec, n, pos, poly, tt, pt, &nh, &nb);
-        void *dtt = nullptr, *dpt = nullptr;
-        ZXA_TRY(ctx.ring.stage(tt.data(), tt.size() * sizeof(double), ctx.stream, &dtt));
-        ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
-        const size_t stride = 2 * (8 + (size_t(1) << nb));
-        ZXA_TRY(ctx.scratch_a.ensure(P.ntiles * stride * sizeof(double)));
-        const unsigned grid = unsigned((P.ntiles + kThreads - 1) / kThreads);
-        hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), pt.size() * sizeof(double), ctx.stream,
-                           static_cast<const double *>(dpt), int(pos.size()), nh, nb, poly.K, P.ntiles,
-                           static_cast<double *>(ctx.scratch_a.ptr));
-        ZXA_HIP(hipGetLastError());
-        ttab = static_cast<const double *>(dtt);
-        tphase = static_cast<const double *>(ctx.scratch_a.ptr);
+        ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase));
     }
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
@@ -903,6 +1159,38 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         ZXA_TRY(launch_p<false>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
     ZXA_TRY(ctx.toc(ndiag > 0 ? F_FUSED : F_U3TILE, bytes));
     return ctx.ring.done(ctx.stream);
+}
+
+template <bool kC64, bool kL1, bool kL2, bool kDiag>
+int launch_adj_t(Ctx &ctx, void *psi, void *lam, const TPass &P0, const TTab *tab, const TAdj *adj,
+                 const double *ttab, const double *tphase, double *partial, int *nwaves)
+{
+    auto k = adj_kernel<kC64, kL1, kL2, kDiag>;
+    const int grid = persistent_grid(ctx, k, P0.ntiles);
+    // the partial buffer holds *nwaves (on entry: its capacity in waves)
+    if (grid * (kThreads / 64) > *nwaves)
+    {
+        set_error("adjoint pass: partial buffer too small for the grid");
+        return EINVAL_;
+    }
+    TPass P = P0;
+    P.gstep = deposit(uint64_t(grid), P.sorted);
+    *nwaves = grid * (kThreads / 64);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, psi, lam, P, tab, adj, ttab, tphase, partial);
+    return hip_check(hipGetLastError(), "adj_kernel launch");
+}
+
+template <bool kC64, bool kDiag>
+int launch_adj_l(Ctx &ctx, void *psi, void *lam, const TPass &P, bool l1, bool l2, const TTab *tab,
+                 const TAdj *adj, const double *ttab, const double *tphase, double *partial, int *nwaves)
+{
+    if (l1 && l2)
+        return launch_adj_t<kC64, true, true, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
+    if (l1)
+        return launch_adj_t<kC64, true, false, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
+    if (l2)
+        return launch_adj_t<kC64, false, true, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
+    return launch_adj_t<kC64, false, false, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
 }
 
 } // namespace
@@ -1019,6 +1307,113 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
     if (gs.empty())
         return diag_apply(ctx, state, prec, n, diag, ndiag);
     return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag);
+}
+
+} // namespace zxa
+
+namespace zxa
+{
+
+int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, const AdjTarget *targets,
+                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads)
+{
+    const bool c64 = prec == C64;
+    if (n < tile_min_bits(prec) || n > 40)
+    {
+        set_error("adjoint pass: state narrower than a tile");
+        return EINVAL_;
+    }
+    if (ndiag > 0 && !window_is_polynomial(n, diag, ndiag))
+    {
+        set_error("adjoint pass: diagonal window is not a phase polynomial");
+        return EINVAL_;
+    }
+    std::vector<U3Gate> ug(ntargets);
+    std::vector<SingleGate> gs;
+    for (size_t i = 0; i < ntargets; i++)
+    {
+        const unsigned q = targets[i].target;
+        if (q >= n)
+        {
+            set_error("adjoint pass: target outside the state");
+            return EINVAL_;
+        }
+        for (size_t j = 0; j < i; j++)
+            if (targets[j].target == q)
+            {
+                set_error("adjoint pass: two entries on one target");
+                return EINVAL_;
+            }
+        ug[i].packed_sel = uint64_t(1) << q;
+        ug[i].delta = uint64_t(1) << q;
+        std::memcpy(ug[i].u, targets[i].m, sizeof(ug[i].u));
+        gs.push_back({q, &ug[i]});
+    }
+    Plan pl;
+    if (!plan_tile(prec, n, gs, pl))
+    {
+        set_error("adjoint pass: more targets than the tile has slots");
+        return EINVAL_;
+    }
+    TPass P;
+    TTab G;
+    bool use[3];
+    fill_pass(prec, n, pl, P, G, use);
+    TAdj A;
+    std::memset(&A, 0, sizeof(A));
+    std::vector<int> param(kAccs, -1);
+    for (size_t i = 0; i < ntargets; i++)
+    {
+        const int slot = pl.slot[i];
+        for (int k = 0; k < 8; k++)
+        {
+            G.g[slot].u[k] = targets[i].m[k];
+            G.g[slot].uf[k] = float(targets[i].m[k]);
+        }
+        for (int o = 0; o < kObs; o++)
+        {
+            for (int k = 0; k < 8; k++)
+            {
+                A.o[slot].a[o][k] = targets[i].a[o][k];
+                A.o[slot].af[o][k] = float(targets[i].a[o][k]);
+            }
+            param[slot * kObs + o] = targets[i].param[o];
+        }
+    }
+    const double *ttab = nullptr;
+    const double *tphase = nullptr;
+    if (ndiag > 0)
+        ZXA_TRY(setup_epilogue(ctx, prec, n, pl.ebit, diag, ndiag, P.ntiles, &ttab, &tphase));
+    void *dtab = nullptr, *dadj = nullptr, *dpar = nullptr;
+    ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
+    ZXA_TRY(ctx.ring.stage(&A, sizeof(A), ctx.stream, &dadj));
+    ZXA_TRY(ctx.ring.stage(param.data(), param.size() * sizeof(int), ctx.stream, &dpar));
+    // per-wave partials: at most (2 workgroups per CU) x CUs x 4 waves
+    const size_t maxw = size_t(8) * size_t(ctx.num_cus > 0 ? ctx.num_cus : 256) * (kThreads / 64);
+    ZXA_TRY(ctx.scratch_f.ensure(maxw * kAccs * sizeof(double)));
+    double *partial = static_cast<double *>(ctx.scratch_f.ptr);
+    const TTab *tab = static_cast<const TTab *>(dtab);
+    const TAdj *adj = static_cast<const TAdj *>(dadj);
+    const uint64_t bytes = 4ull * (uint64_t(1) << n) * elem_bytes(prec);
+    int nwaves = int(maxw);
+    ZXA_TRY(ctx.tic());
+    int rc;
+    if (c64)
+        rc = ndiag ? launch_adj_l<true, true>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
+                                              &nwaves)
+                   : launch_adj_l<true, false>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
+                                               &nwaves);
+    else
+        rc = ndiag ? launch_adj_l<false, true>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
+                                               &nwaves)
+                   : launch_adj_l<false, false>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
+                                                &nwaves);
+    ZXA_TRY(rc);
+    ZXA_TRY(ctx.toc(F_ADJ, bytes));
+    hipLaunchKernelGGL(adj_collect_kernel, dim3(1), dim3(64), 0, ctx.stream, partial, nwaves,
+                       static_cast<const int *>(dpar), dgrads);
+    ZXA_HIP(hipGetLastError());
+    return ctx.ring.done(ctx.stream);
 }
 
 } // namespace zxa

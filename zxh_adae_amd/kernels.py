@@ -104,6 +104,33 @@ class DeviceState:
                                         s0.ctypes.data_as(_lib._u64p), s1.ctypes.data_as(_lib._u64p),
                                         th.ctypes.data_as(_lib._dp), th.size))
 
+    @staticmethod
+    def _terms(terms: Sequence[tuple]):
+        """terms: (coef, i) for Z_i or (coef, i, j) for Z_i Z_j."""
+        qi = np.array([int(t[1]) for t in terms], dtype=np.int32)
+        qj = np.array([int(t[2]) if len(t) > 2 else -1 for t in terms], dtype=np.int32)
+        cf = np.array([float(t[0]) for t in terms], dtype=np.float64)
+        return qi, qj, cf
+
+    def expect_z(self, terms: Sequence[tuple]):
+        """(<psi|H|psi>, <psi|psi>) for H = sum of coef * Z_i [Z_j], FP64 sums."""
+        qi, qj, cf = self._terms(terms)
+        out = np.zeros(2)
+        check(self.lib.zxh_k_expect_z(self.ptr, self.prec, self.nbits, qi.ctypes.data_as(_lib._i32p),
+                                      qj.ctypes.data_as(_lib._i32p), cf.ctypes.data_as(_lib._dp), len(cf),
+                                      out.ctypes.data_as(_lib._dp)))
+        return float(out[0]), float(out[1])
+
+    def apply_z(self, dst: "DeviceState", terms: Sequence[tuple]):
+        """dst = H psi; returns the same sums as expect_z."""
+        assert dst.nbits == self.nbits and dst.prec == self.prec
+        qi, qj, cf = self._terms(terms)
+        out = np.zeros(2)
+        check(self.lib.zxh_k_apply_z(self.ptr, dst.ptr, self.prec, self.nbits, qi.ctypes.data_as(_lib._i32p),
+                                     qj.ctypes.data_as(_lib._i32p), cf.ctypes.data_as(_lib._dp), len(cf),
+                                     out.ctypes.data_as(_lib._dp)))
+        return float(out[0]), float(out[1])
+
     def diag(self, sel0: Sequence[int], sel1: Sequence[int], theta: Sequence[float]):
         s0 = np.ascontiguousarray(sel0, dtype=np.uint64)
         s1 = np.ascontiguousarray(sel1, dtype=np.uint64)
@@ -155,7 +182,7 @@ def sync():
 
 
 FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6, "copy": 7,
-            "scale": 8, "fused": 9}
+            "scale": 8, "fused": 9, "obs": 10, "adjoint": 11}
 
 
 def set_timing(on: bool):
