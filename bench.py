@@ -20,6 +20,12 @@ Reported beside it (same JSON line):
   u3_all        U3 on all 30 qubits as tile passes
   c64           the same QFT step with a complex64 state
   sampling      1e6 shots on the 30-qubit state
+  c5            33q complex64 and 32q complex128 random circuits of 5-qubit
+                Haar blocks (matrix cores), diagonal batches and MCX cascades,
+                40 layers, then 1e6 shots
+  adjoint_c3    two-local ansatz, 28 qubits complex64, 448 parameters, 200
+                Z/ZZ terms: forward and adjoint backward times, grads/s, and
+                the gradient error and state infidelity against complex128
   cpu_baseline  the reference's CPU path (oracle/_ref when built, else the
                 oracle port) on a bounded prefix of the same circuit, timed on
                 this host
@@ -137,6 +143,105 @@ def u3_all(Z, W, K, sim_state, prec, repeats=3):
     return {"gates": m, "passes": passes, "ms": round(kern_ms, 3), "wall_ms": round(wall * 1e3, 3),
             "gates/s": round(m / (kern_ms * 1e-3), 1), "GB/s_per_pass": round(gbps, 1),
             "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+
+
+def adjoint_c3(W, repeats=3, check_fp64=True):
+    """Two-local ansatz, 28 qubits, 8 layers (448 parameters), 200 Z/ZZ
+    terms: complex64 forward and adjoint backward, with the FP64 (complex128)
+    run of the same circuit for the gradient error and the state infidelity."""
+    import numpy as np
+    from zxh_adae_amd.ansatz import Ansatz, two_local
+    n, layers = 28, 8
+    ops, npar = two_local(n, layers)
+    params = np.array(W.ansatz_params(npar, 3))
+    terms = W.zz_observable(n, 200, 3)
+    a = Ansatz(n, ops, npar, "c64")
+    a.set_observable(terms)
+    a.forward(params)
+    a.backward(params)  # warm
+    tf = tb = 0.0
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        e, _ = a.forward(params)
+        t1 = time.perf_counter()
+        g32, _ = a.backward(params)  # reuses the forward state
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    tf /= repeats
+    tb /= repeats
+    a.reset_timing()
+    a.set_timing(True)
+    a.forward(params)
+    a.backward(params)
+    a.set_timing(False)
+    fams = {}
+    for fam in ("fused", "u3_tile", "diag", "obs", "adjoint", "reset"):
+        t = a.timing(fam)
+        if t["launches"]:
+            avg = t["ms"] / t["launches"]
+            fams[fam] = {"launches": t["launches"], "avg_ms": round(avg, 4),
+                         "GB/s": round(t["bytes"] / t["launches"] / (avg * 1e-3) / 1e9, 1)}
+    out = {"qubits": n, "layers": layers, "params": npar, "terms": len(terms), "state": "complex64",
+           "forward_ms": round(tf * 1e3, 3), "backward_ms": round(tb * 1e3, 3),
+           "backward_over_forward": round(tb / tf, 3), "grads/s": round(npar / (tf + tb), 1),
+           "energy": e, "tile_passes_per_forward": a.tile_passes(), "kernels": fams}
+    if check_fp64:
+        a.forward(params)
+        psi32 = a.state()
+        del a
+        b = Ansatz(n, ops, npar, "c128")
+        b.set_observable(terms)
+        e64, _ = b.forward(params)
+        psi64 = b.state()
+        g64, _ = b.backward(params)
+        ov = abs(np.vdot(psi64, psi32.astype(np.complex128)))
+        out["fp64"] = {"max_grad_error": float(np.max(np.abs(g32 - g64))),
+                       "max_abs_grad": float(np.max(np.abs(g64))), "energy_error": abs(e - e64),
+                       "infidelity": float(1.0 - ov),
+                       "max_amp_error": float(np.max(np.abs(psi32.astype(np.complex128) - psi64)))}
+        del psi32, psi64, b
+    return out
+
+
+def c5_circuit(W, K, prec, n, layers=40, shots=1000000, seed=5):
+    """The large random circuit: per layer floor(n/5) Haar 5-qubit blocks
+    (matrix cores), a 64-gate diagonal batch, 8 multi-controlled X gates;
+    then 1e6 shots.  Kernel times by HIP events; MFMA rate of the blocks."""
+    from zxh_adae_amd.kernels import DeviceState, apply_layer, mix_seed, set_basis_zero
+    spec = W.c5_layers(n, layers, seed)
+    st = DeviceState(n, prec)
+    set_basis_zero(st)
+    K.sync()
+    K.reset_timing()
+    K.set_timing(True)
+    t0 = time.perf_counter()
+    for L in spec:
+        apply_layer(st, L)
+    K.sync()
+    t_circ = time.perf_counter() - t0
+    fams = {}
+    for fam in ("block", "diag", "perm"):
+        t = K.timing(fam)
+        if t["launches"]:
+            avg = t["ms"] / t["launches"]
+            fams[fam] = {"launches": t["launches"], "avg_ms": round(avg, 4),
+                         "GB/s": round(t["bytes"] / t["launches"] / (avg * 1e-3) / 1e9, 1)}
+    blk = K.timing("block")
+    flops = 256.0 * (1 << n)  # 32 complex MACs = 8 * 2^5 real flops per amplitude
+    tflops = flops / (blk["ms"] / blk["launches"] * 1e-3) / 1e12
+    peak = 157.3 if prec == "c64" else 78.6
+    t1 = time.perf_counter()
+    st.sample(18, mix_seed(seed, 0), shots)
+    t_samp = time.perf_counter() - t1
+    K.set_timing(False)
+    st.free()
+    nblocks = sum(len(L["blocks"]) for L in spec)
+    ngates = nblocks + sum(len(L["diag"]) + len(L["mcx"]) for L in spec)
+    return {"qubits": n, "state": "complex64" if prec == "c64" else "complex128", "layers": layers,
+            "gates": ngates, "blocks": nblocks, "circuit_s": round(t_circ, 3), "gates/s": round(ngates / t_circ, 1),
+            "kernels": fams, "block_tflops": round(tflops, 2), "block_mfma_frac": round(tflops / peak, 4),
+            "mfma_peak_tflops": peak, "sampling": {"shots": shots, "seconds": round(t_samp, 3)}}
 
 
 def cpu_baseline(W, targets=4):
@@ -286,6 +391,14 @@ def main():
         result["diag_batches"]["c64"] = diag_batches(Z, W, K, sim64.device_state(), "c64")
         result["u3_all"]["c64"] = u3_all(Z, W, K, sim64.device_state(), "c64")
         del sim64
+
+    if not args.no_extras:
+        log("adjoint 28q c64 (two-local, 8 layers, 200 terms)")
+        result["adjoint_c3"] = adjoint_c3(W)
+        log("large random circuit 33q c64, 40 layers, 1e6 shots")
+        result["c5"] = {"c64": c5_circuit(W, K, "c64", 33)}
+        log("large random circuit 32q c128, 40 layers, 1e6 shots")
+        result["c5"]["c128"] = c5_circuit(W, K, "c128", 32)
 
     if not args.no_cpu_baseline:
         log("cpu baseline")

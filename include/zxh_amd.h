@@ -148,6 +148,7 @@ typedef struct {
     double u[8]; /* u00 re,im  u01 re,im  u10 re,im  u11 re,im */
 } zxh_u3_desc;
 
+/* zeroed device allocation (worker_alloc, mem.h:9-19) */
 int zxh_dev_alloc(size_t bytes, void **dptr);
 int zxh_dev_free(void *dptr);
 int zxh_dev_upload(void *dptr, const void *host, size_t bytes);
@@ -206,6 +207,14 @@ int zxh_k_expect_z(const void *state, int precision, size_t local_bits, const in
  * step.  No reference counterpart. */
 int zxh_k_apply_z(const void *src, void *dst, int precision, size_t local_bits, const int32_t *qi,
                   const int32_t *qj, const double *coef, size_t nterms, double *out);
+/* |0...0> in place: worker_set_zero + worker_mem_set (mem.h:9-19) */
+int zxh_k_reset(void *state, int precision, size_t local_bits);
+/* Dense 5-qubit unitary on physical bits targets[0..5) (bit b of the
+ * combination index is targets[b]): u is 32x32 complex, row-major (re, im)
+ * pairs.  One sweep on the matrix cores (FP32 accumulation for complex64,
+ * FP64 for complex128).  No reference counterpart. */
+int zxh_k_block5(void *state, int precision, size_t local_bits, const uint32_t *targets, const double *u);
+
 /* ---- parameterized circuits: expectation values and adjoint gradients ----
  * No reference counterpart (the reference has no gradient path).  Operations
  * act on qubit q = address bit q; param >= 0 selects params[param] as the
@@ -250,6 +259,22 @@ int zxh_ansatz_set_timing(zxh_ansatz *a, int on);
 int zxh_ansatz_get_timing(zxh_ansatz *a, int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
 int zxh_ansatz_reset_timing(zxh_ansatz *a);
 
+/* ---- batches of parameterized circuits (different widths and depths) ----
+ * One arena for every set's states; each forward pass index, the observables
+ * and each backward step run as one persistent launch over all sets, the
+ * whole sequence replayed from a hipGraph.  No reference counterpart. */
+typedef struct zxh_batch zxh_batch;
+int zxh_batch_create(int precision, zxh_batch **out);
+int zxh_batch_destroy(zxh_batch *b);
+int zxh_batch_add(zxh_batch *b, size_t n, const zxh_ansatz_op *ops, size_t nops, size_t nparams,
+                  const int32_t *qi, const int32_t *qj, const double *coef, size_t nterms);
+/* allocates the arena, prepares the launches, captures the graphs */
+int zxh_batch_finalize(zxh_batch *b);
+/* params: all sets' parameters concatenated in set order; energies[2 * sets]
+ * (<H>, <psi|psi>); grads (nullable: forward only) concatenated like params */
+int zxh_batch_run(zxh_batch *b, const double *params, double *energies, double *grads);
+int zxh_batch_info(zxh_batch *b, size_t *sets, size_t *total_params, size_t *launches);
+
 /* tuning knobs (process-wide), set by the autotuner (zxh_adae_amd.autotune):
  *   "diag.ept.c128", "diag.ept.c64": 0 = persistent diagonal sweep,
  *                                    1, 2, 4 = flat grid, 16-byte units per thread
@@ -261,7 +286,8 @@ int zxh_ansatz_reset_timing(zxh_ansatz *a);
 int zxh_tune_set(const char *name, int value);
 int zxh_tune_get(const char *name, int *value);
 /* HIP-event timing of the kernel-level calls (same families as zxh_get_timing,
- * plus 7 copy, 8 scale, 9 fused pass, 10 observable, 11 adjoint pass) */
+ * plus 7 copy, 8 scale, 9 fused pass, 10 observable, 11 adjoint pass,
+ * 12 dense block) */
 int zxh_k_set_timing(int on);
 int zxh_k_get_timing(int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
 int zxh_k_reset_timing(void);

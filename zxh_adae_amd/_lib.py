@@ -83,6 +83,8 @@ _SIGS = [
     ("zxh_k_copy", C.c_int, [_vp, _vp, _sz]),
     ("zxh_k_scale", C.c_int, [_vp, C.c_int, _sz, C.c_double, C.c_double]),
     ("zxh_k_expect_z", C.c_int, [_vp, C.c_int, _sz, _i32p, _i32p, _dp, _sz, _dp]),
+    ("zxh_k_block5", C.c_int, [_vp, C.c_int, _sz, C.POINTER(C.c_uint32), _dp]),
+    ("zxh_k_reset", C.c_int, [_vp, C.c_int, _sz]),
     ("zxh_ansatz_create", C.c_int, [_sz, C.c_int, _vp, _sz, _sz, C.POINTER(_vp)]),
     ("zxh_ansatz_destroy", C.c_int, [_vp]),
     ("zxh_ansatz_set_observable", C.c_int, [_vp, _i32p, _i32p, _dp, _sz]),
@@ -93,6 +95,12 @@ _SIGS = [
     ("zxh_ansatz_set_timing", C.c_int, [_vp, C.c_int]),
     ("zxh_ansatz_get_timing", C.c_int, [_vp, C.c_int, C.POINTER(_u64), _dp, C.POINTER(_u64)]),
     ("zxh_ansatz_reset_timing", C.c_int, [_vp]),
+    ("zxh_batch_create", C.c_int, [C.c_int, C.POINTER(_vp)]),
+    ("zxh_batch_destroy", C.c_int, [_vp]),
+    ("zxh_batch_add", C.c_int, [_vp, _sz, _vp, _sz, _sz, _i32p, _i32p, _dp, _sz]),
+    ("zxh_batch_finalize", C.c_int, [_vp]),
+    ("zxh_batch_run", C.c_int, [_vp, _dp, _dp, _dp]),
+    ("zxh_batch_info", C.c_int, [_vp, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)]),
     ("zxh_k_apply_z", C.c_int, [_vp, _vp, C.c_int, _sz, _i32p, _i32p, _dp, _sz, _dp]),
     ("zxh_tune_set", C.c_int, [C.c_char_p, C.c_int]),
     ("zxh_tune_get", C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),

@@ -130,3 +130,67 @@ def two_local(n: int, layers: int):
         for q in range(n - 1):
             ops.append((CZ, q, q + 1, -1, 0.0))
     return ops, k
+
+
+class AnsatzBatch:
+    """Many parameterized circuits (different widths and depths), evaluated
+    together: forward pass p of every set in one persistent launch, batched
+    observables, one launch per backward step, replayed from a hipGraph."""
+
+    def __init__(self, precision: str = "c64"):
+        self.lib = _lib.load()
+        self.precision = precision
+        prec = _lib.ZXH_C64 if precision in ("c64", "complex64") else _lib.ZXH_C128
+        h = C.c_void_p()
+        check(self.lib.zxh_batch_create(prec, C.byref(h)))
+        self._h = h
+        self.nparams = []
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.zxh_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def add(self, n: int, ops: Sequence[tuple], nparams: int, terms: Sequence[tuple]):
+        arr = (AnsatzOp * max(len(ops), 1))()
+        for i, o in enumerate(ops):
+            kind, q0, q1, param, value = o
+            arr[i].kind, arr[i].q0, arr[i].q1, arr[i].param, arr[i].value = kind, q0, q1, param, value
+        qi = np.array([int(t[1]) for t in terms], dtype=np.int32)
+        qj = np.array([int(t[2]) if len(t) > 2 else -1 for t in terms], dtype=np.int32)
+        cf = np.array([float(t[0]) for t in terms], dtype=np.float64)
+        check(self.lib.zxh_batch_add(self._h, int(n), C.cast(arr, C.c_void_p), len(ops), int(nparams),
+                                     qi.ctypes.data_as(_lib._i32p), qj.ctypes.data_as(_lib._i32p),
+                                     cf.ctypes.data_as(_lib._dp), len(cf)))
+        self.nparams.append(int(nparams))
+
+    def finalize(self):
+        check(self.lib.zxh_batch_finalize(self._h))
+
+    def run(self, params, gradients: bool = True):
+        """params: list of per-set arrays (or one concatenated array).
+        Returns (energies[sets], norms[sets], grads list or None)."""
+        if isinstance(params, (list, tuple)):
+            p = np.concatenate([np.asarray(x, dtype=np.float64).reshape(-1) for x in params])
+        else:
+            p = np.ascontiguousarray(params, dtype=np.float64).reshape(-1)
+        total = sum(self.nparams)
+        if p.size != total:
+            raise ValueError(f"expected {total} parameters, got {p.size}")
+        sets = len(self.nparams)
+        e = np.zeros(2 * sets)
+        g = np.zeros(max(total, 1)) if gradients else None
+        check(self.lib.zxh_batch_run(self._h, p.ctypes.data_as(_lib._dp), e.ctypes.data_as(_lib._dp),
+                                     g.ctypes.data_as(_lib._dp) if gradients else None))
+        grads = None
+        if gradients:
+            grads = np.split(g[:total], np.cumsum(self.nparams)[:-1])
+        return e[0::2], e[1::2], grads
+
+    def info(self):
+        s, t, l = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        check(self.lib.zxh_batch_info(self._h, C.byref(s), C.byref(t), C.byref(l)))
+        return {"sets": s.value, "params": t.value, "launches": l.value}

@@ -21,7 +21,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libzxh_amd.so")
 ARCH = os.environ.get("ZXH_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["ctx.cpp", "engine.cpp", "ansatz.cpp", "capi.cpp", "kernels_diag.hip", "kernels_u3.hip", "kernels_tile.hip", "kernels_obs.hip",
+SOURCES = ["ctx.cpp", "engine.cpp", "ansatz.cpp", "batch.cpp", "capi.cpp", "kernels_diag.hip", "kernels_u3.hip", "kernels_tile.hip", "kernels_obs.hip", "kernels_block.hip",
            "kernels_sample.hip",
            "kernels_util.hip"]
 

@@ -37,7 +37,7 @@ Ansatz::~Ansatz()
         (void)hipFree(dgrads_);
 }
 
-int Ansatz::init(const AnsatzOp *ops, size_t nops)
+int Ansatz::init(const AnsatzOp *ops, size_t nops, bool device)
 {
     if (n_ < tile_min_bits(prec_) || n_ > 36)
     {
@@ -66,9 +66,18 @@ int Ansatz::init(const AnsatzOp *ops, size_t nops)
         }
     }
     ops_.assign(ops, ops + nops);
-    ZXA_TRY(ctx_.init());
     ZXA_TRY(build_schedule());
+    if (!device)
+        return OK;
+    ZXA_TRY(ctx_.init());
     return alloc();
+}
+
+int Ansatz::refresh_schedule()
+{
+    if (tile_seg(prec_) != seg_)
+        return build_schedule();
+    return OK;
 }
 
 int Ansatz::alloc()
@@ -227,35 +236,82 @@ M2 generator(int kind)
 }
 } // namespace
 
+void Ansatz::pass_gates(size_t pi, const double *params, std::vector<U3Gate> &gs) const
+{
+    gs.clear();
+    for (const Target &t : passes_[pi].targets)
+    {
+        M2 u{1.0, 0.0, 0.0, 1.0};
+        for (size_t i : t.ops)
+        {
+            const Mat m = op_matrix(i, params);
+            u = mul({m.a, m.b, m.c, m.d}, u);
+        }
+        U3Gate g;
+        g.packed_sel = uint64_t(1) << t.q;
+        g.delta = uint64_t(1) << t.q;
+        put(u, g.u);
+        gs.push_back(g);
+    }
+}
+
+void Ansatz::pass_window(size_t pi, const double *params, bool negate, std::vector<DiagGate> &w) const
+{
+    w.clear();
+    for (size_t i : passes_[pi].window)
+    {
+        const AnsatzOp &o = ops_[i];
+        const double th = o.kind == A_CZ ? M_PI : (o.param >= 0 ? params[o.param] : o.value);
+        w.push_back({uint64_t(1) << o.q0, uint64_t(1) << o.q1, negate ? -th : th});
+    }
+}
+
+void Ansatz::pass_adjoint(size_t pi, const double *params, std::vector<AdjTarget> &ts) const
+{
+    ts.clear();
+    for (const Target &t : passes_[pi].targets)
+    {
+        std::vector<M2> ms;
+        for (size_t i : t.ops)
+        {
+            const Mat m = op_matrix(i, params);
+            ms.push_back({m.a, m.b, m.c, m.d});
+        }
+        M2 u{1.0, 0.0, 0.0, 1.0};
+        for (const M2 &m : ms)
+            u = mul(m, u);
+        AdjTarget a;
+        std::memset(&a, 0, sizeof(a));
+        a.target = t.q;
+        put(dag(u), a.m);
+        a.param[0] = a.param[1] = -1;
+        int slot = 0;
+        for (size_t k = 0; k < t.ops.size(); k++)
+        {
+            const AnsatzOp &o = ops_[t.ops[k]];
+            if (o.param < 0)
+                continue;
+            M2 v{1.0, 0.0, 0.0, 1.0}; // gates composed after op k
+            for (size_t l = k + 1; l < ms.size(); l++)
+                v = mul(ms[l], v);
+            put(mul(mul(v, generator(o.kind)), dag(v)), a.a[slot]);
+            a.param[slot] = o.param;
+            slot++;
+        }
+        ts.push_back(a);
+    }
+}
+
 int Ansatz::run_forward(const double *params)
 {
-    if (tile_seg(prec_) != seg_)
-        ZXA_TRY(build_schedule());
+    ZXA_TRY(refresh_schedule());
     ZXA_TRY(state_reset(ctx_, psi_, prec_, size_t(1) << n_));
-    for (const Pass &p : passes_)
+    std::vector<U3Gate> gs;
+    std::vector<DiagGate> w;
+    for (size_t p = 0; p < passes_.size(); p++)
     {
-        std::vector<U3Gate> gs;
-        for (const Target &t : p.targets)
-        {
-            M2 u{1.0, 0.0, 0.0, 1.0};
-            for (size_t i : t.ops)
-            {
-                const Mat m = op_matrix(i, params);
-                u = mul({m.a, m.b, m.c, m.d}, u);
-            }
-            U3Gate g;
-            g.packed_sel = uint64_t(1) << t.q;
-            g.delta = uint64_t(1) << t.q;
-            put(u, g.u);
-            gs.push_back(g);
-        }
-        std::vector<DiagGate> w;
-        for (size_t i : p.window)
-        {
-            const AnsatzOp &o = ops_[i];
-            const double th = o.kind == A_CZ ? M_PI : (o.param >= 0 ? params[o.param] : o.value);
-            w.push_back({uint64_t(1) << o.q0, uint64_t(1) << o.q1, th});
-        }
+        pass_gates(p, params, gs);
+        pass_window(p, params, false, w);
         ZXA_TRY(fused_pass_apply(ctx_, psi_, prec_, n_, gs.data(), gs.size(), w.data(), w.size()));
     }
     return OK;
@@ -292,61 +348,20 @@ int Ansatz::backward(const double *params, double *grads, double *energy)
         energy[1] = e2[1];
     }
     ZXA_HIP(hipMemsetAsync(dgrads_, 0, std::max<size_t>(1, nparams_) * sizeof(double), ctx_.stream));
-
-    auto negated_window = [&](const Pass &p) {
-        std::vector<DiagGate> w;
-        for (size_t i : p.window)
-        {
-            const AnsatzOp &o = ops_[i];
-            const double th = o.kind == A_CZ ? M_PI : (o.param >= 0 ? params[o.param] : o.value);
-            w.push_back({uint64_t(1) << o.q0, uint64_t(1) << o.q1, -th});
-        }
-        return w;
-    };
+    std::vector<DiagGate> w;
     if (!passes_.empty() && !passes_.back().window.empty())
     {
-        const std::vector<DiagGate> w = negated_window(passes_.back());
+        pass_window(passes_.size() - 1, params, true, w);
         ZXA_TRY(diag_apply(ctx_, psi_, prec_, n_, w.data(), w.size()));
         ZXA_TRY(diag_apply(ctx_, lam_, prec_, n_, w.data(), w.size()));
     }
+    std::vector<AdjTarget> ts;
     for (size_t jj = passes_.size(); jj-- > 0;)
     {
-        const Pass &p = passes_[jj];
-        std::vector<AdjTarget> ts;
-        for (const Target &t : p.targets)
-        {
-            std::vector<M2> ms;
-            for (size_t i : t.ops)
-            {
-                const Mat m = op_matrix(i, params);
-                ms.push_back({m.a, m.b, m.c, m.d});
-            }
-            M2 u{1.0, 0.0, 0.0, 1.0};
-            for (const M2 &m : ms)
-                u = mul(m, u);
-            AdjTarget a;
-            std::memset(&a, 0, sizeof(a));
-            a.target = t.q;
-            put(dag(u), a.m);
-            a.param[0] = a.param[1] = -1;
-            int slot = 0;
-            for (size_t k = 0; k < t.ops.size(); k++)
-            {
-                const AnsatzOp &o = ops_[t.ops[k]];
-                if (o.param < 0)
-                    continue;
-                M2 v{1.0, 0.0, 0.0, 1.0}; // gates composed after op k
-                for (size_t l = k + 1; l < ms.size(); l++)
-                    v = mul(ms[l], v);
-                put(mul(mul(v, generator(o.kind)), dag(v)), a.a[slot]);
-                a.param[slot] = o.param;
-                slot++;
-            }
-            ts.push_back(a);
-        }
-        std::vector<DiagGate> w;
+        pass_adjoint(jj, params, ts);
+        w.clear();
         if (jj > 0)
-            w = negated_window(passes_[jj - 1]);
+            pass_window(jj - 1, params, true, w);
         ZXA_TRY(adjoint_pass_apply(ctx_, psi_, lam_, prec_, n_, ts.data(), ts.size(), w.data(), w.size(), dgrads_));
     }
     if (nparams_ > 0)

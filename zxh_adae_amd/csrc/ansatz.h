@@ -54,7 +54,9 @@ class Ansatz
   public:
     Ansatz(unsigned n, int prec, size_t nparams);
     ~Ansatz();
-    int init(const AnsatzOp *ops, size_t nops);
+    // device = false: schedule only (no stream, no device states), for the
+    // batched engine, which keeps every set's states in one arena.
+    int init(const AnsatzOp *ops, size_t nops, bool device = true);
     int set_observable(const int32_t *qi, const int32_t *qj, const double *coef, size_t nterms);
     // energy[0] = <psi|H|psi>, energy[1] = <psi|psi>
     int forward(const double *params, double *energy);
@@ -74,6 +76,34 @@ class Ansatz
     Ctx &ctx()
     {
         return ctx_;
+    }
+    int precision() const
+    {
+        return prec_;
+    }
+    size_t nparams() const
+    {
+        return nparams_;
+    }
+    // Rebuilds the schedule if the tile geometry knob changed since.
+    int refresh_schedule();
+    // Pass p of the forward schedule: its U3 gates (fused matrices at params)
+    // and its diagonal window (angles negated when negate).
+    void pass_gates(size_t p, const double *params, std::vector<U3Gate> &gates) const;
+    void pass_window(size_t p, const double *params, bool negate, std::vector<DiagGate> &window) const;
+    // The adjoint targets of pass p (uncompute matrices, transition operators).
+    void pass_adjoint(size_t p, const double *params, std::vector<AdjTarget> &targets) const;
+    const std::vector<int32_t> &obs_qi() const
+    {
+        return qi_;
+    }
+    const std::vector<int32_t> &obs_qj() const
+    {
+        return qj_;
+    }
+    const std::vector<double> &obs_coef() const
+    {
+        return coef_;
     }
 
   private:

@@ -3,6 +3,7 @@
 
 #include "engine.h"
 #include "ansatz.h"
+#include "batch.h"
 
 #include <algorithm>
 #include <cstring>
@@ -18,6 +19,11 @@ struct zxh_sim
 struct zxh_ansatz
 {
     zxa::Ansatz *a;
+};
+
+struct zxh_batch
+{
+    zxa::AnsatzBatch *b;
 };
 
 namespace
@@ -311,6 +317,14 @@ int zxh_dev_alloc(size_t bytes, void **dptr)
         *dptr = nullptr;
         return fail(ZXH_ENOMEM, hipGetErrorString(e));
     }
+    // zeroed, like worker_alloc (mem_cuda.cu:28-40)
+    e = hipMemset(*dptr, 0, bytes);
+    if (e != hipSuccess)
+    {
+        (void)hipFree(*dptr);
+        *dptr = nullptr;
+        return fail(ZXH_EHIP, hipGetErrorString(e));
+    }
     return ZXH_OK;
 }
 
@@ -527,6 +541,29 @@ int zxh_k_apply_z(const void *src, void *dst, int precision, size_t local_bits, 
     return observable_call(src, dst, precision, local_bits, qi, qj, coef, nterms, out);
 }
 
+int zxh_k_reset(void *state, int precision, size_t local_bits)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    WITH_CTX(c);
+    return finish(c, zxa::state_reset(*c, state, precision, size_t(1) << local_bits));
+}
+
+int zxh_k_block5(void *state, int precision, size_t local_bits, const uint32_t *targets, const double *u)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    if (!targets || !u)
+        return fail(ZXH_EINVAL, "null targets or matrix");
+    unsigned t[5];
+    for (int b = 0; b < 5; b++)
+        t[b] = targets[b];
+    WITH_CTX(c);
+    return finish(c, zxa::block5_apply(*c, state, precision, unsigned(local_bits), t, u));
+}
+
 // ---- parameterized circuits ----
 
 #define CHECK_ANS(x)                                                                                                   \
@@ -641,6 +678,68 @@ int zxh_ansatz_reset_timing(zxh_ansatz *a)
     for (auto &f : a->a->ctx().fam)
         f = zxa::FamilyTiming();
     return rc;
+}
+
+#define CHECK_BATCH(x)                                                                                                 \
+    if (!(x) || !(x)->b)                                                                                               \
+        return fail(ZXH_EINVAL, "null batch handle");
+
+int zxh_batch_create(int precision, zxh_batch **out)
+{
+    if (!out)
+        return fail(ZXH_EINVAL, "null output handle");
+    *out = nullptr;
+    if (!zxa::runtime_active())
+        return fail(ZXH_ESTATE, "init must be called before constructing or using batch objects");
+    if (precision != ZXH_C128 && precision != ZXH_C64)
+        return fail(ZXH_EINVAL, "precision must be ZXH_C128 or ZXH_C64");
+    *out = new zxh_batch{new zxa::AnsatzBatch(precision)};
+    return ZXH_OK;
+}
+
+int zxh_batch_destroy(zxh_batch *b)
+{
+    if (!b)
+        return ZXH_OK;
+    delete b->b;
+    delete b;
+    return ZXH_OK;
+}
+
+int zxh_batch_add(zxh_batch *b, size_t n, const zxh_ansatz_op *ops, size_t nops, size_t nparams,
+                  const int32_t *qi, const int32_t *qj, const double *coef, size_t nterms)
+{
+    CHECK_BATCH(b);
+    if ((nops > 0 && !ops) || (nterms > 0 && (!qi || !qj || !coef)))
+        return fail(ZXH_EINVAL, "null arrays");
+    return b->b->add(unsigned(std::min<size_t>(n, 64)), reinterpret_cast<const zxa::AnsatzOp *>(ops), nops,
+                     nparams, qi, qj, coef, nterms);
+}
+
+int zxh_batch_finalize(zxh_batch *b)
+{
+    CHECK_BATCH(b);
+    return b->b->finalize();
+}
+
+int zxh_batch_run(zxh_batch *b, const double *params, double *energies, double *grads)
+{
+    CHECK_BATCH(b);
+    if (!params && b->b->total_params())
+        return fail(ZXH_EINVAL, "null params");
+    return b->b->run(params, energies, grads);
+}
+
+int zxh_batch_info(zxh_batch *b, size_t *sets, size_t *total_params, size_t *launches)
+{
+    CHECK_BATCH(b);
+    if (sets)
+        *sets = b->b->sets();
+    if (total_params)
+        *total_params = b->b->total_params();
+    if (launches)
+        *launches = b->b->launches();
+    return ZXH_OK;
 }
 
 int zxh_tune_set(const char *name, int value)

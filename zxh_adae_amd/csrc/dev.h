@@ -88,7 +88,8 @@ enum Family
     F_FUSED = 9,
     F_OBS = 10,
     F_ADJ = 11,
-    F_COUNT = 12
+    F_BLOCK = 12,
+    F_COUNT = 13
 };
 
 struct FamilyTiming
@@ -235,6 +236,73 @@ struct AdjTarget
 // dgrads: device array of parameter gradients, accumulated in stream order.
 int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned nbits, const AdjTarget *targets,
                        size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads);
+// Dense 5-qubit block: y[t] = sum_t' u[t][t'] x[t'] over every group of 32
+// amplitudes differing in the target bits (bit b of t = targets[b]); u is
+// 32x32 complex, row-major (re, im) pairs.  One sweep, matrix cores.
+int block5_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const unsigned *targets, const double *u);
+// ---- batched tile passes (many independent states per launch) ----
+// One state's pass in a batched launch.  Forward launches use gates (the
+// U3 gates of the pass); adjoint launches use targets (AdjTarget); window
+// is the epilogue with the angles to apply.  The tables for the matrices are
+// written per run (batch_set_*), the rest is fixed at prepare time.
+struct BatchPassSpec
+{
+    void *psi = nullptr;
+    void *lam = nullptr; // adjoint launches
+    unsigned n = 0;
+    std::vector<unsigned> targets;
+    std::vector<DiagGate> window;
+    double *grads = nullptr; // adjoint: the state's gradient array (device)
+    size_t nparams = 0;      // adjoint: its length
+};
+struct BatchLaunch
+{
+    int prec = 0;
+    bool adj = false;
+    size_t nitems = 0;
+    uint64_t total_tiles = 0;
+    int grid = 0;
+    void *d_items = nullptr, *d_tabs = nullptr, *d_adjs = nullptr, *d_params = nullptr, *d_partial = nullptr,
+         *d_collect = nullptr, *d_epi = nullptr;
+    void *h_tables = nullptr; // pinned mirror of d_tabs | d_adjs | d_params
+    size_t tab_bytes = 0, adj_bytes = 0, par_bytes = 0;
+    std::vector<std::vector<int>> slots; // per item: slot of each target
+    ~BatchLaunch();
+};
+int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec> &specs, BatchLaunch &out);
+// Matrices of item i for this run (into the pinned mirror).
+void batch_set_gates(BatchLaunch &b, size_t item, const U3Gate *gates, size_t ngates);
+void batch_set_adjoint(BatchLaunch &b, size_t item, const AdjTarget *targets, size_t ntargets);
+// Copies the mirror to the device tables (async on ctx.stream).
+int batch_upload(Ctx &ctx, BatchLaunch &b);
+// The batched kernel (and, for adjoint launches, the gradient collection).
+int batch_launch(Ctx &ctx, BatchLaunch &b);
+// Batched observables: per state E, <psi|psi> (into energies[2 i], device)
+// and, when dst is set, dst = H psi.  Block coefficients fixed at prepare.
+struct ObsSpec
+{
+    const void *src = nullptr;
+    void *dst = nullptr;
+    unsigned n = 0;
+    std::vector<int32_t> qi, qj;
+    std::vector<double> coef;
+};
+struct ObsBatch
+{
+    int prec = 0;
+    size_t nitems = 0;
+    uint64_t total_blocks = 0;
+    int grid = 0;
+    void *d_items = nullptr, *d_tables = nullptr, *d_partial = nullptr;
+    std::vector<void *> bufs;
+    ~ObsBatch();
+};
+int obs_batch_prepare(Ctx &ctx, int prec, const std::vector<ObsSpec> &specs, double *d_energies, ObsBatch &out);
+int obs_batch_launch(Ctx &ctx, ObsBatch &b);
+// Graph-capturable fills: zero bytes (multiple of 16) and amplitude 0 = 1 of
+// each state in the device array dstates.
+int zero_fill(Ctx &ctx, void *p, size_t bytes);
+int set_ones(Ctx &ctx, const uint64_t *dstates, int count, int prec);
 // One general pair sweep (any delta / selector).
 int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
 

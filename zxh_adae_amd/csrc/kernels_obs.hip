@@ -175,6 +175,140 @@ __global__ __launch_bounds__(kThreads) void obs_kernel(const void *__restrict__ 
     wg_reduce2(eh.s, en.s, sh, partial + 2 * blockIdx.x);
 }
 
+// Batched: one block of one state per work item (states of different
+// widths and observables in one launch); per-block partial sums.
+struct OItem
+{
+    uint64_t first;   // first global block index of this state
+    uint64_t src, dst; // dst = 0: expectation only
+    uint64_t ltab;    // lane / row tables (as obs_kernel)
+    uint64_t coef;    // block coefficients
+};
+
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void obs_batch_kernel(const OItem *__restrict__ items, int nitems,
+                                                             uint64_t total, double *__restrict__ partial)
+{
+    using G = OGeo<kC64>;
+    constexpr int E = G::kEPU;
+    __shared__ double sh[2 * kThreads];
+    const int t = threadIdx.x;
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x)
+    {
+        ZXA_CONST OItem *IT = (ZXA_CONST OItem *)items;
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (IT[mid].first <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        ZXA_CONST OItem &I = IT[lo];
+        const uint64_t blk = g - I.first;
+        const double *ltab = reinterpret_cast<const double *>(I.ltab);
+        double A[E], d[E][kRowBits];
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+            const double *a = ltab + (size_t(t) * E + e) * (1 + kRowBits);
+            A[e] = a[0];
+#pragma unroll
+            for (int j = 0; j < kRowBits; j++)
+                d[e][j] = a[1 + j];
+        }
+        ZXA_CONST double *R = (ZXA_CONST double *)(ltab + size_t(kThreads) * E * (1 + kRowBits));
+        ZXA_CONST double *c = (ZXA_CONST double *)(reinterpret_cast<const double *>(I.coef) + blk * kCoefStride);
+        double T[E], D[E][kRowBits];
+#pragma unroll
+        for (int e = 0; e < E; e++)
+        {
+            const int x = kC64 ? (t << 1) | e : t;
+            double s = c[G::kBlock] + A[e];
+#pragma unroll
+            for (int b = 0; b < G::kLane; b++)
+                if ((x >> b) & 1)
+                    s += c[b];
+            T[e] = s;
+#pragma unroll
+            for (int j = 0; j < kRowBits; j++)
+                D[e][j] = d[e][j] + c[G::kLane + j];
+        }
+        const uint64_t base = blk << (G::kBlock - (kC64 ? 1 : 0));
+        const bool apply = I.dst != 0;
+        Kahan eh, en;
+        double gg[E];
+#pragma unroll
+        for (int e = 0; e < E; e++)
+            gg[e] = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < kRows; k++)
+        {
+            const int r = k ^ (k >> 1);
+            if (k > 0)
+            {
+                const int j = __builtin_ctz(k);
+                const bool up = (r >> j) & 1;
+#pragma unroll
+                for (int e = 0; e < E; e++)
+                    gg[e] += up ? D[e][j] : -D[e][j];
+            }
+            const uint64_t u = base + (uint64_t(r) << 8) + t;
+            if constexpr (kC64)
+            {
+                const fvec4 v = __builtin_nontemporal_load(reinterpret_cast<const fvec4 *>(I.src) + u);
+                const double h0 = T[0] + R[r] + gg[0];
+                const double h1 = T[1] + R[r] + gg[1];
+                const double p0 = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+                const double p1 = double(v.z) * double(v.z) + double(v.w) * double(v.w);
+                eh.add(p0 * h0);
+                eh.add(p1 * h1);
+                en.add(p0);
+                en.add(p1);
+                if (apply)
+                {
+                    const float f0 = float(h0), f1 = float(h1);
+                    __builtin_nontemporal_store(fvec4{v.x * f0, v.y * f0, v.z * f1, v.w * f1},
+                                                reinterpret_cast<fvec4 *>(I.dst) + u);
+                }
+            }
+            else
+            {
+                const dvec2 v = __builtin_nontemporal_load(reinterpret_cast<const dvec2 *>(I.src) + u);
+                const double h = T[0] + R[r] + gg[0];
+                const double p = v.x * v.x + v.y * v.y;
+                eh.add(p * h);
+                en.add(p);
+                if (apply)
+                    __builtin_nontemporal_store(dvec2{v.x * h, v.y * h}, reinterpret_cast<dvec2 *>(I.dst) + u);
+            }
+        }
+        wg_reduce2(eh.s, en.s, sh, partial + 2 * g);
+        __syncthreads();
+    }
+}
+
+// Per state: its blocks' partials in order -> energies[2 i], energies[2 i + 1].
+__global__ __launch_bounds__(64) void obs_batch_collect_kernel(const OItem *__restrict__ items, int nitems,
+                                                               uint64_t total, const double *__restrict__ partial,
+                                                               double *__restrict__ energies)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nitems)
+        return;
+    const uint64_t b0 = items[i].first;
+    const uint64_t b1 = i + 1 < nitems ? items[i + 1].first : total;
+    Kahan a, n;
+    for (uint64_t b = b0; b < b1; b++)
+    {
+        a.add(partial[2 * b]);
+        n.add(partial[2 * b + 1]);
+    }
+    energies[2 * i] = a.s;
+    energies[2 * i + 1] = n.s;
+}
+
 // Small states (fewer address bits than one block): h evaluated per
 // amplitude from the polynomial.  poly: K, L[n], Q[n*n].
 template <bool kC64, bool kApply>
@@ -401,6 +535,133 @@ int observable_apply(Ctx &ctx, const void *src, void *dst, int prec, unsigned n,
         ZXA_HIP(hipStreamSynchronize(ctx.stream));
     }
     return ctx.ring.done(ctx.stream);
+}
+
+ObsBatch::~ObsBatch()
+{
+    for (void *p : bufs)
+        if (p)
+            (void)hipFree(p);
+}
+
+namespace
+{
+// Lane and row tables of obs_kernel for one observable polynomial.
+std::vector<double> lane_row_tables(bool c64, const Poly &P)
+{
+    const int E = c64 ? 2 : 1;
+    const int LB = c64 ? OGeo<true>::kLane : OGeo<false>::kLane;
+    std::vector<double> lt(size_t(kThreads) * E * (1 + kRowBits) + kRows, 0.0);
+    for (int t = 0; t < kThreads; t++)
+        for (int e = 0; e < E; e++)
+        {
+            const int x = c64 ? (t << 1) | e : t;
+            double *a = lt.data() + (size_t(t) * E + e) * (1 + kRowBits);
+            double s = 0.0;
+            for (int b = 0; b < LB; b++)
+            {
+                if (!((x >> b) & 1))
+                    continue;
+                s += P.L[b];
+                for (int c = b + 1; c < LB; c++)
+                    if ((x >> c) & 1)
+                        s += P.q(b, c);
+            }
+            a[0] = s;
+            for (int j = 0; j < kRowBits; j++)
+            {
+                double dj = 0.0;
+                for (int b = 0; b < LB; b++)
+                    if ((x >> b) & 1)
+                        dj += P.q(b, LB + j);
+                a[1 + j] = dj;
+            }
+        }
+    double *R = lt.data() + size_t(kThreads) * E * (1 + kRowBits);
+    for (int r = 0; r < kRows; r++)
+    {
+        double s = 0.0;
+        for (int j = 0; j < kRowBits; j++)
+        {
+            if (!((r >> j) & 1))
+                continue;
+            s += P.L[LB + j];
+            for (int k = j + 1; k < kRowBits; k++)
+                if ((r >> k) & 1)
+                    s += P.q(LB + j, LB + k);
+        }
+        R[r] = s;
+    }
+    return lt;
+}
+} // namespace
+
+int obs_batch_prepare(Ctx &ctx, int prec, const std::vector<ObsSpec> &specs, double *d_energies, ObsBatch &b)
+{
+    const bool c64 = prec == C64;
+    b.prec = prec;
+    b.nitems = specs.size();
+    const unsigned bb = c64 ? OGeo<true>::kBlock : OGeo<false>::kBlock;
+    std::vector<OItem> items(specs.size());
+    uint64_t first = 0;
+    for (size_t i = 0; i < specs.size(); i++)
+    {
+        const ObsSpec &sp = specs[i];
+        if (sp.n < bb || sp.n > 40)
+        {
+            set_error("observable batch: state narrower than one block (14 bits complex128, 15 complex64)");
+            return EINVAL_;
+        }
+        Poly P;
+        observable_poly(sp.n, sp.qi.data(), sp.qj.data(), sp.coef.data(), sp.coef.size(), P);
+        const std::vector<double> lt = lane_row_tables(c64, P);
+        void *dlt = nullptr;
+        ZXA_HIP(hipMalloc(&dlt, lt.size() * sizeof(double)));
+        b.bufs.push_back(dlt);
+        ZXA_HIP(hipMemcpy(dlt, lt.data(), lt.size() * sizeof(double), hipMemcpyHostToDevice));
+        const int nco = int(bb), nh = int(sp.n) - nco;
+        std::vector<int> inner(nco), outer(nh);
+        for (int k = 0; k < nco; k++)
+            inner[k] = k;
+        for (int k = 0; k < nh; k++)
+            outer[k] = nco + k;
+        const uint64_t nblocks = uint64_t(1) << nh;
+        void *dco = nullptr;
+        ZXA_HIP(hipMalloc(&dco, nblocks * kCoefStride * sizeof(double)));
+        b.bufs.push_back(dco);
+        ZXA_TRY(launch_block_coef(ctx, P, inner.data(), nco, outer.data(), nh, static_cast<double *>(dco)));
+        ZXA_TRY(ctx.ring.done(ctx.stream));
+        items[i].first = first;
+        items[i].src = reinterpret_cast<uint64_t>(sp.src);
+        items[i].dst = reinterpret_cast<uint64_t>(sp.dst);
+        items[i].ltab = reinterpret_cast<uint64_t>(dlt);
+        items[i].coef = reinterpret_cast<uint64_t>(dco);
+        first += nblocks;
+    }
+    b.total_blocks = first;
+    ZXA_HIP(hipMalloc(&b.d_items, std::max<size_t>(1, items.size()) * sizeof(OItem)));
+    b.bufs.push_back(b.d_items);
+    ZXA_HIP(hipMemcpy(b.d_items, items.data(), items.size() * sizeof(OItem), hipMemcpyHostToDevice));
+    ZXA_HIP(hipMalloc(&b.d_partial, std::max<uint64_t>(1, first) * 2 * sizeof(double)));
+    b.bufs.push_back(b.d_partial);
+    b.d_tables = d_energies;
+    b.grid = grid_of(ctx, c64 ? obs_batch_kernel<true> : obs_batch_kernel<false>, first);
+    return hip_check(hipStreamSynchronize(ctx.stream), "observable batch prepare");
+}
+
+int obs_batch_launch(Ctx &ctx, ObsBatch &b)
+{
+    if (!b.nitems)
+        return OK;
+    auto k = b.prec == C64 ? obs_batch_kernel<true> : obs_batch_kernel<false>;
+    hipLaunchKernelGGL(k, dim3(b.grid), dim3(kThreads), 0, ctx.stream, static_cast<const OItem *>(b.d_items),
+                       int(b.nitems), b.total_blocks, static_cast<double *>(b.d_partial));
+    ZXA_HIP(hipGetLastError());
+    const unsigned g2 = unsigned((b.nitems + 63) / 64);
+    hipLaunchKernelGGL(obs_batch_collect_kernel, dim3(g2), dim3(64), 0, ctx.stream,
+                       static_cast<const OItem *>(b.d_items), int(b.nitems), b.total_blocks,
+                       static_cast<const double *>(b.d_partial), static_cast<double *>(b.d_tables));
+    return hip_check(hipGetLastError(), "observable batch launch");
 }
 
 } // namespace zxa

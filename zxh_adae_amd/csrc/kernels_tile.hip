@@ -534,6 +534,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 // ---------------------------------------------------------------------------
 
 constexpr int kObs = 2;                     // transition operators per slot
+constexpr int kWavesPerTile = kThreads / 64;
 constexpr int kAccs = (kUnitBits + 1) * kObs; // accumulator lanes (< 64)
 
 struct TObs
@@ -580,9 +581,11 @@ __device__ __forceinline__ void load_obs(Real<kC64> (&a)[8], ZXA_CONST TObs &o, 
 
 // Slot s of layout L (register bit i; s = kIntra for the complex64 in-unit
 // bit): transition values into the lane-indexed accumulator, then M on both.
-template <bool kC64, int L>
+// emit(a, tot): the wave-wide transition value tot of accumulator a (slot
+// * kObs + operator), identical in every lane.
+template <bool kC64, int L, typename Emit>
 __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
-                                           ZXA_CONST TTab *G, ZXA_CONST TAdj *A, double &acc, int lane)
+                                           ZXA_CONST TTab *G, ZXA_CONST TAdj *A, Emit &&emit)
 {
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -612,8 +615,7 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
                     part += pair_im<double>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x, w[r1].y, a);
                 }
             }
-            const double tot = wave_sum(part);
-            acc += lane == k * kObs + o ? tot : 0.0;
+            emit(k * kObs + o, wave_sum(part));
         }
         Real<kC64> m[8];
         load_matrix<kC64>(m, G->g[k]);
@@ -637,8 +639,7 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
 #pragma unroll
             for (int r = 0; r < kRegs; r++)
                 part += double(pair_im<float>(v[r].x, v[r].y, v[r].z, v[r].w, w[r].x, w[r].y, w[r].z, w[r].w, a));
-            const double tot = wave_sum(part);
-            acc += lane == kIntra * kObs + o ? tot : 0.0;
+            emit(kIntra * kObs + o, wave_sum(part));
         }
         float m[8];
         load_matrix<true>(m, G->g[kIntra]);
@@ -671,6 +672,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     if constexpr (kDiag)
         epi.init(ttab, t);
     double acc = 0.0;
+    auto emit = [&](int a, double tot) { acc += lane == a ? tot : 0.0; };
     uint64_t base = deposit(uint64_t(blockIdx.x), P.sorted);
     for (uint64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x)
     {
@@ -681,17 +683,17 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         T v[kRegs], w[kRegs];
         load_tile<kC64>(v, pp, base | toff, G);
         load_tile<kC64>(w, pl, base | toff, G);
-        adj_layout<kC64, 0>(v, w, G, A, acc, lane);
+        adj_layout<kC64, 0>(v, w, G, A, emit);
         if constexpr (kL1)
         {
             transpose<kC64, 0, 1>(v, lds, t);
             transpose<kC64, 0, 1>(w, lds, t);
-            adj_layout<kC64, 1>(v, w, G, A, acc, lane);
+            adj_layout<kC64, 1>(v, w, G, A, emit);
             if constexpr (kL2)
             {
                 transpose<kC64, 1, 2>(v, lds, t);
                 transpose<kC64, 1, 2>(w, lds, t);
-                adj_layout<kC64, 2>(v, w, G, A, acc, lane);
+                adj_layout<kC64, 2>(v, w, G, A, emit);
                 transpose<kC64, 2, 0>(v, lds, t);
                 transpose<kC64, 2, 0>(w, lds, t);
             }
@@ -705,7 +707,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         {
             transpose<kC64, 0, 2>(v, lds, t);
             transpose<kC64, 0, 2>(w, lds, t);
-            adj_layout<kC64, 2>(v, w, G, A, acc, lane);
+            adj_layout<kC64, 2>(v, w, G, A, emit);
             transpose<kC64, 2, 0>(v, lds, t);
             transpose<kC64, 2, 0>(w, lds, t);
         }
@@ -1193,6 +1195,153 @@ int launch_adj_l(Ctx &ctx, void *psi, void *lam, const TPass &P, bool l1, bool l
     return launch_adj_t<kC64, false, false, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
 }
 
+// ---------------------------------------------------------------------------
+// Batched tile passes: one pass of many independent states (different widths
+// and tables) in one persistent launch.  Work item = one tile of one state;
+// the launch's items are the states' tile ranges, concatenated.  Every tile
+// goes through all three layouts (identity matrices where a state's pass has
+// no target there) and the epilogue when the state's pass has a window.
+// ---------------------------------------------------------------------------
+struct BItem
+{
+    uint64_t first;   // first global tile index of this state's pass
+    uint64_t ntiles;
+    uint64_t psi;     // state (16-byte units)
+    uint64_t lam;     // adjoint: second state
+    uint64_t tab;     // TTab
+    uint64_t adj;     // adjoint: TAdj
+    uint64_t ttab;    // epilogue per-thread table, 0: no window
+    uint64_t tphase;  // epilogue per-tile factors
+    uint64_t partial; // adjoint: per tile and wave, kAccs accumulators
+    uint32_t ub[kUnitBits];
+    uint32_t sorted[kUnitBits];
+};
+
+template <bool kC64, bool kAdj>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_tile_kernel(
+    const BItem *__restrict__ items, int nitems, uint64_t total)
+{
+    using T = typename UnitT<kC64>::T;
+    __shared__ T lds[kUnits];
+    const int t = threadIdx.x;
+    const int lane = t & 63;
+    for (uint64_t g = blockIdx.x; g < total; g += gridDim.x)
+    {
+        const BItem *I0 = items;
+        asm volatile("" : "+s"(I0));
+        ZXA_CONST BItem *IT = (ZXA_CONST BItem *)I0;
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi)
+        {
+            const int mid = (lo + hi + 1) >> 1;
+            if (IT[mid].first <= g)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        ZXA_CONST BItem &I = IT[lo];
+        TPass P;
+#pragma unroll
+        for (int k = 0; k < kUnitBits; k++)
+        {
+            P.ub[k] = I.ub[k];
+            P.sorted[k] = I.sorted[k];
+        }
+        const uint64_t local = g - I.first;
+        const uint64_t base = deposit(local, P.sorted);
+        const uint64_t toff = thr_off<0>(t, P);
+        ZXA_CONST TTab *G = (ZXA_CONST TTab *)I.tab;
+        T *pp = reinterpret_cast<T *>(I.psi);
+        T v[kRegs];
+        load_tile<kC64>(v, pp, base | toff, G);
+        if constexpr (kAdj)
+        {
+            T *pl = reinterpret_cast<T *>(I.lam);
+            ZXA_CONST TAdj *A = (ZXA_CONST TAdj *)I.adj;
+            T w[kRegs];
+            load_tile<kC64>(w, pl, base | toff, G);
+            // one wave-wide value per accumulator, stored by lane 0 at once
+            double *pw = reinterpret_cast<double *>(I.partial) + (local * kWavesPerTile + (t >> 6)) * kAccs;
+            auto emit = [&](int a, double tot) {
+                if (lane == 0)
+                    pw[a] = tot;
+            };
+            adj_layout<kC64, 0>(v, w, G, A, emit);
+            transpose<kC64, 0, 1>(v, lds, t);
+            transpose<kC64, 0, 1>(w, lds, t);
+            adj_layout<kC64, 1>(v, w, G, A, emit);
+            transpose<kC64, 1, 2>(v, lds, t);
+            transpose<kC64, 1, 2>(w, lds, t);
+            adj_layout<kC64, 2>(v, w, G, A, emit);
+            transpose<kC64, 2, 0>(v, lds, t);
+            transpose<kC64, 2, 0>(w, lds, t);
+            if (I.ttab != 0)
+            {
+                Epi<kC64> epi;
+                epi.init(reinterpret_cast<const double *>(I.ttab), t);
+                epi.template apply_n<true>(
+                    v, w, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * Epi<kC64>::kStride),
+                    t);
+            }
+            store_tile<kC64>(v, pp, base | toff, G);
+            store_tile<kC64>(w, pl, base | toff, G);
+        }
+        else
+        {
+            apply_layout<kC64, false, 0>(v, P, G, base, toff);
+            transpose<kC64, 0, 1>(v, lds, t);
+            apply_layout<kC64, false, 1>(v, P, G, base, toff);
+            transpose<kC64, 1, 2>(v, lds, t);
+            apply_layout<kC64, false, 2>(v, P, G, base, toff);
+            transpose<kC64, 2, 0>(v, lds, t);
+            if (I.ttab != 0)
+            {
+                Epi<kC64> epi;
+                epi.init(reinterpret_cast<const double *>(I.ttab), t);
+                epi.apply(v, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * Epi<kC64>::kStride),
+                          t);
+            }
+            store_tile<kC64>(v, pp, base | toff, G);
+        }
+    }
+}
+
+// Per item: the accumulators of its tiles and waves summed in a fixed order,
+// then added into grads[param[a]] in order (one thread).
+struct BCollect
+{
+    uint64_t partial;
+    uint64_t ntiles;
+    uint64_t params;  // int32[kAccs]
+    uint64_t grads;   // double*
+    int64_t nparams;  // length of grads
+};
+
+__global__ __launch_bounds__(64) void batch_collect_kernel(const BCollect *__restrict__ items)
+{
+    __shared__ double tot[64];
+    const BCollect &I = items[blockIdx.x];
+    const int j = threadIdx.x;
+    double s = 0.0;
+    if (j < kAccs)
+    {
+        const double *pp = reinterpret_cast<const double *>(I.partial);
+        const uint64_t nw = I.ntiles * kWavesPerTile;
+        for (uint64_t w = 0; w < nw; w++)
+            s += pp[w * kAccs + j];
+    }
+    tot[j] = s;
+    __syncthreads();
+    if (j == 0)
+    {
+        const int32_t *pm = reinterpret_cast<const int32_t *>(I.params);
+        double *gr = reinterpret_cast<double *>(I.grads);
+        for (int a = 0; a < kAccs; a++)
+            if (pm[a] >= 0 && pm[a] < I.nparams)
+                gr[pm[a]] += tot[a];
+    }
+}
+
 } // namespace
 
 unsigned tile_seg(int prec)
@@ -1414,6 +1563,265 @@ int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, con
                        static_cast<const int *>(dpar), dgrads);
     ZXA_HIP(hipGetLastError());
     return ctx.ring.done(ctx.stream);
+}
+
+} // namespace zxa
+
+namespace zxa
+{
+
+BatchLaunch::~BatchLaunch()
+{
+    // d_adjs and d_params point into the d_tabs allocation
+    for (void *p : {d_items, d_tabs, d_partial, d_collect, d_epi})
+        if (p)
+            (void)hipFree(p);
+    if (h_tables)
+        (void)hipHostFree(h_tables);
+}
+
+namespace
+{
+template <typename T>
+int dmalloc(T **p, size_t bytes)
+{
+    void *v = nullptr;
+    hipError_t e = hipMalloc(&v, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess)
+    {
+        set_error(std::string("batch: device allocation failed: ") + hipGetErrorString(e));
+        return ENOMEM_;
+    }
+    *p = static_cast<T *>(v);
+    return OK;
+}
+} // namespace
+
+int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec> &specs, BatchLaunch &b)
+{
+    const bool c64 = prec == C64;
+    b.prec = prec;
+    b.adj = adj;
+    b.nitems = specs.size();
+    b.slots.assign(specs.size(), {});
+    if (specs.empty())
+        return OK;
+    std::vector<BItem> items(specs.size());
+    std::vector<TTab> tabs(specs.size());
+    std::vector<int32_t> params(specs.size() * kAccs, -1);
+    // epilogue tables: per item, per-thread table then per-tile factors
+    std::vector<size_t> epi_off(specs.size(), 0), epi_tt(specs.size(), 0);
+    size_t epi_bytes = 0;
+    const int nb = c64 ? 5 : 4;
+    const size_t ttab_doubles = size_t(kThreads) * 2 * (1 + nb);
+    const size_t stride = 2 * (8 + (size_t(1) << nb));
+    uint64_t first = 0, partial_doubles = 0;
+    std::vector<Plan> plans(specs.size());
+    for (size_t i = 0; i < specs.size(); i++)
+    {
+        const BatchPassSpec &sp = specs[i];
+        if (sp.n < tile_min_bits(prec) || sp.n > 40)
+        {
+            set_error("batch: state narrower than a tile");
+            return EINVAL_;
+        }
+        std::vector<U3Gate> ug(sp.targets.size());
+        std::vector<SingleGate> gs;
+        for (size_t k = 0; k < sp.targets.size(); k++)
+        {
+            ug[k].packed_sel = ug[k].delta = uint64_t(1) << sp.targets[k];
+            gs.push_back({sp.targets[k], &ug[k]});
+        }
+        if (!plan_tile(prec, sp.n, gs, plans[i]))
+        {
+            set_error("batch: pass does not fit one tile");
+            return EINVAL_;
+        }
+        TPass P;
+        bool use[3];
+        fill_pass(prec, sp.n, plans[i], P, tabs[i], use);
+        b.slots[i] = plans[i].slot;
+        BItem &it = items[i];
+        std::memset(&it, 0, sizeof(it));
+        it.first = first;
+        it.ntiles = P.ntiles;
+        it.psi = reinterpret_cast<uint64_t>(sp.psi);
+        it.lam = reinterpret_cast<uint64_t>(sp.lam);
+        for (int k = 0; k < kUnitBits; k++)
+        {
+            it.ub[k] = P.ub[k];
+            it.sorted[k] = P.sorted[k];
+        }
+        first += P.ntiles;
+        if (adj)
+        {
+            it.partial = partial_doubles; // offset for now
+            partial_doubles += P.ntiles * kWavesPerTile * kAccs;
+        }
+        if (!sp.window.empty())
+        {
+            if (!window_is_polynomial(sp.n, sp.window.data(), sp.window.size()))
+            {
+                set_error("batch: window is not a phase polynomial");
+                return EINVAL_;
+            }
+            epi_tt[i] = epi_bytes;
+            epi_bytes += ttab_doubles * sizeof(double);
+            epi_off[i] = epi_bytes;
+            epi_bytes += P.ntiles * stride * sizeof(double);
+        }
+    }
+    b.total_tiles = first;
+    b.tab_bytes = specs.size() * sizeof(TTab);
+    b.adj_bytes = adj ? specs.size() * sizeof(TAdj) : 0;
+    b.par_bytes = adj ? params.size() * sizeof(int32_t) : 0;
+    char *dtabs = nullptr;
+    ZXA_TRY(dmalloc(&dtabs, b.tab_bytes + b.adj_bytes + b.par_bytes));
+    b.d_tabs = dtabs;
+    b.d_adjs = adj ? dtabs + b.tab_bytes : nullptr;
+    b.d_params = adj ? dtabs + b.tab_bytes + b.adj_bytes : nullptr;
+    ZXA_HIP(hipHostMalloc(&b.h_tables, b.tab_bytes + b.adj_bytes + b.par_bytes, hipHostMallocDefault));
+    std::memcpy(b.h_tables, tabs.data(), b.tab_bytes);
+    if (adj)
+    {
+        std::memset(static_cast<char *>(b.h_tables) + b.tab_bytes, 0, b.adj_bytes);
+        std::memcpy(static_cast<char *>(b.h_tables) + b.tab_bytes + b.adj_bytes, params.data(), b.par_bytes);
+    }
+    // the device tables start valid (identity gates, no parameters), so a
+    // launch before the first set/upload is harmless
+    ZXA_HIP(hipMemcpy(dtabs, b.h_tables, b.tab_bytes + b.adj_bytes + b.par_bytes, hipMemcpyHostToDevice));
+    char *depi = nullptr;
+    if (epi_bytes)
+        ZXA_TRY(dmalloc(&depi, epi_bytes));
+    b.d_epi = depi;
+    double *dpart = nullptr;
+    if (adj)
+        ZXA_TRY(dmalloc(&dpart, partial_doubles * sizeof(double)));
+    b.d_partial = dpart;
+    // epilogue tables (fixed): per-thread table uploaded, per-tile factors by tile_phase_kernel
+    for (size_t i = 0; i < specs.size(); i++)
+    {
+        BItem &it = items[i];
+        it.tab = reinterpret_cast<uint64_t>(dtabs + i * sizeof(TTab));
+        if (adj)
+        {
+            it.adj = reinterpret_cast<uint64_t>(dtabs + b.tab_bytes + i * sizeof(TAdj));
+            it.partial = reinterpret_cast<uint64_t>(dpart + it.partial);
+        }
+        const BatchPassSpec &sp = specs[i];
+        if (sp.window.empty())
+            continue;
+        Poly poly;
+        window_polynomial(sp.n, sp.window.data(), sp.window.size(), poly);
+        std::vector<double> tt, pt;
+        int nh = 0, nb2 = 0;
+        epilogue_tables(prec, sp.n, plans[i].ebit, poly, tt, pt, &nh, &nb2);
+        ZXA_HIP(hipMemcpy(depi + epi_tt[i], tt.data(), tt.size() * sizeof(double), hipMemcpyHostToDevice));
+        double *dpt = nullptr;
+        ZXA_TRY(dmalloc(&dpt, pt.size() * sizeof(double)));
+        ZXA_HIP(hipMemcpy(dpt, pt.data(), pt.size() * sizeof(double), hipMemcpyHostToDevice));
+        const unsigned grid = unsigned((it.ntiles + kThreads - 1) / kThreads);
+        hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), pt.size() * sizeof(double), ctx.stream, dpt,
+                           int(plans[i].ebit.size()), nh, nb2, poly.K, it.ntiles,
+                           reinterpret_cast<double *>(depi + epi_off[i]));
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipStreamSynchronize(ctx.stream);
+        (void)hipFree(dpt);
+        ZXA_HIP(e);
+        it.ttab = reinterpret_cast<uint64_t>(depi + epi_tt[i]);
+        it.tphase = reinterpret_cast<uint64_t>(depi + epi_off[i]);
+    }
+    ZXA_TRY(dmalloc(&b.d_items, items.size() * sizeof(BItem)));
+    ZXA_HIP(hipMemcpy(b.d_items, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
+    if (adj)
+    {
+        std::vector<BCollect> col(specs.size());
+        for (size_t i = 0; i < specs.size(); i++)
+        {
+            col[i].partial = items[i].partial;
+            col[i].ntiles = items[i].ntiles;
+            col[i].params =
+                reinterpret_cast<uint64_t>(dtabs + b.tab_bytes + b.adj_bytes + i * kAccs * sizeof(int32_t));
+            col[i].grads = reinterpret_cast<uint64_t>(specs[i].grads);
+            col[i].nparams = int64_t(specs[i].nparams);
+        }
+        ZXA_TRY(dmalloc(&b.d_collect, col.size() * sizeof(BCollect)));
+        ZXA_HIP(hipMemcpy(b.d_collect, col.data(), col.size() * sizeof(BCollect), hipMemcpyHostToDevice));
+    }
+    auto k = c64 ? (adj ? batch_tile_kernel<true, true> : batch_tile_kernel<true, false>)
+                 : (adj ? batch_tile_kernel<false, true> : batch_tile_kernel<false, false>);
+    b.grid = persistent_grid(ctx, k, b.total_tiles);
+    return OK;
+}
+
+void batch_set_gates(BatchLaunch &b, size_t item, const U3Gate *gates, size_t ngates)
+{
+    TTab *tab = reinterpret_cast<TTab *>(b.h_tables) + item;
+    for (size_t k = 0; k < ngates && k < b.slots[item].size(); k++)
+    {
+        TGate &tg = tab->g[b.slots[item][k]];
+        for (int i = 0; i < 8; i++)
+        {
+            tg.u[i] = gates[k].u[i];
+            tg.uf[i] = float(gates[k].u[i]);
+        }
+    }
+}
+
+void batch_set_adjoint(BatchLaunch &b, size_t item, const AdjTarget *targets, size_t ntargets)
+{
+    TTab *tab = reinterpret_cast<TTab *>(b.h_tables) + item;
+    TAdj *adj = reinterpret_cast<TAdj *>(static_cast<char *>(b.h_tables) + b.tab_bytes) + item;
+    int32_t *par = reinterpret_cast<int32_t *>(static_cast<char *>(b.h_tables) + b.tab_bytes + b.adj_bytes) +
+                   item * kAccs;
+    for (size_t k = 0; k < ntargets && k < b.slots[item].size(); k++)
+    {
+        const int slot = b.slots[item][k];
+        TGate &tg = tab->g[slot];
+        for (int i = 0; i < 8; i++)
+        {
+            tg.u[i] = targets[k].m[i];
+            tg.uf[i] = float(targets[k].m[i]);
+        }
+        for (int o = 0; o < kObs; o++)
+        {
+            for (int i = 0; i < 8; i++)
+            {
+                adj->o[slot].a[o][i] = targets[k].a[o][i];
+                adj->o[slot].af[o][i] = float(targets[k].a[o][i]);
+            }
+            par[slot * kObs + o] = targets[k].param[o];
+        }
+    }
+}
+
+int batch_upload(Ctx &ctx, BatchLaunch &b)
+{
+    if (!b.nitems)
+        return OK;
+    return hip_check(hipMemcpyAsync(b.d_tabs, b.h_tables, b.tab_bytes + b.adj_bytes + b.par_bytes,
+                                    hipMemcpyHostToDevice, ctx.stream),
+                     "batch table upload");
+}
+
+int batch_launch(Ctx &ctx, BatchLaunch &b)
+{
+    if (!b.nitems)
+        return OK;
+    const bool c64 = b.prec == C64;
+    auto k = c64 ? (b.adj ? batch_tile_kernel<true, true> : batch_tile_kernel<true, false>)
+                 : (b.adj ? batch_tile_kernel<false, true> : batch_tile_kernel<false, false>);
+    hipLaunchKernelGGL(k, dim3(b.grid), dim3(kThreads), 0, ctx.stream, static_cast<const BItem *>(b.d_items),
+                       int(b.nitems), b.total_tiles);
+    ZXA_HIP(hipGetLastError());
+    if (b.adj)
+    {
+        hipLaunchKernelGGL(batch_collect_kernel, dim3(unsigned(b.nitems)), dim3(64), 0, ctx.stream,
+                           static_cast<const BCollect *>(b.d_collect));
+        ZXA_HIP(hipGetLastError());
+    }
+    return OK;
 }
 
 } // namespace zxa

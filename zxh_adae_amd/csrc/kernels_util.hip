@@ -89,6 +89,27 @@ __global__ __launch_bounds__(kThreads) void scale_kernel(void *__restrict__ stat
     }
 }
 
+// Zero fill of n16 16-byte units (graph-capturable, unlike a memset node).
+__global__ __launch_bounds__(kThreads) void zero_kernel(float4 *__restrict__ p, uint64_t n16)
+{
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kThreads;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x; i < n16; i += stride)
+        p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Amplitude 0 of each listed state = 1.
+template <bool kC64>
+__global__ void set_ones_kernel(const uint64_t *__restrict__ states, int count)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count)
+        return;
+    if constexpr (kC64)
+        reinterpret_cast<float2 *>(states[i])[0] = make_float2(1.f, 0.f);
+    else
+        reinterpret_cast<double2 *>(states[i])[0] = make_double2(1.0, 0.0);
+}
+
 template <typename K>
 unsigned stream_grid(Ctx &ctx, K kernel, uint64_t n16)
 {
@@ -135,6 +156,31 @@ int scale_apply(Ctx &ctx, void *state, int prec, unsigned n, double re, double i
                            ctx.stream, state, n16, re, im);
     ZXA_HIP(hipGetLastError());
     return ctx.toc(F_SCALE, 2ull * bytes);
+}
+
+int zero_fill(Ctx &ctx, void *p, size_t bytes)
+{
+    if (bytes % 16)
+    {
+        set_error("zero_fill: size not a multiple of 16 bytes");
+        return EINVAL_;
+    }
+    const uint64_t n16 = bytes / 16;
+    const unsigned grid = unsigned(std::max<uint64_t>(1, std::min<uint64_t>((n16 + kThreads - 1) / kThreads, 1u << 16)));
+    hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<float4 *>(p), n16);
+    return hip_check(hipGetLastError(), "zero_kernel launch");
+}
+
+int set_ones(Ctx &ctx, const uint64_t *dstates, int count, int prec)
+{
+    if (count <= 0)
+        return OK;
+    const unsigned grid = unsigned((count + kThreads - 1) / kThreads);
+    if (prec == C64)
+        hipLaunchKernelGGL(set_ones_kernel<true>, dim3(grid), dim3(kThreads), 0, ctx.stream, dstates, count);
+    else
+        hipLaunchKernelGGL(set_ones_kernel<false>, dim3(grid), dim3(kThreads), 0, ctx.stream, dstates, count);
+    return hip_check(hipGetLastError(), "set_ones_kernel launch");
 }
 
 } // namespace zxa

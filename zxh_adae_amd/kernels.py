@@ -131,6 +131,14 @@ class DeviceState:
                                      out.ctypes.data_as(_lib._dp)))
         return float(out[0]), float(out[1])
 
+    def block5(self, targets: Sequence[int], u: np.ndarray):
+        """Dense 5-qubit unitary u (32x32, bit b of its index = targets[b])."""
+        t = np.ascontiguousarray(targets, dtype=np.uint32)
+        assert t.size == 5
+        m = np.ascontiguousarray(np.asarray(u, dtype=np.complex128).reshape(32, 32)).view(np.float64)
+        check(self.lib.zxh_k_block5(self.ptr, self.prec, self.nbits, t.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                    m.ctypes.data_as(_lib._dp)))
+
     def diag(self, sel0: Sequence[int], sel1: Sequence[int], theta: Sequence[float]):
         s0 = np.ascontiguousarray(sel0, dtype=np.uint64)
         s1 = np.ascontiguousarray(sel1, dtype=np.uint64)
@@ -182,7 +190,7 @@ def sync():
 
 
 FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6, "copy": 7,
-            "scale": 8, "fused": 9, "obs": 10, "adjoint": 11}
+            "scale": 8, "fused": 9, "obs": 10, "adjoint": 11, "block": 12}
 
 
 def set_timing(on: bool):
@@ -199,3 +207,38 @@ def timing(family: str) -> dict:
     launches, ms, nbytes = C.c_uint64(), C.c_double(), C.c_uint64()
     check(_lib.load().zxh_k_get_timing(FAMILIES[family], C.byref(launches), C.byref(ms), C.byref(nbytes)))
     return {"launches": launches.value, "ms": ms.value, "bytes": nbytes.value}
+
+
+def apply_layer(st: DeviceState, layer: dict):
+    """One layer of workloads.c5_layers on a device state: the dense blocks,
+    the diagonal batch as one sweep, then the multi-controlled X cascade."""
+    for targets, u in layer["blocks"]:
+        st.block5(targets, u)
+    s0, s1, th = [], [], []
+    for code, cq, q, theta in layer["diag"]:
+        if code == 4:  # P
+            s0.append(1 << q)
+            s1.append(NO_SEL)
+        else:  # CP
+            s0.append(1 << cq)
+            s1.append(1 << q)
+        th.append(theta)
+    if th:
+        st.diag(s0, s1, th)
+    for mask, t in layer["mcx"]:
+        st.mcx(mask, t)
+
+
+def set_basis_zero(st: DeviceState):
+    """|0...0> in place (zero fill, then amplitude 0 = 1)."""
+    check(st.lib.zxh_k_reset(st.ptr, st.prec, st.nbits))
+
+
+def mix_seed(base: int, salt: int = 0) -> int:
+    """SplitMix64 finaliser of base + golden gamma + salt: the sampler's
+    threshold-stream seed (measure_general.cpp:16-22)."""
+    m = 0xFFFFFFFFFFFFFFFF
+    z = (base + 0x9E3779B97F4A7C15 + salt) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)

@@ -110,3 +110,74 @@ def random_mixed(n: int, count: int, seed: int, kinds=(H, U3, P, CP, RZ, Z, X, C
                 cq += 1
         gs.append((k, cq, q, r.uniform() * TWO_PI, r.uniform() * TWO_PI, r.uniform() * TWO_PI))
     return gs
+
+
+def normal(r: SplitMix64) -> float:
+    """N(0, 1) by the Box-Muller transform of two stream uniforms (explicit,
+    so the value does not depend on a library's normal generator)."""
+    u1 = r.uniform()
+    u2 = r.uniform()
+    return math.sqrt(-2.0 * math.log(1.0 - u1)) * math.cos(TWO_PI * u2)
+
+
+def zz_observable(n: int, count: int, seed: int) -> List[tuple]:
+    """count terms (coef, i) for Z_i with probability 1/2, else (coef, i, j)
+    for Z_i Z_j on distinct i < j; coefficients N(0, 1)."""
+    r = SplitMix64(seed)
+    terms = []
+    for _ in range(count):
+        if r.uniform() < 0.5:
+            terms.append((normal(r), r.below(n)))
+        else:
+            i = r.below(n)
+            j = r.below(n - 1)
+            if j >= i:
+                j += 1
+            terms.append((normal(r), min(i, j), max(i, j)))
+    return terms
+
+
+def ansatz_params(count: int, seed: int) -> List[float]:
+    """count angles uniform in [-pi, pi)."""
+    r = SplitMix64(seed)
+    return [math.pi * (2.0 * r.uniform() - 1.0) for _ in range(count)]
+
+
+def haar_unitary(r: SplitMix64, d: int = 32):
+    """Haar-random d x d unitary: QR of a complex Ginibre matrix (entries
+    N(0,1)/sqrt(2) + i N(0,1)/sqrt(2) from the stream by Box-Muller) with the
+    phases of R's diagonal moved into Q (Mezzadri's construction)."""
+    import numpy as np
+    z = np.array([[complex(normal(r), normal(r)) for _ in range(d)] for _ in range(d)]) / math.sqrt(2.0)
+    q, rr = np.linalg.qr(z)
+    ph = np.diag(rr) / np.abs(np.diag(rr))
+    return q * ph[None, :]
+
+
+def c5_layers(n: int, layers: int, seed: int, mcx_per_layer: int = 8):
+    """The large random circuit: per layer, floor(n/5) disjoint 5-qubit Haar
+    blocks on a random qubit permutation, a diagonal batch of 64 P/CP gates,
+    and a cascade of multi-controlled X gates (2..6 controls, distinct random
+    qubits).  Returns a list of layers, each a dict with
+      "blocks": [(targets[5], u 32x32)], "diag": [(code, cq, q, theta)],
+      "mcx": [(control_mask, target)]."""
+    r = SplitMix64(seed)
+    out = []
+    for _ in range(layers):
+        perm = list(range(n))
+        for i in range(n - 1, 0, -1):  # Fisher-Yates from the stream
+            j = r.below(i + 1)
+            perm[i], perm[j] = perm[j], perm[i]
+        blocks = [(perm[5 * b:5 * b + 5], haar_unitary(r)) for b in range(n // 5)]
+        diag = [(g[0], g[1], g[2], g[3]) for g in diag_batch(n, 64, r.next_u64() & 0xFFFFFFFF)]
+        mcx = []
+        for _ in range(mcx_per_layer):
+            k = 2 + r.below(5)
+            qs = []
+            while len(qs) < k + 1:
+                q = r.below(n)
+                if q not in qs:
+                    qs.append(q)
+            mcx.append((sum(1 << q for q in qs[:k]), qs[k]))
+        out.append({"blocks": blocks, "diag": diag, "mcx": mcx})
+    return out
