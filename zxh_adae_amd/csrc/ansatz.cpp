@@ -362,7 +362,8 @@ int Ansatz::backward(const double *params, double *grads, double *energy)
         w.clear();
         if (jj > 0)
             pass_window(jj - 1, params, true, w);
-        ZXA_TRY(adjoint_pass_apply(ctx_, psi_, lam_, prec_, n_, ts.data(), ts.size(), w.data(), w.size(), dgrads_));
+        ZXA_TRY(adjoint_pass_apply(ctx_, psi_, lam_, prec_, n_, ts.data(), ts.size(), w.data(), w.size(), dgrads_,
+                                   nparams_));
     }
     if (nparams_ > 0)
         ZXA_HIP(hipMemcpyAsync(grads, dgrads_, nparams_ * sizeof(double), hipMemcpyDeviceToHost, ctx_.stream));

@@ -287,7 +287,7 @@ int AnsatzBatch::fill_tables(const double *params, bool backward)
         {
             const Set &s = sets_[bwd_map_[r][k].first];
             s.a->pass_adjoint(bwd_map_[r][k].second, params + s.poff, ts);
-            batch_set_adjoint(*bwd_[r], k, ts.data(), ts.size());
+            ZXA_TRY(batch_set_adjoint(*bwd_[r], k, ts.data(), ts.size()));
         }
         ZXA_TRY(batch_upload(ctx_, *bwd_[r]));
     }

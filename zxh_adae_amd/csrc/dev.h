@@ -221,8 +221,9 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Ga
                      const DiagGate *diag, size_t ndiag);
 // One target of an adjoint pass: the uncompute matrix m (= U^dagger of the
 // forward pass's fused single-qubit gate on this qubit, (re, im) x 4), up to
-// two transition operators a[k] (2x2 Hermitian; zero when unused) whose
-// values Im <lam| a[k] |psi> are added to grads[param[k]] (param < 0: none).
+// two transition operators a[k] (2x2 traceless Hermitian; zero when unused)
+// whose values Im <lam| a[k] |psi> are added to grads[param[k]] (param < 0:
+// none; param >= nparams is rejected).
 struct AdjTarget
 {
     unsigned target;
@@ -235,7 +236,7 @@ struct AdjTarget
 // then the diagonal window (given with the angles to apply) on both.
 // dgrads: device array of parameter gradients, accumulated in stream order.
 int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned nbits, const AdjTarget *targets,
-                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads);
+                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads, size_t nparams);
 // Dense 5-qubit block: y[t] = sum_t' u[t][t'] x[t'] over every group of 32
 // amplitudes differing in the target bits (bit b of t = targets[b]); u is
 // 32x32 complex, row-major (re, im) pairs.  One sweep, matrix cores.
@@ -262,17 +263,18 @@ struct BatchLaunch
     size_t nitems = 0;
     uint64_t total_tiles = 0;
     int grid = 0;
-    void *d_items = nullptr, *d_tabs = nullptr, *d_adjs = nullptr, *d_params = nullptr, *d_partial = nullptr,
-         *d_collect = nullptr, *d_epi = nullptr;
-    void *h_tables = nullptr; // pinned mirror of d_tabs | d_adjs | d_params
-    size_t tab_bytes = 0, adj_bytes = 0, par_bytes = 0;
+    void *d_items = nullptr, *d_tabs = nullptr, *d_adjs = nullptr, *d_partial = nullptr, *d_collect = nullptr,
+         *d_epi = nullptr;
+    void *h_tables = nullptr; // pinned mirror of d_tabs | d_adjs (d_adjs points into d_tabs)
+    size_t tab_bytes = 0, adj_bytes = 0;
     std::vector<std::vector<int>> slots; // per item: slot of each target
+    std::vector<size_t> nparams;         // adjoint: per item, length of its gradient array
     ~BatchLaunch();
 };
 int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec> &specs, BatchLaunch &out);
 // Matrices of item i for this run (into the pinned mirror).
 void batch_set_gates(BatchLaunch &b, size_t item, const U3Gate *gates, size_t ngates);
-void batch_set_adjoint(BatchLaunch &b, size_t item, const AdjTarget *targets, size_t ntargets);
+int batch_set_adjoint(BatchLaunch &b, size_t item, const AdjTarget *targets, size_t ntargets);
 // Copies the mirror to the device tables (async on ctx.stream).
 int batch_upload(Ctx &ctx, BatchLaunch &b);
 // The batched kernel (and, for adjoint launches, the gradient collection).

@@ -407,10 +407,13 @@ __device__ __forceinline__ void load_tile(typename UnitT<kC64>::T (&v)[kRegs], c
         v[r] = __builtin_nontemporal_load(p + (at | G->roff[r]));
 }
 
+// The store recomputes its addresses (at is laundered): keeping the loads'
+// 64-bit addresses alive across a tile's work costs two registers per unit.
 template <bool kC64>
 __device__ __forceinline__ void store_tile(const typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *p,
                                            uint64_t at, ZXA_CONST TTab *G)
 {
+    asm volatile("" : "+v"(at));
 #pragma unroll
     for (int r = 0; r < kRegs; r++)
         __builtin_nontemporal_store(v[r], p + (at | G->roff[r]));
@@ -523,101 +526,165 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 // ---------------------------------------------------------------------------
 // Adjoint pass: the reverse of one forward tile pass, on two states at once.
 // psi is the ansatz state and lam = (later gates)^dagger H psi, both taken at
-// the point just after the forward pass's single-qubit gates.  For every
-// target slot the pass accumulates up to two transition values
-//     Im <lam| A |psi>      (A: a 2x2 Hermitian operator on the slot's qubit,
-//                            the generator of a rotation conjugated by the
-//                            gates composed after it; the rotation's gradient)
-// and then applies the slot's uncompute matrix M = U^dagger to both states.
-// The diagonal epilogue (if any) is the previous forward pass's window with
-// negated angles, applied to both states.
+// the point just after the forward pass's single-qubit gates.  A rotation's
+// gradient is a transition value Im <lam| A |psi>, with A the rotation's
+// generator conjugated by the gates composed after it on the same qubit: a
+// traceless Hermitian 2x2 operator [[a, b], [conj(b), -a]] on the slot's
+// qubit.  For such A
+//     Im <lam|A|psi> = a S_d + Re(b) S_e + Im(b) S_f
+// with three sums over the slot's amplitude pairs (p0, p1), (l0, l1):
+//     S_d = Im(conj(l0) p0) - Im(conj(l1) p1)
+//     S_e = Im(conj(l0) p1) + Im(conj(l1) p0)
+//     S_f = Re(conj(l0) p1) - Re(conj(l1) p0)
+// so the kernel accumulates the three sums of every slot (whatever the
+// operators), and the collection combines them with each operator's (a, b).
+// After a slot's sums the pass applies the slot's uncompute matrix
+// M = U^dagger to both states.  The diagonal epilogue (if any) is the
+// previous forward pass's window with negated angles, on both states.
+// Per tile the sums are reduced over the wave by recursive halving: a step
+// on lane bit m splits a lane's list of length len at h = ceil(len / 2);
+// lanes with bit m keep (and add the partner's copy of) the upper part, the
+// others the lower part.  Each layout's list (its 4 slots x 3 sums, plus the
+// complex64 in-unit slot in layout 0) takes the steps of bits 32 and 16 right
+// after it is formed; the three remainders, concatenated, take the steps of
+// bits 8, 4, 2, 1.  Then every lane holds the wave total of one sum (or 0),
+// which it adds into its own double accumulator.
 // ---------------------------------------------------------------------------
 
-constexpr int kObs = 2;                     // transition operators per slot
+constexpr int kObs = 2;  // gradient operators per slot
+constexpr int kSums = 3; // bilinear sums per slot
 constexpr int kWavesPerTile = kThreads / 64;
-constexpr int kAccs = (kUnitBits + 1) * kObs; // accumulator lanes (< 64)
+constexpr int kAccs = (kUnitBits + 1) * kObs; // gradient entries per pass
 
-struct TObs
+__host__ __device__ constexpr int half_up(int n)
 {
-    double a[kObs][8]; // A (re, im) per operator; zero when unused
-    float af[kObs][8];
-};
-struct TAdj
-{
-    TObs o[kUnitBits + 1];
-};
-
-// Im(conj(l0) (A00 p0 + A01 p1) + conj(l1) (A10 p0 + A11 p1))
-template <typename R>
-__device__ __forceinline__ R pair_im(R p0x, R p0y, R p1x, R p1y, R l0x, R l0y, R l1x, R l1y, const R (&a)[8])
-{
-    const R a0x = a[0] * p0x - a[1] * p0y + a[2] * p1x - a[3] * p1y;
-    const R a0y = a[0] * p0y + a[1] * p0x + a[2] * p1y + a[3] * p1x;
-    const R a1x = a[4] * p0x - a[5] * p0y + a[6] * p1x - a[7] * p1y;
-    const R a1y = a[4] * p0y + a[5] * p0x + a[6] * p1y + a[7] * p1x;
-    return l0x * a0y - l0y * a0x + l1x * a1y - l1y * a1x;
+    return (n + 1) / 2;
 }
-
-__device__ __forceinline__ double wave_sum(double v)
+// list length of layout L's sums, and of its remainder after two steps
+__host__ __device__ constexpr int group_len(bool c64, int L)
 {
-#pragma unroll
-    for (int m = 32; m > 0; m >>= 1)
-        v += __shfl_xor(v, m);
-    return v;
+    return 4 * kSums + (c64 && L == 0 ? kSums : 0);
 }
-
+__host__ __device__ constexpr int group_rest(bool c64, int L)
+{
+    return half_up(half_up(group_len(c64, L)));
+}
 template <bool kC64>
-__device__ __forceinline__ void load_obs(Real<kC64> (&a)[8], ZXA_CONST TObs &o, int k)
+constexpr int kRest = group_rest(kC64, 0) + group_rest(kC64, 1) + group_rest(kC64, 2);
+
+// Lane holding the wave total of sum j of layout L's list.
+__host__ __device__ constexpr int sum_lane(bool c64, int L, int j)
 {
-#pragma unroll
-    for (int i = 0; i < 8; i++)
+    int lane = 0;
+    int len = group_len(c64, L);
+    for (int m = 32; m >= 16; m >>= 1)
     {
-        if constexpr (kC64)
-            a[i] = o.af[k][i];
-        else
-            a[i] = o.a[k][i];
+        const int h = half_up(len);
+        if (j >= h)
+        {
+            j -= h;
+            lane |= m;
+        }
+        len = h;
+    }
+    for (int g = 0; g < L; g++)
+        j += group_rest(c64, g);
+    len = group_rest(c64, 0) + group_rest(c64, 1) + group_rest(c64, 2);
+    for (int m = 8; m >= 1; m >>= 1)
+    {
+        const int h = half_up(len);
+        if (j >= h)
+        {
+            j -= h;
+            lane |= m;
+        }
+        len = h;
+    }
+    return lane;
+}
+static_assert(kRest<true> <= 16 && kRest<false> <= 16, "four final steps reduce the remainders to one");
+
+template <int kLen, int kMask, typename R>
+__device__ __forceinline__ void halve(const R (&x)[kLen], R (&y)[half_up(kLen)], int lane)
+{
+    constexpr int kH = half_up(kLen);
+    const bool up = (lane & kMask) != 0;
+#pragma unroll
+    for (int i = 0; i < kH; i++)
+    {
+        const R lo = x[i];
+        const R hi = i + kH < kLen ? x[i + kH] : R(0);
+        y[i] = (up ? hi : lo) + R(__shfl_xor(up ? lo : hi, kMask));
     }
 }
 
-// Slot s of layout L (register bit i; s = kIntra for the complex64 in-unit
-// bit): transition values into the lane-indexed accumulator, then M on both.
-// emit(a, tot): the wave-wide transition value tot of accumulator a (slot
-// * kObs + operator), identical in every lane.
-template <bool kC64, int L, typename Emit>
-__device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
-                                           ZXA_CONST TTab *G, ZXA_CONST TAdj *A, Emit &&emit)
+template <int kLen, int kMask>
+__device__ __forceinline__ double halve_all(const double (&x)[kLen], int lane)
 {
+    if constexpr (kLen == 1)
+    {
+        return x[0];
+    }
+    else
+    {
+        double y[half_up(kLen)];
+        halve<kLen, kMask, double>(x, y, lane);
+        return halve_all<half_up(kLen), kMask / 2>(y, lane);
+    }
+}
+
+// One gradient entry of a pass: param (< 0: none) += c[0] S_d + c[1] S_e +
+// c[2] S_f of the slot whose sums sit in lanes lane[0..2].
+struct TGrad
+{
+    int32_t param;
+    int32_t lane[kSums];
+    double c[kSums];
+};
+
+// The three sums of one amplitude pair, accumulated.
+template <typename R>
+__device__ __forceinline__ void pair_sums(R p0x, R p0y, R p1x, R p1y, R l0x, R l0y, R l1x, R l1y, R &sd, R &se,
+                                          R &sf)
+{
+    sd += l0x * p0y - l0y * p0x - l1x * p1y + l1y * p1x;
+    se += l0x * p1y - l0y * p1x + l1x * p0y - l1y * p0x;
+    sf += l0x * p1x + l0y * p1y - l1x * p0x - l1y * p0y;
+}
+
+// Slots of layout L (register bit i at list position 3 i; the complex64
+// in-unit slot of layout 0 at 12): sums, then M on both states; the list's
+// first two halving steps into rest.
+template <bool kC64, int L>
+__device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
+                                           ZXA_CONST TTab *G, Real<kC64> (&rest)[group_rest(kC64, L)], int lane)
+{
+    using R = Real<kC64>;
+    constexpr int kN = group_len(kC64, L);
+    R s[kN];
 #pragma unroll
     for (int i = 0; i < 4; i++)
     {
         const int k = reg_bit(L, i);
+        R sd = 0, se = 0, sf = 0;
 #pragma unroll
-        for (int o = 0; o < kObs; o++)
+        for (int r = 0; r < kRegs; r++)
         {
-            Real<kC64> a[8];
-            load_obs<kC64>(a, A->o[k], o);
-            double part = 0.0;
-#pragma unroll
-            for (int r = 0; r < kRegs; r++)
-            {
-                if ((r >> i) & 1)
-                    continue;
-                const int r1 = r | (1 << i);
-                if constexpr (kC64)
-                {
-                    part += double(pair_im<float>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x,
-                                                  w[r1].y, a));
-                    part += double(pair_im<float>(v[r].z, v[r].w, v[r1].z, v[r1].w, w[r].z, w[r].w, w[r1].z,
-                                                  w[r1].w, a));
-                }
-                else
-                {
-                    part += pair_im<double>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x, w[r1].y, a);
-                }
-            }
-            emit(k * kObs + o, wave_sum(part));
+            if ((r >> i) & 1)
+                continue;
+            const int r1 = r | (1 << i);
+            pair_sums<R>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x, w[r1].y, sd, se, sf);
+            if constexpr (kC64)
+                pair_sums<float>(v[r].z, v[r].w, v[r1].z, v[r1].w, w[r].z, w[r].w, w[r1].z, w[r1].w, sd, se, sf);
         }
-        Real<kC64> m[8];
+        // the sums read the slot's inputs: formed here (not sunk towards
+        // their use after the stores, which keeps old copies of the states)
+        asm volatile("" : "+v"(sd), "+v"(se), "+v"(sf));
+        __builtin_amdgcn_sched_barrier(0);
+        s[i * kSums + 0] = sd;
+        s[i * kSums + 1] = se;
+        s[i * kSums + 2] = sf;
+        R m[8];
         load_matrix<kC64>(m, G->g[k]);
 #pragma unroll
         for (int r = 0; r < kRegs; r++)
@@ -627,20 +694,20 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
             unit_pair<kC64, false>(v[r], v[r | (1 << i)], m, false, false);
             unit_pair<kC64, false>(w[r], w[r | (1 << i)], m, false, false);
         }
+        // one slot at a time: interleaving slots only lengthens live ranges
+        __builtin_amdgcn_sched_barrier(0);
     }
     if constexpr (kC64 && L == 0)
     {
+        float sd = 0, se = 0, sf = 0;
 #pragma unroll
-        for (int o = 0; o < kObs; o++)
-        {
-            float a[8];
-            load_obs<true>(a, A->o[kIntra], o);
-            double part = 0.0;
-#pragma unroll
-            for (int r = 0; r < kRegs; r++)
-                part += double(pair_im<float>(v[r].x, v[r].y, v[r].z, v[r].w, w[r].x, w[r].y, w[r].z, w[r].w, a));
-            emit(kIntra * kObs + o, wave_sum(part));
-        }
+        for (int r = 0; r < kRegs; r++)
+            pair_sums<float>(v[r].x, v[r].y, v[r].z, v[r].w, w[r].x, w[r].y, w[r].z, w[r].w, sd, se, sf);
+        asm volatile("" : "+v"(sd), "+v"(se), "+v"(sf));
+        __builtin_amdgcn_sched_barrier(0);
+        s[4 * kSums + 0] = sd;
+        s[4 * kSums + 1] = se;
+        s[4 * kSums + 2] = sf;
         float m[8];
         load_matrix<true>(m, G->g[kIntra]);
 #pragma unroll
@@ -652,19 +719,87 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
             w[r] = fvec4{pw.ax, pw.ay, pw.bx, pw.by};
         }
     }
+    R h[half_up(kN)];
+    halve<kN, 32, R>(s, h, lane);
+    halve<half_up(kN), 16, R>(h, rest, lane);
+#pragma unroll
+    for (int i = 0; i < group_rest(kC64, L); i++)
+        asm volatile("" : "+v"(rest[i]));
 }
 
-// partial: per wave, kAccs doubles (lane-indexed accumulators).
+// Remainders of the three layouts (zeros for an unused layout).
+template <bool kC64>
+struct AdjRest
+{
+    Real<kC64> r0[group_rest(kC64, 0)], r1[group_rest(kC64, 1)], r2[group_rest(kC64, 2)];
+    __device__ __forceinline__ double total(int lane) const
+    {
+        double c[kRest<kC64>];
+        int n = 0;
+#pragma unroll
+        for (int i = 0; i < group_rest(kC64, 0); i++)
+            c[n++] = double(r0[i]);
+#pragma unroll
+        for (int i = 0; i < group_rest(kC64, 1); i++)
+            c[n++] = double(r1[i]);
+#pragma unroll
+        for (int i = 0; i < group_rest(kC64, 2); i++)
+            c[n++] = double(r2[i]);
+        return halve_all<kRest<kC64>, 8>(c, lane);
+    }
+};
+
+// All layouts of one tile on both states.
+template <bool kC64, bool kL1, bool kL2>
+__device__ __forceinline__ void adj_tile(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
+                                         typename UnitT<kC64>::T *lds, ZXA_CONST TTab *G, int t, AdjRest<kC64> &R)
+{
+    const int lane = t & 63;
+#pragma unroll
+    for (int i = 0; i < group_rest(kC64, 1); i++)
+        R.r1[i] = 0;
+#pragma unroll
+    for (int i = 0; i < group_rest(kC64, 2); i++)
+        R.r2[i] = 0;
+    adj_layout<kC64, 0>(v, w, G, R.r0, lane);
+    if constexpr (kL1)
+    {
+        transpose<kC64, 0, 1>(v, lds, t);
+        transpose<kC64, 0, 1>(w, lds, t);
+        adj_layout<kC64, 1>(v, w, G, R.r1, lane);
+        if constexpr (kL2)
+        {
+            transpose<kC64, 1, 2>(v, lds, t);
+            transpose<kC64, 1, 2>(w, lds, t);
+            adj_layout<kC64, 2>(v, w, G, R.r2, lane);
+            transpose<kC64, 2, 0>(v, lds, t);
+            transpose<kC64, 2, 0>(w, lds, t);
+        }
+        else
+        {
+            transpose<kC64, 1, 0>(v, lds, t);
+            transpose<kC64, 1, 0>(w, lds, t);
+        }
+    }
+    else if constexpr (kL2)
+    {
+        transpose<kC64, 0, 2>(v, lds, t);
+        transpose<kC64, 0, 2>(w, lds, t);
+        adj_layout<kC64, 2>(v, w, G, R.r2, lane);
+        transpose<kC64, 2, 0>(v, lds, t);
+        transpose<kC64, 2, 0>(w, lds, t);
+    }
+}
+
+// partial: one row of 64 lane accumulators per wave.
 template <bool kC64, bool kL1, bool kL2, bool kDiag>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void adj_kernel(
     void *__restrict__ psi, void *__restrict__ lam, const TPass P, const TTab *__restrict__ tabp,
-    const TAdj *__restrict__ adjp, const double *__restrict__ ttab, const double *__restrict__ tphase,
-    double *__restrict__ partial)
+    const double *__restrict__ ttab, const double *__restrict__ tphase, double *__restrict__ partial)
 {
     using T = typename UnitT<kC64>::T;
     __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
     const int t = threadIdx.x;
-    const int lane = t & 63;
     T *pp = static_cast<T *>(psi);
     T *pl = static_cast<T *>(lam);
     const uint64_t toff = thr_off<0>(t, P);
@@ -672,74 +807,62 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     if constexpr (kDiag)
         epi.init(ttab, t);
     double acc = 0.0;
-    auto emit = [&](int a, double tot) { acc += lane == a ? tot : 0.0; };
     uint64_t base = deposit(uint64_t(blockIdx.x), P.sorted);
     for (uint64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x)
     {
         ZXA_CONST TTab *G = tables(tabp);
-        const TAdj *A0 = adjp;
-        asm volatile("" : "+s"(A0));
-        ZXA_CONST TAdj *A = (ZXA_CONST TAdj *)A0;
         T v[kRegs], w[kRegs];
         load_tile<kC64>(v, pp, base | toff, G);
         load_tile<kC64>(w, pl, base | toff, G);
-        adj_layout<kC64, 0>(v, w, G, A, emit);
-        if constexpr (kL1)
-        {
-            transpose<kC64, 0, 1>(v, lds, t);
-            transpose<kC64, 0, 1>(w, lds, t);
-            adj_layout<kC64, 1>(v, w, G, A, emit);
-            if constexpr (kL2)
-            {
-                transpose<kC64, 1, 2>(v, lds, t);
-                transpose<kC64, 1, 2>(w, lds, t);
-                adj_layout<kC64, 2>(v, w, G, A, emit);
-                transpose<kC64, 2, 0>(v, lds, t);
-                transpose<kC64, 2, 0>(w, lds, t);
-            }
-            else
-            {
-                transpose<kC64, 1, 0>(v, lds, t);
-                transpose<kC64, 1, 0>(w, lds, t);
-            }
-        }
-        else if constexpr (kL2)
-        {
-            transpose<kC64, 0, 2>(v, lds, t);
-            transpose<kC64, 0, 2>(w, lds, t);
-            adj_layout<kC64, 2>(v, w, G, A, emit);
-            transpose<kC64, 2, 0>(v, lds, t);
-            transpose<kC64, 2, 0>(w, lds, t);
-        }
+        AdjRest<kC64> R;
+        adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
         if constexpr (kDiag)
             epi.template apply_n<true>(v, w, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), t);
         store_tile<kC64>(v, pp, base | toff, G);
         store_tile<kC64>(w, pl, base | toff, G);
+        acc += R.total(t & 63);
         base = next_base(base, P);
     }
-    if (lane < kAccs)
-        partial[(size_t(blockIdx.x) * (kThreads / 64) + (t >> 6)) * kAccs + lane] = acc;
+    partial[(size_t(blockIdx.x) * kWavesPerTile + (t >> 6)) * 64 + (t & 63)] = acc;
 }
 
-// Sums the per-wave accumulators in a fixed order and adds slot/operator
-// (s, o)'s total into grads[param[s * kObs + o]] (param < 0: not a
-// gradient).  One thread per accumulator; distinct accumulators of one pass
-// may share a parameter, so the adds into grads run in one thread, in order.
-__global__ __launch_bounds__(64) void adj_collect_kernel(const double *__restrict__ partial, int nwaves,
-                                                         const int *__restrict__ param, double *__restrict__ grads)
+// Sums rows [0, nrows) of 64 lane accumulators in a fixed order, then adds
+// every gradient entry into grads[param] (param < nparams) in entry order, so
+// that entries of one pass sharing a parameter never race.
+__device__ __forceinline__ void collect_rows(const double *__restrict__ partial, uint64_t nrows,
+                                             const TGrad *__restrict__ tg, int64_t nparams, double *__restrict__ grads)
 {
+    static_assert(kWavesPerTile == 4, "four row groups");
+    __shared__ double part[kWavesPerTile][64];
     __shared__ double tot[64];
-    const int j = threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int q = threadIdx.x >> 6;
     double s = 0.0;
-    if (j < kAccs)
-        for (int w = 0; w < nwaves; w++)
-            s += partial[size_t(w) * kAccs + j];
-    tot[j] = s;
+    for (uint64_t r = q; r < nrows; r += kWavesPerTile)
+        s += partial[r * 64 + lane];
+    part[q][lane] = s;
     __syncthreads();
-    if (j == 0)
+    if (threadIdx.x < 64)
+        tot[lane] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
         for (int a = 0; a < kAccs; a++)
-            if (param[a] >= 0)
-                grads[param[a]] += tot[a];
+        {
+            const TGrad &g = tg[a];
+            if (g.param < 0 || g.param >= nparams)
+                continue;
+            grads[g.param] += g.c[0] * tot[g.lane[0] & 63] + g.c[1] * tot[g.lane[1] & 63] +
+                              g.c[2] * tot[g.lane[2] & 63];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void adj_collect_kernel(const double *__restrict__ partial, uint64_t nrows,
+                                                               const TGrad *__restrict__ tg, int64_t nparams,
+                                                               double *__restrict__ grads)
+{
+    collect_rows(partial, nrows, tg, nparams, grads);
 }
 
 __device__ __forceinline__ cd cis(double a)
@@ -1164,8 +1287,8 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
 }
 
 template <bool kC64, bool kL1, bool kL2, bool kDiag>
-int launch_adj_t(Ctx &ctx, void *psi, void *lam, const TPass &P0, const TTab *tab, const TAdj *adj,
-                 const double *ttab, const double *tphase, double *partial, int *nwaves)
+int launch_adj_t(Ctx &ctx, void *psi, void *lam, const TPass &P0, const TTab *tab, const double *ttab,
+                 const double *tphase, double *partial, int *nwaves)
 {
     auto k = adj_kernel<kC64, kL1, kL2, kDiag>;
     const int grid = persistent_grid(ctx, k, P0.ntiles);
@@ -1178,21 +1301,76 @@ int launch_adj_t(Ctx &ctx, void *psi, void *lam, const TPass &P0, const TTab *ta
     TPass P = P0;
     P.gstep = deposit(uint64_t(grid), P.sorted);
     *nwaves = grid * (kThreads / 64);
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, psi, lam, P, tab, adj, ttab, tphase, partial);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, psi, lam, P, tab, ttab, tphase, partial);
     return hip_check(hipGetLastError(), "adj_kernel launch");
 }
 
 template <bool kC64, bool kDiag>
 int launch_adj_l(Ctx &ctx, void *psi, void *lam, const TPass &P, bool l1, bool l2, const TTab *tab,
-                 const TAdj *adj, const double *ttab, const double *tphase, double *partial, int *nwaves)
+                 const double *ttab, const double *tphase, double *partial, int *nwaves)
 {
     if (l1 && l2)
-        return launch_adj_t<kC64, true, true, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
+        return launch_adj_t<kC64, true, true, kDiag>(ctx, psi, lam, P, tab, ttab, tphase, partial, nwaves);
     if (l1)
-        return launch_adj_t<kC64, true, false, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
+        return launch_adj_t<kC64, true, false, kDiag>(ctx, psi, lam, P, tab, ttab, tphase, partial, nwaves);
     if (l2)
-        return launch_adj_t<kC64, false, true, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
-    return launch_adj_t<kC64, false, false, kDiag>(ctx, psi, lam, P, tab, adj, ttab, tphase, partial, nwaves);
+        return launch_adj_t<kC64, false, true, kDiag>(ctx, psi, lam, P, tab, ttab, tphase, partial, nwaves);
+    return launch_adj_t<kC64, false, false, kDiag>(ctx, psi, lam, P, tab, ttab, tphase, partial, nwaves);
+}
+
+// Gradient entries of one target in slot `slot` (out: kAccs entries).  The
+// operators must be traceless and Hermitian (rotation generators conjugated
+// by unitaries); a = (A00 - A11) / 2, b = (A01 + conj(A10)) / 2.
+int grad_entries(bool c64, int slot, const AdjTarget &tg, size_t nparams, TGrad *out)
+{
+    // the slot's layout and list position
+    int L = 0, pos = 4 * kSums;
+    if (slot != kIntra)
+        for (int l = 0; l < 3; l++)
+            for (int i = 0; i < 4; i++)
+                if (reg_bit(l, i) == slot)
+                {
+                    L = l;
+                    pos = i * kSums;
+                }
+    for (int o = 0; o < kObs; o++)
+    {
+        TGrad &g = out[slot * kObs + o];
+        const double *A = tg.a[o];
+        g.param = tg.param[o];
+        for (int j = 0; j < kSums; j++)
+            g.lane[j] = sum_lane(c64, L, pos + j);
+        g.c[0] = 0.5 * (A[0] - A[6]);
+        g.c[1] = 0.5 * (A[2] + A[4]);
+        g.c[2] = 0.5 * (A[3] - A[5]);
+        if (g.param < 0)
+            continue;
+        if (size_t(g.param) >= nparams)
+        {
+            set_error("adjoint pass: parameter index outside the gradient array");
+            return EINVAL_;
+        }
+        double scale = 1.0;
+        for (int k = 0; k < 8; k++)
+            scale = std::max(scale, std::fabs(A[k]));
+        const double off = std::max({std::fabs(A[0] + A[6]), std::fabs(A[1]), std::fabs(A[7]), std::fabs(A[2] - A[4]),
+                                     std::fabs(A[3] + A[5])});
+        if (off > 1e-9 * scale)
+        {
+            set_error("adjoint pass: transition operator is not traceless Hermitian");
+            return EINVAL_;
+        }
+    }
+    return OK;
+}
+
+void clear_grads(TGrad *g)
+{
+    for (int a = 0; a < kAccs; a++)
+    {
+        std::memset(&g[a], 0, sizeof(TGrad));
+        g[a].param = -1;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1209,10 +1387,9 @@ struct BItem
     uint64_t psi;     // state (16-byte units)
     uint64_t lam;     // adjoint: second state
     uint64_t tab;     // TTab
-    uint64_t adj;     // adjoint: TAdj
     uint64_t ttab;    // epilogue per-thread table, 0: no window
     uint64_t tphase;  // epilogue per-tile factors
-    uint64_t partial; // adjoint: per tile and wave, kAccs accumulators
+    uint64_t partial; // adjoint: per tile and wave, a row of 64 lane sums
     uint32_t ub[kUnitBits];
     uint32_t sorted[kUnitBits];
 };
@@ -1257,24 +1434,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         if constexpr (kAdj)
         {
             T *pl = reinterpret_cast<T *>(I.lam);
-            ZXA_CONST TAdj *A = (ZXA_CONST TAdj *)I.adj;
             T w[kRegs];
             load_tile<kC64>(w, pl, base | toff, G);
-            // one wave-wide value per accumulator, stored by lane 0 at once
-            double *pw = reinterpret_cast<double *>(I.partial) + (local * kWavesPerTile + (t >> 6)) * kAccs;
-            auto emit = [&](int a, double tot) {
-                if (lane == 0)
-                    pw[a] = tot;
-            };
-            adj_layout<kC64, 0>(v, w, G, A, emit);
-            transpose<kC64, 0, 1>(v, lds, t);
-            transpose<kC64, 0, 1>(w, lds, t);
-            adj_layout<kC64, 1>(v, w, G, A, emit);
-            transpose<kC64, 1, 2>(v, lds, t);
-            transpose<kC64, 1, 2>(w, lds, t);
-            adj_layout<kC64, 2>(v, w, G, A, emit);
-            transpose<kC64, 2, 0>(v, lds, t);
-            transpose<kC64, 2, 0>(w, lds, t);
+            AdjRest<kC64> R;
+            adj_tile<kC64, true, true>(v, w, lds, G, t, R);
             if (I.ttab != 0)
             {
                 Epi<kC64> epi;
@@ -1285,6 +1448,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             }
             store_tile<kC64>(v, pp, base | toff, G);
             store_tile<kC64>(w, pl, base | toff, G);
+            reinterpret_cast<double *>(I.partial)[(local * kWavesPerTile + (t >> 6)) * 64 + lane] = R.total(lane);
         }
         else
         {
@@ -1306,40 +1470,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 }
 
-// Per item: the accumulators of its tiles and waves summed in a fixed order,
-// then added into grads[param[a]] in order (one thread).
+// Per item (one workgroup each): the rows of its tiles and waves summed in a
+// fixed order, then its gradient entries added in order.
 struct BCollect
 {
     uint64_t partial;
     uint64_t ntiles;
-    uint64_t params;  // int32[kAccs]
+    uint64_t grad;    // TGrad[kAccs]
     uint64_t grads;   // double*
     int64_t nparams;  // length of grads
 };
 
-__global__ __launch_bounds__(64) void batch_collect_kernel(const BCollect *__restrict__ items)
+__global__ __launch_bounds__(kThreads) void batch_collect_kernel(const BCollect *__restrict__ items)
 {
-    __shared__ double tot[64];
     const BCollect &I = items[blockIdx.x];
-    const int j = threadIdx.x;
-    double s = 0.0;
-    if (j < kAccs)
-    {
-        const double *pp = reinterpret_cast<const double *>(I.partial);
-        const uint64_t nw = I.ntiles * kWavesPerTile;
-        for (uint64_t w = 0; w < nw; w++)
-            s += pp[w * kAccs + j];
-    }
-    tot[j] = s;
-    __syncthreads();
-    if (j == 0)
-    {
-        const int32_t *pm = reinterpret_cast<const int32_t *>(I.params);
-        double *gr = reinterpret_cast<double *>(I.grads);
-        for (int a = 0; a < kAccs; a++)
-            if (pm[a] >= 0 && pm[a] < I.nparams)
-                gr[pm[a]] += tot[a];
-    }
+    collect_rows(reinterpret_cast<const double *>(I.partial), I.ntiles * kWavesPerTile,
+                 reinterpret_cast<const TGrad *>(I.grad), I.nparams, reinterpret_cast<double *>(I.grads));
 }
 
 } // namespace
@@ -1464,7 +1610,7 @@ namespace zxa
 {
 
 int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, const AdjTarget *targets,
-                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads)
+                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads, size_t nparams)
 {
     const bool c64 = prec == C64;
     if (n < tile_min_bits(prec) || n > 40)
@@ -1508,9 +1654,8 @@ int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, con
     TTab G;
     bool use[3];
     fill_pass(prec, n, pl, P, G, use);
-    TAdj A;
-    std::memset(&A, 0, sizeof(A));
-    std::vector<int> param(kAccs, -1);
+    TGrad tg[kAccs];
+    clear_grads(tg);
     for (size_t i = 0; i < ntargets; i++)
     {
         const int slot = pl.slot[i];
@@ -1519,48 +1664,35 @@ int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, con
             G.g[slot].u[k] = targets[i].m[k];
             G.g[slot].uf[k] = float(targets[i].m[k]);
         }
-        for (int o = 0; o < kObs; o++)
-        {
-            for (int k = 0; k < 8; k++)
-            {
-                A.o[slot].a[o][k] = targets[i].a[o][k];
-                A.o[slot].af[o][k] = float(targets[i].a[o][k]);
-            }
-            param[slot * kObs + o] = targets[i].param[o];
-        }
+        ZXA_TRY(grad_entries(c64, slot, targets[i], nparams, tg));
     }
     const double *ttab = nullptr;
     const double *tphase = nullptr;
     if (ndiag > 0)
         ZXA_TRY(setup_epilogue(ctx, prec, n, pl.ebit, diag, ndiag, P.ntiles, &ttab, &tphase));
-    void *dtab = nullptr, *dadj = nullptr, *dpar = nullptr;
+    void *dtab = nullptr, *dgr = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
-    ZXA_TRY(ctx.ring.stage(&A, sizeof(A), ctx.stream, &dadj));
-    ZXA_TRY(ctx.ring.stage(param.data(), param.size() * sizeof(int), ctx.stream, &dpar));
-    // per-wave partials: at most (2 workgroups per CU) x CUs x 4 waves
-    const size_t maxw = size_t(8) * size_t(ctx.num_cus > 0 ? ctx.num_cus : 256) * (kThreads / 64);
-    ZXA_TRY(ctx.scratch_f.ensure(maxw * kAccs * sizeof(double)));
+    ZXA_TRY(ctx.ring.stage(tg, sizeof(tg), ctx.stream, &dgr));
+    // per-wave rows of 64 lane accumulators: at most 8 workgroups per CU
+    const size_t maxw = size_t(8) * size_t(ctx.num_cus > 0 ? ctx.num_cus : 256) * kWavesPerTile;
+    ZXA_TRY(ctx.scratch_f.ensure(maxw * 64 * sizeof(double)));
     double *partial = static_cast<double *>(ctx.scratch_f.ptr);
     const TTab *tab = static_cast<const TTab *>(dtab);
-    const TAdj *adj = static_cast<const TAdj *>(dadj);
     const uint64_t bytes = 4ull * (uint64_t(1) << n) * elem_bytes(prec);
     int nwaves = int(maxw);
     ZXA_TRY(ctx.tic());
     int rc;
     if (c64)
-        rc = ndiag ? launch_adj_l<true, true>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
-                                              &nwaves)
-                   : launch_adj_l<true, false>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
-                                               &nwaves);
+        rc = ndiag ? launch_adj_l<true, true>(ctx, psi, lam, P, use[1], use[2], tab, ttab, tphase, partial, &nwaves)
+                   : launch_adj_l<true, false>(ctx, psi, lam, P, use[1], use[2], tab, ttab, tphase, partial, &nwaves);
     else
-        rc = ndiag ? launch_adj_l<false, true>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
-                                               &nwaves)
-                   : launch_adj_l<false, false>(ctx, psi, lam, P, use[1], use[2], tab, adj, ttab, tphase, partial,
+        rc = ndiag ? launch_adj_l<false, true>(ctx, psi, lam, P, use[1], use[2], tab, ttab, tphase, partial, &nwaves)
+                   : launch_adj_l<false, false>(ctx, psi, lam, P, use[1], use[2], tab, ttab, tphase, partial,
                                                 &nwaves);
     ZXA_TRY(rc);
     ZXA_TRY(ctx.toc(F_ADJ, bytes));
-    hipLaunchKernelGGL(adj_collect_kernel, dim3(1), dim3(64), 0, ctx.stream, partial, nwaves,
-                       static_cast<const int *>(dpar), dgrads);
+    hipLaunchKernelGGL(adj_collect_kernel, dim3(1), dim3(kThreads), 0, ctx.stream, partial, uint64_t(nwaves),
+                       static_cast<const TGrad *>(dgr), int64_t(nparams), dgrads);
     ZXA_HIP(hipGetLastError());
     return ctx.ring.done(ctx.stream);
 }
@@ -1572,7 +1704,7 @@ namespace zxa
 
 BatchLaunch::~BatchLaunch()
 {
-    // d_adjs and d_params point into the d_tabs allocation
+    // d_adjs points into the d_tabs allocation
     for (void *p : {d_items, d_tabs, d_partial, d_collect, d_epi})
         if (p)
             (void)hipFree(p);
@@ -1608,7 +1740,10 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         return OK;
     std::vector<BItem> items(specs.size());
     std::vector<TTab> tabs(specs.size());
-    std::vector<int32_t> params(specs.size() * kAccs, -1);
+    std::vector<TGrad> grads(adj ? specs.size() * kAccs : 0);
+    for (size_t i = 0; i < specs.size() && adj; i++)
+        clear_grads(&grads[i * kAccs]);
+    b.nparams.assign(specs.size(), 0);
     // epilogue tables: per item, per-thread table then per-tile factors
     std::vector<size_t> epi_off(specs.size(), 0), epi_tt(specs.size(), 0);
     size_t epi_bytes = 0;
@@ -1656,7 +1791,8 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         if (adj)
         {
             it.partial = partial_doubles; // offset for now
-            partial_doubles += P.ntiles * kWavesPerTile * kAccs;
+            partial_doubles += P.ntiles * kWavesPerTile * 64;
+            b.nparams[i] = sp.nparams;
         }
         if (!sp.window.empty())
         {
@@ -1673,23 +1809,18 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
     }
     b.total_tiles = first;
     b.tab_bytes = specs.size() * sizeof(TTab);
-    b.adj_bytes = adj ? specs.size() * sizeof(TAdj) : 0;
-    b.par_bytes = adj ? params.size() * sizeof(int32_t) : 0;
+    b.adj_bytes = grads.size() * sizeof(TGrad);
     char *dtabs = nullptr;
-    ZXA_TRY(dmalloc(&dtabs, b.tab_bytes + b.adj_bytes + b.par_bytes));
+    ZXA_TRY(dmalloc(&dtabs, b.tab_bytes + b.adj_bytes));
     b.d_tabs = dtabs;
     b.d_adjs = adj ? dtabs + b.tab_bytes : nullptr;
-    b.d_params = adj ? dtabs + b.tab_bytes + b.adj_bytes : nullptr;
-    ZXA_HIP(hipHostMalloc(&b.h_tables, b.tab_bytes + b.adj_bytes + b.par_bytes, hipHostMallocDefault));
+    ZXA_HIP(hipHostMalloc(&b.h_tables, b.tab_bytes + b.adj_bytes, hipHostMallocDefault));
     std::memcpy(b.h_tables, tabs.data(), b.tab_bytes);
     if (adj)
-    {
-        std::memset(static_cast<char *>(b.h_tables) + b.tab_bytes, 0, b.adj_bytes);
-        std::memcpy(static_cast<char *>(b.h_tables) + b.tab_bytes + b.adj_bytes, params.data(), b.par_bytes);
-    }
-    // the device tables start valid (identity gates, no parameters), so a
-    // launch before the first set/upload is harmless
-    ZXA_HIP(hipMemcpy(dtabs, b.h_tables, b.tab_bytes + b.adj_bytes + b.par_bytes, hipMemcpyHostToDevice));
+        std::memcpy(static_cast<char *>(b.h_tables) + b.tab_bytes, grads.data(), b.adj_bytes);
+    // the device tables start valid (no gradient entries), so a launch
+    // before the first set/upload is harmless
+    ZXA_HIP(hipMemcpy(dtabs, b.h_tables, b.tab_bytes + b.adj_bytes, hipMemcpyHostToDevice));
     char *depi = nullptr;
     if (epi_bytes)
         ZXA_TRY(dmalloc(&depi, epi_bytes));
@@ -1704,10 +1835,7 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         BItem &it = items[i];
         it.tab = reinterpret_cast<uint64_t>(dtabs + i * sizeof(TTab));
         if (adj)
-        {
-            it.adj = reinterpret_cast<uint64_t>(dtabs + b.tab_bytes + i * sizeof(TAdj));
             it.partial = reinterpret_cast<uint64_t>(dpart + it.partial);
-        }
         const BatchPassSpec &sp = specs[i];
         if (sp.window.empty())
             continue;
@@ -1741,8 +1869,7 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         {
             col[i].partial = items[i].partial;
             col[i].ntiles = items[i].ntiles;
-            col[i].params =
-                reinterpret_cast<uint64_t>(dtabs + b.tab_bytes + b.adj_bytes + i * kAccs * sizeof(int32_t));
+            col[i].grad = reinterpret_cast<uint64_t>(dtabs + b.tab_bytes + i * kAccs * sizeof(TGrad));
             col[i].grads = reinterpret_cast<uint64_t>(specs[i].grads);
             col[i].nparams = int64_t(specs[i].nparams);
         }
@@ -1769,12 +1896,11 @@ void batch_set_gates(BatchLaunch &b, size_t item, const U3Gate *gates, size_t ng
     }
 }
 
-void batch_set_adjoint(BatchLaunch &b, size_t item, const AdjTarget *targets, size_t ntargets)
+int batch_set_adjoint(BatchLaunch &b, size_t item, const AdjTarget *targets, size_t ntargets)
 {
     TTab *tab = reinterpret_cast<TTab *>(b.h_tables) + item;
-    TAdj *adj = reinterpret_cast<TAdj *>(static_cast<char *>(b.h_tables) + b.tab_bytes) + item;
-    int32_t *par = reinterpret_cast<int32_t *>(static_cast<char *>(b.h_tables) + b.tab_bytes + b.adj_bytes) +
-                   item * kAccs;
+    TGrad *gr = reinterpret_cast<TGrad *>(static_cast<char *>(b.h_tables) + b.tab_bytes) + item * kAccs;
+    clear_grads(gr);
     for (size_t k = 0; k < ntargets && k < b.slots[item].size(); k++)
     {
         const int slot = b.slots[item][k];
@@ -1784,23 +1910,16 @@ void batch_set_adjoint(BatchLaunch &b, size_t item, const AdjTarget *targets, si
             tg.u[i] = targets[k].m[i];
             tg.uf[i] = float(targets[k].m[i]);
         }
-        for (int o = 0; o < kObs; o++)
-        {
-            for (int i = 0; i < 8; i++)
-            {
-                adj->o[slot].a[o][i] = targets[k].a[o][i];
-                adj->o[slot].af[o][i] = float(targets[k].a[o][i]);
-            }
-            par[slot * kObs + o] = targets[k].param[o];
-        }
+        ZXA_TRY(grad_entries(b.prec == C64, slot, targets[k], b.nparams[item], gr));
     }
+    return OK;
 }
 
 int batch_upload(Ctx &ctx, BatchLaunch &b)
 {
     if (!b.nitems)
         return OK;
-    return hip_check(hipMemcpyAsync(b.d_tabs, b.h_tables, b.tab_bytes + b.adj_bytes + b.par_bytes,
+    return hip_check(hipMemcpyAsync(b.d_tabs, b.h_tables, b.tab_bytes + b.adj_bytes,
                                     hipMemcpyHostToDevice, ctx.stream),
                      "batch table upload");
 }
@@ -1817,7 +1936,7 @@ int batch_launch(Ctx &ctx, BatchLaunch &b)
     ZXA_HIP(hipGetLastError());
     if (b.adj)
     {
-        hipLaunchKernelGGL(batch_collect_kernel, dim3(unsigned(b.nitems)), dim3(64), 0, ctx.stream,
+        hipLaunchKernelGGL(batch_collect_kernel, dim3(unsigned(b.nitems)), dim3(kThreads), 0, ctx.stream,
                            static_cast<const BCollect *>(b.d_collect));
         ZXA_HIP(hipGetLastError());
     }
