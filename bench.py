@@ -204,6 +204,80 @@ def adjoint_c3(W, repeats=3, check_fp64=True):
     return out
 
 
+def batch_c4(W, repeats=3, sequential=64):
+    """512 parameter sets of the two-local ansatz, widths 16..24 and depths
+    2..10, 50-term observables, complex64: one batched engine (forward pass p
+    of every set in one launch, replayed from a hipGraph).  Timed forward-only
+    and forward+adjoint runs; the single-circuit path on every k-th set gives
+    the sequential estimate and the per-set gradient check."""
+    import numpy as np
+    from zxh_adae_amd.ansatz import Ansatz, AnsatzBatch, two_local
+    sets = W.c4_sets()
+    b = AnsatzBatch("c64")
+    nps = []
+    for n, layers, terms in sets:
+        ops, npar = two_local(n, layers)
+        b.add(n, ops, npar, terms)
+        nps.append(npar)
+    t0 = time.perf_counter()
+    b.finalize()
+    t_setup = time.perf_counter() - t0
+    offs = np.concatenate([[0], np.cumsum(nps)])
+    params = np.array(W.ansatz_params(int(offs[-1]), 4))
+    b.run(params, gradients=True)
+    tf = tb = 0.0
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        b.run(params, gradients=False)
+        t1 = time.perf_counter()
+        e, norms, grads = b.run(params, gradients=True)
+        t2 = time.perf_counter()
+        tf += t1 - t0
+        tb += t2 - t1
+    tf /= repeats
+    tb /= repeats
+    info = b.info()
+    # algorithmic bytes (S = state bytes): forward = reset write S + passes x 2S
+    # + observable read S; backward adds lam = H psi (2S), the last window on
+    # both states (4S) and passes x 4S
+    fwd_bytes = bwd_bytes = 0
+    for i in range(len(sets)):
+        nb, ps = b.set_info(i)
+        S = 8 << nb
+        fwd_bytes += S + ps * 2 * S + S
+        bwd_bytes += S + ps * 2 * S + 2 * S + 4 * S + ps * 4 * S
+    del b
+    step = max(1, len(sets) // sequential)
+    t_seq = 0.0
+    nseq = 0
+    gdiff = ediff = 0.0
+    for i in range(0, len(sets), step):
+        n, layers, terms = sets[i]
+        ops, npar = two_local(n, layers)
+        a = Ansatz(n, ops, npar, "c64")
+        a.set_observable(terms)
+        p = params[offs[i]:offs[i + 1]]
+        a.forward(p)
+        a.backward(p)
+        t0 = time.perf_counter()
+        e1, _ = a.forward(p)
+        g1, _ = a.backward(p)
+        t_seq += time.perf_counter() - t0
+        nseq += 1
+        gdiff = max(gdiff, float(np.max(np.abs(g1 - grads[i]))))
+        ediff = max(ediff, abs(e1 - e[i]))
+        a.close()
+    seq_est = t_seq * len(sets) / nseq
+    return {"sets": len(sets), "qubits": "16..24", "layers": "2..10", "terms": 50, "state": "complex64",
+            "params": int(offs[-1]), "launches_per_run": info["launches"], "setup_s": round(t_setup, 3),
+            "forward_ms": round(tf * 1e3, 3), "forward_backward_ms": round(tb * 1e3, 3),
+            "sets/s": round(len(sets) / tb, 1), "grads/s": round(int(offs[-1]) / tb, 1),
+            "forward_GB/s": round(fwd_bytes / tf / 1e9, 1), "forward_backward_GB/s": round(bwd_bytes / tb / 1e9, 1),
+            "sequential_single_circuit_ms": round(seq_est * 1e3, 3), "sequential_sample": nseq,
+            "speedup_vs_sequential": round(seq_est / tb, 2),
+            "max_grad_diff_vs_single": gdiff, "max_energy_diff_vs_single": ediff}
+
+
 def c5_circuit(W, K, prec, n, layers=40, shots=1000000, seed=5):
     """The large random circuit: per layer floor(n/5) Haar 5-qubit blocks
     (matrix cores), a 64-gate diagonal batch, 8 multi-controlled X gates;
@@ -395,6 +469,8 @@ def main():
     if not args.no_extras:
         log("adjoint 28q c64 (two-local, 8 layers, 200 terms)")
         result["adjoint_c3"] = adjoint_c3(W)
+        log("batched ansatz: 512 sets, 16..24 qubits, 2..10 layers, c64")
+        result["batch_c4"] = batch_c4(W)
         log("large random circuit 33q c64, 40 layers, 1e6 shots")
         result["c5"] = {"c64": c5_circuit(W, K, "c64", 33)}
         log("large random circuit 32q c128, 40 layers, 1e6 shots")

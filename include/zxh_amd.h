@@ -274,6 +274,8 @@ int zxh_batch_finalize(zxh_batch *b);
  * (<H>, <psi|psi>); grads (nullable: forward only) concatenated like params */
 int zxh_batch_run(zxh_batch *b, const double *params, double *energies, double *grads);
 int zxh_batch_info(zxh_batch *b, size_t *sets, size_t *total_params, size_t *launches);
+/* one set's width and number of tile passes per forward (after finalize) */
+int zxh_batch_set_info(zxh_batch *b, size_t set, unsigned *nbits, size_t *passes);
 
 /* tuning knobs (process-wide), set by the autotuner (zxh_adae_amd.autotune):
  *   "diag.ept.c128", "diag.ept.c64": 0 = persistent diagonal sweep,

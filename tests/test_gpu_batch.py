@@ -43,6 +43,7 @@ def test_batch_matches_single_circuits_over_replays(gpu_engine, prec, widths):
         for k, (n, ops, npar, terms) in enumerate(sets):
             a = Ansatz(n, ops, npar, prec)
             a.set_observable(terms)
+            assert b.set_info(k) == (n, a.tile_passes())
             gk, ek = a.backward(params[k])
             scale = max(1.0, sum(abs(t[0]) for t in terms))
             assert abs(e[k] - ek) < tol * scale, (replay, k)
@@ -62,3 +63,6 @@ def test_batch_rejects_narrow_and_unfinalized(gpu_engine):
         b.run(np.zeros(npar))
     with pytest.raises(ZXHError):
         b.add(8, *two_local(8, 1), [(1.0, 0)])
+    b.finalize()
+    with pytest.raises(ZXHError):
+        b.set_info(1)

@@ -101,6 +101,7 @@ _SIGS = [
     ("zxh_batch_finalize", C.c_int, [_vp]),
     ("zxh_batch_run", C.c_int, [_vp, _dp, _dp, _dp]),
     ("zxh_batch_info", C.c_int, [_vp, C.POINTER(_sz), C.POINTER(_sz), C.POINTER(_sz)]),
+    ("zxh_batch_set_info", C.c_int, [_vp, _sz, C.POINTER(C.c_uint), C.POINTER(_sz)]),
     ("zxh_k_apply_z", C.c_int, [_vp, _vp, C.c_int, _sz, _i32p, _i32p, _dp, _sz, _dp]),
     ("zxh_tune_set", C.c_int, [C.c_char_p, C.c_int]),
     ("zxh_tune_get", C.c_int, [C.c_char_p, C.POINTER(C.c_int)]),

@@ -194,3 +194,9 @@ class AnsatzBatch:
         s, t, l = C.c_size_t(), C.c_size_t(), C.c_size_t()
         check(self.lib.zxh_batch_info(self._h, C.byref(s), C.byref(t), C.byref(l)))
         return {"sets": s.value, "params": t.value, "launches": l.value}
+
+    def set_info(self, i: int):
+        """(qubits, tile passes per forward) of set i, after finalize."""
+        nb, ps = C.c_uint(), C.c_size_t()
+        check(self.lib.zxh_batch_set_info(self._h, int(i), C.byref(nb), C.byref(ps)))
+        return nb.value, ps.value

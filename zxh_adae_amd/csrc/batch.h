@@ -41,6 +41,14 @@ class AnsatzBatch
     {
         return total_params_;
     }
+    unsigned set_nbits(size_t i) const
+    {
+        return sets_[i].a->nbits();
+    }
+    size_t set_passes(size_t i) const
+    {
+        return sets_[i].a->tile_passes();
+    }
     size_t launches() const
     {
         return fwd_.size() + bwd_.size() + (init_ ? 1 : 0) + 2;

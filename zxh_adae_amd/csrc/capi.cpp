@@ -742,6 +742,18 @@ int zxh_batch_info(zxh_batch *b, size_t *sets, size_t *total_params, size_t *lau
     return ZXH_OK;
 }
 
+int zxh_batch_set_info(zxh_batch *b, size_t set, unsigned *nbits, size_t *passes)
+{
+    CHECK_BATCH(b);
+    if (set >= b->b->sets())
+        return fail(ZXH_EINVAL, "batch: set index out of range");
+    if (nbits)
+        *nbits = b->b->set_nbits(set);
+    if (passes)
+        *passes = b->b->set_passes(set);
+    return ZXH_OK;
+}
+
 int zxh_tune_set(const char *name, int value)
 {
     if (!name)

@@ -181,3 +181,17 @@ def c5_layers(n: int, layers: int, seed: int, mcx_per_layer: int = 8):
             mcx.append((sum(1 << q for q in qs[:k]), qs[k]))
         out.append({"blocks": blocks, "diag": diag, "mcx": mcx})
     return out
+
+
+def c4_sets(count: int = 512, seed: int = 4, terms: int = 50):
+    """The batched uneven ansatz workload: count parameter sets of the
+    two-local ansatz with widths n ~ U{16..24} and depths ~ U{2..10}, each
+    with its own terms-term Z/ZZ observable.  Returns a list of
+    (n, layers, observable terms); the observables' seeds derive from seed."""
+    r = SplitMix64(seed)
+    out = []
+    for i in range(count):
+        n = 16 + r.below(9)
+        layers = 2 + r.below(9)
+        out.append((n, layers, zz_observable(n, terms, (seed << 32) + i)))
+    return out
