@@ -355,17 +355,29 @@ def kernels_only(Z, K, W, repeats, prec):
     th = [g[3] for g in gs]
     one = [((1 << 29), (1 << 29), W.u3_matrix(0.3, 0.2, 0.1))]
     allq = [((1 << g[2]), (1 << g[2]), W.u3_matrix(g[3], g[4], g[5])) for g in W.u3_all(NQ, 77)]
+    # a fused pass as the QFT's: 12 U3 targets (the lowest free bits and the
+    # highest the geometry allows) plus a 32-gate P/CP window, autotuned geometry
+    from zxh_adae_amd import autotune
+    tuned = autotune.tune_tile(st)
+    seg = tuned["best"]
+    low = seg + (1 if prec == "c64" else 0)
+    ftargets = list(range(low)) + list(range(NQ - (12 - seg), NQ))
+    fdescs = [((1 << q), (1 << q), W.u3_matrix(0.3 + 0.01 * q, 0.2, 0.1)) for q in ftargets]
+    wg = W.diag_batch(NQ, 32, 4343)
+    w0 = [(1 << g[2]) for g in wg]
+    w1 = [NO_SEL if g[0] == W.P else (1 << g[1]) for g in wg]
+    wt = [g[3] for g in wg]
     K.reset_timing()
     K.set_timing(True)
-    out = {}
+    out = {"tile_seg": seg}
     for name, fn in (("copy", lambda: other.copy_from(st)), ("scale", lambda: st.scale(1.0, 0.0)),
                      ("diag", lambda: st.diag(s0, s1, th)), ("u3_one", lambda: st.u3_batch(one)),
-                     ("u3_all", lambda: st.u3_batch(allq))):
+                     ("u3_all", lambda: st.u3_batch(allq)), ("fused", lambda: st.fused_pass(fdescs, w0, w1, wt))):
         log(f"kernel {name} x{repeats} ({prec})")
         for _ in range(repeats):
             fn()
         K.sync()
-    for fam in ("copy", "scale", "diag", "u3_tile"):
+    for fam in ("copy", "scale", "diag", "u3_tile", "fused"):
         t = K.timing(fam)
         if t["launches"]:
             avg = t["ms"] / t["launches"]

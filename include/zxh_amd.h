@@ -282,9 +282,11 @@ int zxh_batch_set_info(zxh_batch *b, size_t set, unsigned *nbits, size_t *passes
  *                                    1, 2, 4 = flat grid, 16-byte units per thread
  *   "diag.nt.c128", "diag.nt.c64": nontemporal loads/stores in the flat
  *                                  sweep (0/1)
- *   "tile.seg.c128", "tile.seg.c64": 4, 5, 6 contiguous 16-byte-unit address
+ *   "tile.seg.c128", "tile.seg.c64": 3, 4, 5, 6 contiguous 16-byte-unit address
  *                                    bits per tile pass (targets per pass: 12
- *                                    minus this above them) */
+ *                                    minus this above them)
+ *   "tile.bufs.c128", "tile.bufs.c64": 2 or 3 register buffers per tile-pass
+ *                                      workgroup (tiles in flight) */
 int zxh_tune_set(const char *name, int value);
 int zxh_tune_get(const char *name, int *value);
 /* HIP-event timing of the kernel-level calls (same families as zxh_get_timing,

@@ -37,16 +37,18 @@ def gpu_engine():
     return Z
 
 
-@pytest.fixture(params=[4, 5, 6], ids=lambda s: f"seg{s}")
+@pytest.fixture(params=[(3, 2), (4, 2), (5, 2), (6, 2), (3, 3), (6, 3)], ids=lambda p: f"seg{p[0]}-bufs{p[1]}")
 def tile_seg(request, gpu_engine):
-    """Runs a test under each tile geometry (knobs tile.seg.c128 and
-    tile.seg.c64: 4, 5 or 6 contiguous unit-address bits per tile) and
-    restores the knobs."""
+    """Runs a test under each tile geometry (knobs tile.seg.<prec>: 3, 4, 5
+    or 6 contiguous unit-address bits per tile) and with two or three register
+    buffers per tile-pass workgroup (knobs tile.bufs.<prec>); restores the
+    knobs.  Yields the geometry."""
     from zxh_adae_amd import autotune
-    knobs = ("tile.seg.c128", "tile.seg.c64")
-    old = [autotune.tune_get(k) for k in knobs]
-    for k in knobs:
-        autotune.tune_set(k, request.param)
-    yield request.param
-    for k, v in zip(knobs, old):
+    seg, bufs = request.param
+    knobs = {"tile.seg.c128": seg, "tile.seg.c64": seg, "tile.bufs.c128": bufs, "tile.bufs.c64": bufs}
+    old = {k: autotune.tune_get(k) for k in knobs}
+    for k, v in knobs.items():
+        autotune.tune_set(k, v)
+    yield seg
+    for k, v in old.items():
         autotune.tune_set(k, v)

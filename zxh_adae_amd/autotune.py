@@ -67,14 +67,28 @@ def tune_diag(state: "K.DeviceState", repeats: int = 3, batch: int = 64) -> Dict
             "ms": {f"ept{e}_nt{n}": round(v, 4) for (e, n), v in results.items()}}
 
 
-TILE_SEGS = (4, 5, 6)
+TILE_SEGS = (3, 4, 5, 6)
 
 
-def tune_tile(state: "K.DeviceState", repeats: int = 3, window: int = 32) -> Dict[str, object]:
-    """Times one fused tile pass per tile geometry on state (U3 on every low
-    target and on the highest targets the geometry allows, then a window of
-    P/CP gates) and sets the geometry with the shortest pass.  The state is
-    changed by the passes."""
+def layer_passes(n: int, seg: int, c64: bool):
+    """Targets of each tile pass of a full-width single-qubit layer under
+    geometry seg: the low bits ride in the first pass, then 12 - seg upper
+    targets per pass."""
+    low = seg + (1 if c64 else 0)
+    slots = 12 - seg
+    upper = list(range(low, n))
+    chunks = [upper[i:i + slots] for i in range(0, len(upper), slots)] or [[]]
+    chunks[0] = list(range(low)) + chunks[0]
+    return chunks
+
+
+def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dict[str, object]:
+    """Times a full-width layer per tile geometry on state: U3 on every qubit,
+    split into the geometry's tile passes, each pass fused with a window of
+    P/CP gates.  Fewer contiguous bits per tile leave room for more targets
+    per pass (fewer passes) at shorter contiguous runs, so the choice is by
+    the layer's total time.  Sets the fastest geometry; the state is changed
+    by the passes."""
     c64 = state.prec == _lib.ZXH_C64
     knob = "tile.seg.c64" if c64 else "tile.seg.c128"
     n = state.nbits
@@ -84,20 +98,24 @@ def tune_tile(state: "K.DeviceState", repeats: int = 3, window: int = 32) -> Dic
     s1 = [K.NO_SEL if g[0] == W.P else (1 << g[1]) for g in gs]
     th = [g[3] for g in gs]
     results: Dict[int, float] = {}
+    npasses: Dict[int, int] = {}
     for seg in TILE_SEGS:
         tune_set(knob, seg)
-        low = seg + (1 if c64 else 0)
-        targets = list(range(low)) + list(range(n - (12 - seg), n))
-        descs = [((1 << q), (1 << q), W.u3_matrix(0.3 + 0.01 * q, 0.2, 0.1)) for q in targets]
-        state.fused_pass(descs, s0, s1, th)  # warm
+        passes = [[((1 << q), (1 << q), W.u3_matrix(0.3 + 0.01 * q, 0.2, 0.1)) for q in chunk]
+                  for chunk in layer_passes(n, seg, c64)]
+        for descs in passes:  # warm
+            state.fused_pass(descs, s0, s1, th)
         K.reset_timing()
         K.set_timing(True)
         for _ in range(repeats):
-            state.fused_pass(descs, s0, s1, th)
+            for descs in passes:
+                state.fused_pass(descs, s0, s1, th)
         t = K.timing("fused")
         K.set_timing(False)
-        results[seg] = t["ms"] / max(t["launches"], 1)
+        results[seg] = t["ms"] / repeats
+        npasses[seg] = len(passes)
     best = min(results, key=results.get)
     tune_set(knob, best)
     return {"knob": knob, "best": best, "previous": old,
-            "ms": {f"seg{k}": round(v, 4) for k, v in results.items()}}
+            "layer_ms": {f"seg{k}": round(v, 4) for k, v in results.items()},
+            "passes": {f"seg{k}": v for k, v in npasses.items()}}

@@ -255,8 +255,10 @@ Knob g_knobs[] = {
     {"diag.ept.c64", 2, 0, 4},
     {"diag.nt.c128", 1, 0, 1},
     {"diag.nt.c64", 1, 0, 1},
-    {"tile.seg.c128", 4, 4, 6},
-    {"tile.seg.c64", 4, 4, 6},
+    {"tile.seg.c128", 4, 3, 6},
+    {"tile.seg.c64", 4, 3, 6},
+    {"tile.bufs.c128", 2, 2, 3},
+    {"tile.bufs.c64", 2, 2, 3},
 };
 std::mutex g_knob_mu;
 } // namespace

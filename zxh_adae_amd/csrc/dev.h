@@ -176,7 +176,8 @@ struct Ctx
 //   diag.ept.c128 / diag.ept.c64  0 = persistent sweep, 1/2/4 = flat grid with
 //                                 that many 16-byte units per thread
 //   diag.nt.c128 / diag.nt.c64     nontemporal loads/stores in the flat sweep
-//   tile.seg.c128 / tile.seg.c64   contiguous unit-address bits of a tile (4..6)
+//   tile.seg.c128 / tile.seg.c64   contiguous unit-address bits of a tile (3..6)
+//   tile.bufs.c128 / tile.bufs.c64 register buffers per tile-pass workgroup (2, 3)
 int tune_get(const char *name);
 int tune_set(const char *name, int value);
 
@@ -203,7 +204,7 @@ struct U3Gate
     double u[8];
 };
 
-// Tile geometry (tuning knob tile.seg.<prec> = 4, 5 or 6 contiguous
+// Tile geometry (tuning knob tile.seg.<prec> = 3, 4, 5 or 6 contiguous
 // 16-byte-unit address bits): a tile pass reaches every target below tile_low_bits(prec)
 // and up to tile_upper_slots(prec) more; it needs a state of at least
 // tile_min_bits(prec) bits (12 for complex128, 13 for complex64).

@@ -8,9 +8,10 @@
 //
 // A tile is 4096 16-byte units (complex128: one amplitude; complex64: two
 // amplitudes adjacent in address bit 0) whose unit addresses differ in 12
-// chosen "unit bits": the four lowest unit-address bits, so that each 16-lane
-// group of a wave moves 256 contiguous bytes, plus up to eight target bits.
-// A complex64 tile therefore spans address bits 0..4 and eight more.  Each
+// chosen "unit bits": the seg lowest unit-address bits (the tile geometry,
+// seg = 3..6: 128 B to 1 KiB contiguous per wave access), plus up to 12 - seg
+// target bits; fewer contiguous bits leave room for more targets per pass.
+// A complex64 tile therefore spans address bits 0..seg and 12 - seg more.  Each
 // workgroup streams through tiles; each tile is loaded once, every gate whose
 // target lies in the tile is applied, the diagonal epilogue multiplies each
 // amplitude by its phase, and the tile is written back.
@@ -78,10 +79,11 @@ struct TGate
     uint32_t pad;
 };
 
-// Kernel arguments of a pass.  Unit bits 0..3 are unit-address bits 0..3;
-// the host assigns the other eight freely (targets first into the register
-// bits of layout 0, then of layout 1), so ub is not sorted; sorted holds the
-// same positions ascending.
+// Kernel arguments of a pass.  Unit bits 0..seg-1 are unit-address bits
+// 0..seg-1 (seg = 3..6, the tile geometry); the host assigns the others
+// freely (targets first into the register bits of layout 0, then of layout
+// 1, then unit bit 3), so ub is not sorted; sorted holds the same positions
+// ascending.
 struct TPass
 {
     uint32_t ub[kUnitBits];     // unit-address bit of each unit bit
@@ -104,7 +106,8 @@ struct TTab
 
 // Register bits of the three layouts (unit bits); the thread bits are the
 // remaining eight in ascending order.  Layout 0 is the load/store layout: its
-// thread bits start with unit bits 0..3, the contiguous ones.
+// thread bits start with unit bits 0..5, of which the lowest seg are the
+// contiguous ones.
 __host__ __device__ constexpr int reg_bit(int L, int i)
 {
     return L == 0 ? 8 + i : (L == 1 ? 4 + i : i);
@@ -520,6 +523,82 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         t1 = t3;
         b1 = b3;
         m1 = m3;
+    }
+}
+
+// The same pipeline with three register buffers: while tile k is worked on,
+// tiles k + 1 and k + 2 are in flight (twice the loads in flight per
+// workgroup; the workgroup count per CU is bound by LDS).  Past the last tile
+// the prefetch re-reads the last valid one.
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void tile_kernel3(
+    void *__restrict__ state, const TPass P, const TTab *__restrict__ tabp, const double *__restrict__ ttab,
+    const double *__restrict__ tphase)
+{
+    using T = typename UnitT<kC64>::T;
+    __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
+    const int t = threadIdx.x;
+    T *p = static_cast<T *>(state);
+    const uint64_t toff = thr_off<0>(t, P);
+    Epi<kC64> epi;
+    if constexpr (kDiag)
+        epi.init(ttab, t);
+    const uint64_t grid = gridDim.x;
+
+    T a[kRegs], b[kRegs], c[kRegs];
+    uint64_t t0 = blockIdx.x;
+    uint64_t b0 = deposit(t0, P.sorted);
+    ZXA_CONST TTab *G = tables(tabp);
+    load_tile<kC64>(a, p, b0 | toff, G);
+    uint64_t t1 = t0 + grid;
+    bool m1 = t1 < P.ntiles;
+    uint64_t b1 = m1 ? next_base(b0, P) : b0;
+    load_tile<kC64>(b, p, b1 | toff, G);
+    uint64_t t2 = t1 + grid;
+    bool m2 = m1 && t2 < P.ntiles;
+    uint64_t b2 = m2 ? next_base(b1, P) : b1;
+    load_tile<kC64>(c, p, b2 | toff, G);
+    tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t0, b0);
+    for (;;)
+    {
+        G = tables(tabp);
+        store_tile<kC64>(a, p, b0 | toff, G);
+        if (!m1)
+            break;
+        const uint64_t t3 = t2 + grid;
+        const bool m3 = m2 && t3 < P.ntiles;
+        const uint64_t b3 = m3 ? next_base(b2, P) : b2;
+        load_tile<kC64>(a, p, b3 | toff, G);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip>(b, lds, P, G, tphase, epi, t, toff, t1, b1);
+
+        G = tables(tabp);
+        store_tile<kC64>(b, p, b1 | toff, G);
+        if (!m2)
+            break;
+        const uint64_t t4 = t3 + grid;
+        const bool m4 = m3 && t4 < P.ntiles;
+        const uint64_t b4 = m4 ? next_base(b3, P) : b3;
+        load_tile<kC64>(b, p, b4 | toff, G);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip>(c, lds, P, G, tphase, epi, t, toff, t2, b2);
+
+        G = tables(tabp);
+        store_tile<kC64>(c, p, b2 | toff, G);
+        if (!m3)
+            break;
+        const uint64_t t5 = t4 + grid;
+        const bool m5 = m4 && t5 < P.ntiles;
+        const uint64_t b5 = m5 ? next_base(b4, P) : b4;
+        load_tile<kC64>(c, p, b5 | toff, G);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t3, b3);
+
+        t0 = t3;
+        b0 = b3;
+        t1 = t4;
+        b1 = b4;
+        m1 = m4;
+        t2 = t5;
+        b2 = b5;
+        m2 = m5;
     }
 }
 
@@ -952,7 +1031,12 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t tiles)
 template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
 int launch_t(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const double *ttab, const double *tphase)
 {
+    // register buffers per workgroup (knob tile.bufs.<prec>); the flip
+    // variants keep two (their tables take the registers of a third)
     auto k = tile_kernel<kC64, kL1, kL2, kDiag, kFlip>;
+    if constexpr (!kFlip)
+        if (tune_get(kC64 ? "tile.bufs.c64" : "tile.bufs.c128") == 3)
+            k = tile_kernel3<kC64, kL1, kL2, kDiag, false>;
     const int grid = persistent_grid(ctx, k, P0.ntiles);
     TPass P = P0;
     P.gstep = deposit(uint64_t(grid), P.sorted);
@@ -993,11 +1077,12 @@ struct SingleGate
 // Tile plan: the address bit of every tile element bit e (complex128: e is
 // the unit bit; complex64: e = 0 is address bit 0 inside a unit and e = k + 1
 // is unit bit k), and the unit bit of every target.  Unit bits 0..seg-1 are
-// the lowest unit-address bits (seg = 4, 5 or 6: 256 B, 512 B or 1 KiB
+// the lowest unit-address bits (seg = 3, 4, 5 or 6: 128 B to 1 KiB
 // contiguous per wave access); targets above them go to the register bits of
-// layout 0 (unit bits 8..11), then of layout 1 from unit bit 7 down; the
-// remaining slots take the lowest other bits in ascending order (so that in
-// the load layout the contiguous run continues across lanes where it can).
+// layout 0 (unit bits 8..11), then of layout 1 from unit bit 7 down, then
+// unit bit 3 (layout 2); the remaining slots take the lowest other bits in
+// ascending order (so that in the load layout the contiguous run continues
+// across lanes where it can).
 struct Plan
 {
     std::vector<unsigned> ebit; // tile element bit -> address bit
@@ -1023,9 +1108,10 @@ bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl
     std::sort(upper.begin(), upper.end());
     if (upper.size() > tile_upper_slots(prec) || n < tile_min_bits(prec))
         return false;
-    // targets: layout-0 register bits, then layout-1 register bits from the top
+    // targets: layout-0 register bits, then layout-1 register bits from the
+    // top, then (geometry 3) layout 2's register bit 3
     std::vector<int> order;
-    for (int k : {8, 9, 10, 11, 7, 6, 5, 4})
+    for (int k : {8, 9, 10, 11, 7, 6, 5, 4, 3})
         if (k >= seg)
             order.push_back(k);
     for (size_t u = 0; u < upper.size(); u++)
@@ -1493,7 +1579,7 @@ __global__ __launch_bounds__(kThreads) void batch_collect_kernel(const BCollect 
 unsigned tile_seg(int prec)
 {
     const int s = tune_get(prec == C64 ? "tile.seg.c64" : "tile.seg.c128");
-    return s >= 4 && s <= 6 ? unsigned(s) : 4u;
+    return s >= 3 && s <= 6 ? unsigned(s) : 4u;
 }
 
 unsigned tile_low_bits(int prec)
