@@ -26,9 +26,18 @@ Reported beside it (same JSON line):
   adjoint_c3    two-local ansatz, 28 qubits complex64, 448 parameters, 200
                 Z/ZZ terms: forward and adjoint backward times, grads/s, and
                 the gradient error and state infidelity against complex128
+  batch_c4      512 parameter sets (16..24 qubits, 2..10 layers, 50 terms,
+                complex64) in one batched engine replayed from a hipGraph
   cpu_baseline  the reference's CPU path (oracle/_ref when built, else the
                 oracle port) on a bounded prefix of the same circuit, timed on
                 this host
+
+roofline.traffic is the HBM bytes per launch of the dominant kernel as the
+two counter runs summarised in profiles/pmc_traffic_qft30_c128.txt measured
+them (FETCH_SIZE and WRITE_SIZE scaled by the in-place scale kernel).
+
+--section NAME runs one extra alone (after the tile-geometry autotuning the
+full run does first), for profiling it.
 """
 from __future__ import annotations
 
@@ -59,6 +68,23 @@ def family_stats(sim, families=("diag", "u3_tile", "fused", "u3_pair", "reset", 
     return out
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "pmc_traffic_qft30_c128.txt")
+
+
+def measured_traffic(family):
+    """HBM bytes per launch of a kernel family from the committed counter
+    summary (line "traffic <family> <bytes>"), or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic" and parts[1] == family:
+                    return int(parts[2])
+    except OSError:
+        return None
+    return None
+
+
 def roofline_of(stats):
     fam = max(stats, key=lambda k: stats[k]["ms"])
     t = stats[fam]
@@ -66,7 +92,7 @@ def roofline_of(stats):
     avg_ms = t["ms"] / t["launches"]
     achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9
     return fam, {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(fam),
                  "kernel": fam, "bytes_per_launch": int(per_launch_bytes), "avg_launch_ms": round(avg_ms, 4),
                  "launches": t["launches"]}
 
@@ -396,6 +422,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--kernels", choices=["c128", "c64"], default=None,
                     help="only launch each hot kernel --steps times on a 30-qubit state (profiling)")
+    ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128"], default=None,
+                    help="run one extra alone, after the tile-geometry autotuning (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine is single-GPU by design",
@@ -412,6 +440,23 @@ def main():
                           "qubits": NQ}), flush=True)
         return
     from zxh_adae_amd import autotune
+    if args.section:
+        prec = "c128" if args.section == "c5_c128" else "c64"
+        st = K.DeviceState(NQ, prec)
+        st.scale(0.0)
+        tuned = autotune.tune_tile(st)
+        st.free()
+        log(f"section {args.section} (tile geometry {tuned['best']})")
+        if args.section == "adjoint_c3":
+            out = adjoint_c3(W)
+        elif args.section == "batch_c4":
+            out = batch_c4(W)
+        elif args.section == "c5_c64":
+            out = c5_circuit(W, K, "c64", 33)
+        else:
+            out = c5_circuit(W, K, "c128", 32)
+        print(json.dumps({args.section: out, "autotune": tuned}), flush=True)
+        return
 
     # the resident 30-qubit state; the autotuner picks the diagonal-sweep variant on it
     sim = Z.ZXH(NQ, False, True, precision="c128", policy="fused")

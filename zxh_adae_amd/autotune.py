@@ -82,13 +82,18 @@ def layer_passes(n: int, seg: int, c64: bool):
     return chunks
 
 
+TILE_TOLERANCE = 0.03
+
+
 def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dict[str, object]:
     """Times a full-width layer per tile geometry on state: U3 on every qubit,
     split into the geometry's tile passes, each pass fused with a window of
     P/CP gates.  Fewer contiguous bits per tile leave room for more targets
     per pass (fewer passes) at shorter contiguous runs, so the choice is by
-    the layer's total time.  Sets the fastest geometry; the state is changed
-    by the passes."""
+    the layer's total time: the geometry with the longest runs whose layer
+    is within TILE_TOLERANCE of the fastest (longer runs keep passes with
+    fewer targets fast, and a smaller difference is within run-to-run
+    variation).  Sets that geometry; the state is changed by the passes."""
     c64 = state.prec == _lib.ZXH_C64
     knob = "tile.seg.c64" if c64 else "tile.seg.c128"
     n = state.nbits
@@ -114,7 +119,8 @@ def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dic
         K.set_timing(False)
         results[seg] = t["ms"] / repeats
         npasses[seg] = len(passes)
-    best = min(results, key=results.get)
+    fastest = min(results.values())
+    best = max(k for k, v in results.items() if v <= fastest * (1.0 + TILE_TOLERANCE))
     tune_set(knob, best)
     return {"knob": knob, "best": best, "previous": old,
             "layer_ms": {f"seg{k}": round(v, 4) for k, v in results.items()},
