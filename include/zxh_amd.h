@@ -286,7 +286,10 @@ int zxh_batch_set_info(zxh_batch *b, size_t set, unsigned *nbits, size_t *passes
  *                                    bits per tile pass (targets per pass: 12
  *                                    minus this above them)
  *   "tile.bufs.c128", "tile.bufs.c64": 2 or 3 register buffers per tile-pass
- *                                      workgroup (tiles in flight) */
+ *                                      workgroup (tiles in flight)
+ *   "block.variant.c128", "block.variant.c64": dense-block kernel variant,
+ *       bit 0 prefetch the next wave tile, bit 1 matrix re-read from LDS,
+ *       bit 2 cached (not nontemporal) loads and stores (0..7) */
 int zxh_tune_set(const char *name, int value);
 int zxh_tune_get(const char *name, int *value);
 /* HIP-event timing of the kernel-level calls (same families as zxh_get_timing,

@@ -60,6 +60,35 @@ def test_block_matches_dense_groups(gpu_engine, n, targets, prec):
     assert abs(np.linalg.norm(got) - np.linalg.norm(psi.astype(np.complex128))) < (1e-12 if prec == "c128" else 1e-5)
 
 
+@pytest.mark.parametrize("variant", range(8))
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_every_kernel_variant_matches_dense_groups(gpu_engine, prec, variant):
+    """Each variant of the block kernel (prefetch, matrix from LDS or
+    registers, cached or nontemporal access), on a state with several wave
+    tiles per wave so the prefetch path runs, and low targets."""
+    from zxh_adae_amd import autotune
+    from zxh_adae_amd.kernels import DeviceState
+    knob = f"block.variant.{prec}"
+    old = autotune.tune_get(knob)
+    rng = np.random.default_rng(variant + (100 if prec == "c64" else 0))
+    n, targets = 23, [0, 1, 7, 15, 22]
+    u = haar(rng)
+    dt = np.complex64 if prec == "c64" else np.complex128
+    psi = rand_state(rng, n, dt)
+    d = DeviceState(n, prec)
+    d.upload(psi)
+    try:
+        autotune.tune_set(knob, variant)
+        d.block5(targets, u)
+        d.block5(targets[::-1], u.conj().T)
+    finally:
+        autotune.tune_set(knob, old)
+    got = d.download().astype(np.complex128)
+    want = apply_block(apply_block(psi, n, targets, u), n, targets[::-1], u.conj().T)
+    tol = 1e-12 if prec == "c128" else 4e-6
+    assert np.max(np.abs(got - want)) < tol
+
+
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 def test_kronecker_block_equals_u3_pass(gpu_engine, prec):
     """u = U4 x U3 x U2 x U1 x U0 (U_b on target b) as one block equals the

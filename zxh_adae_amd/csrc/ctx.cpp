@@ -259,6 +259,8 @@ Knob g_knobs[] = {
     {"tile.seg.c64", 4, 3, 6},
     {"tile.bufs.c128", 2, 2, 3},
     {"tile.bufs.c64", 2, 2, 3},
+    {"block.variant.c128", 4, 0, 7},
+    {"block.variant.c64", 7, 0, 7},
 };
 std::mutex g_knob_mu;
 } // namespace

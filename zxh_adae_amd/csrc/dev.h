@@ -178,6 +178,8 @@ struct Ctx
 //   diag.nt.c128 / diag.nt.c64     nontemporal loads/stores in the flat sweep
 //   tile.seg.c128 / tile.seg.c64   contiguous unit-address bits of a tile (3..6)
 //   tile.bufs.c128 / tile.bufs.c64 register buffers per tile-pass workgroup (2, 3)
+//   block.variant.c128 / .c64      dense-block kernel variant (bit 0 prefetch,
+//                                  bit 1 matrix from LDS, bit 2 cached access)
 int tune_get(const char *name);
 int tune_set(const char *name, int value);
 

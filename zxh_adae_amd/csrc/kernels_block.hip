@@ -20,7 +20,18 @@
 // from HBM into the MFMA operand layout: lane (n, h) holds the amplitudes of
 // group n whose combination index t is congruent to h mod 64/G; the state is
 // not staged through LDS.  The group bits are the lowest non-target address
-// bits, so a load instruction reads runs of G consecutive amplitudes (256 B).
+// bits, so a load instruction reads runs of G consecutive amplitudes (256 B)
+// when no target is among the lowest bits.  When targets are low bits the
+// runs of one instruction shrink to 8 or 16 bytes, and the wave's 16 (8) load
+// instructions together cover whole lines: ordinary cached loads and stores
+// then let L2 merge them, where nontemporal ones fetch each line several
+// times (a c64 block on qubits 0, 1, 7, 15, 22 of 33: 84 ms nontemporal,
+// 33 ms cached).  Variants (knob block.variant.<prec>): prefetch of the next
+// wave tile, the matrix re-read from LDS per tile (fewer registers, more
+// waves) or held in registers, cached or nontemporal access; the defaults
+// were chosen on the 32/33-qubit random circuits: complex64 prefetch + LDS +
+// cached (29.6 ms per 33-qubit sweep), complex128 registers + cached (28.2 ms
+// per 32-qubit sweep; prefetching with FP64 matrix cores busy was slower).
 // M is pre-arranged on the host in the A operand's lane order and read from
 // LDS, one word per lane per step (conflict-free).  Each wave owns disjoint
 // groups: it reads them all, multiplies, and writes them back.
@@ -73,7 +84,61 @@ __host__ __device__ __forceinline__ uint64_t deposit_bits(uint64_t w, const uint
     return w;
 }
 
+// The matrix is re-read from LDS at every tile: an opaque zero added to the
+// LDS index keeps the compiler from hoisting the 64 loop-invariant operand
+// words per lane into registers, which would cost 64 (complex64) or 128
+// (complex128) VGPRs and halve the waves per SIMD.
+template <bool kNT, typename V>
+__device__ __forceinline__ V ld(const V *p)
+{
+    if constexpr (kNT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+template <bool kNT, typename V>
+__device__ __forceinline__ void st(V v, V *p)
+{
+    if constexpr (kNT)
+        __builtin_nontemporal_store(v, p);
+    else
+        *p = v;
+}
+
+template <bool kLds>
+__device__ __forceinline__ int opaque_zero()
+{
+    int z = 0;
+    if constexpr (kLds)
+        asm volatile("" : "+v"(z));
+    return z;
+}
+
+// One wave tile of complex64: Y = M X on the 32 groups in x, stored at base.
+template <bool kLds, bool kNT>
+__device__ __forceinline__ void block_c64_tile(const fvec2 (&x)[16], const float *fm, int lane, fvec2 *state,
+                                               uint64_t at, ZXA_CONST BTab *T)
+{
+    const int z = opaque_zero<kLds>();
+    fx16 c0 = {}, c1 = {};
+#pragma unroll
+    for (int s = 0; s < 32; s++)
+    {
+        const float b = s < 16 ? x[s].x : x[s - 16].y;
+        c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(fm[(s * 2 + 0) * 64 + lane + z], b, c0, 0, 0, 0);
+        c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(fm[(s * 2 + 1) * 64 + lane + z], b, c1, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++)
+        st<kNT>(fvec2{c0[r], c1[r]}, state + (at | T->tout[r]));
+}
+
 // complex64: frag[(s * 2 + blk) * 64 + lane] = M[32 blk + lane % 32][2 s + lane / 32]
+// Each wave streams through its wave tiles with two register buffers: the
+// loads of the next tile are issued before the current tile's products, so
+// HBM stays busy while the matrix cores run (past the last tile the prefetch
+// re-reads the current one).
+template <bool kPre, bool kLds, bool kNT>
 __global__ __launch_bounds__(kThreads) void block_kernel_c64(fvec2 *__restrict__ state, const BArgs A,
                                                              const BTab *__restrict__ tabp,
                                                              const float *__restrict__ frag,
@@ -87,31 +152,78 @@ __global__ __launch_bounds__(kThreads) void block_kernel_c64(fvec2 *__restrict__
     const uint64_t gin = lane_off[lane];
     const uint64_t gout = lane_off[64 + lane];
     ZXA_CONST BTab *T = (ZXA_CONST BTab *)tabp;
-    const uint64_t wave = uint64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    uint64_t w = uint64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
     const uint64_t wstride = uint64_t(gridDim.x) * kWaves;
-    uint64_t base = deposit_bits(wave, A.sorted, A.nbits);
-    for (uint64_t w = wave; w < A.nwtiles; w += wstride)
+    if (w >= A.nwtiles)
+        return;
+    uint64_t b0 = deposit_bits(w, A.sorted, A.nbits);
+    fvec2 xa[16], xb[16];
+#pragma unroll
+    for (int s = 0; s < 16; s++)
+        xa[s] = ld<kNT>(state + (b0 | gin | T->tin[s]));
+    if constexpr (!kPre)
     {
-        fvec2 x[16];
+        for (;;)
+        {
+            block_c64_tile<kLds, kNT>(xa, fm, lane, state, b0 | gout, T);
+            w += wstride;
+            if (w >= A.nwtiles)
+                return;
+            b0 = ((b0 | A.tmask) + A.gstep) & ~A.tmask;
+#pragma unroll
+            for (int s = 0; s < 16; s++)
+                xa[s] = ld<kNT>(state + (b0 | gin | T->tin[s]));
+        }
+    }
+    for (;;)
+    {
+        const bool m1 = w + wstride < A.nwtiles;
+        const uint64_t b1 = m1 ? ((b0 | A.tmask) + A.gstep) & ~A.tmask : b0;
 #pragma unroll
         for (int s = 0; s < 16; s++)
-            x[s] = __builtin_nontemporal_load(state + (base | gin | T->tin[s]));
-        fx16 c0 = {}, c1 = {};
+            xb[s] = ld<kNT>(state + (b1 | gin | T->tin[s]));
+        block_c64_tile<kLds, kNT>(xa, fm, lane, state, b0 | gout, T);
+        if (!m1)
+            break;
+        w += wstride;
+        const bool m2 = w + wstride < A.nwtiles;
+        const uint64_t b2 = m2 ? ((b1 | A.tmask) + A.gstep) & ~A.tmask : b1;
 #pragma unroll
-        for (int s = 0; s < 32; s++)
-        {
-            const float b = s < 16 ? x[s].x : x[s - 16].y;
-            c0 = __builtin_amdgcn_mfma_f32_32x32x2f32(fm[(s * 2 + 0) * 64 + lane], b, c0, 0, 0, 0);
-            c1 = __builtin_amdgcn_mfma_f32_32x32x2f32(fm[(s * 2 + 1) * 64 + lane], b, c1, 0, 0, 0);
-        }
+        for (int s = 0; s < 16; s++)
+            xa[s] = ld<kNT>(state + (b2 | gin | T->tin[s]));
+        block_c64_tile<kLds, kNT>(xb, fm, lane, state, b1 | gout, T);
+        if (!m2)
+            break;
+        w += wstride;
+        b0 = b2;
+    }
+}
+
+// One wave tile of complex128.
+template <bool kLds, bool kNT>
+__device__ __forceinline__ void block_c128_tile(const dvec2 (&x)[8], const double *fm, int lane, dvec2 *state,
+                                                uint64_t at, ZXA_CONST BTab *T)
+{
+    const int z = opaque_zero<kLds>();
+    dx4 c[4] = {};
 #pragma unroll
-        for (int r = 0; r < 16; r++)
-            __builtin_nontemporal_store(fvec2{c0[r], c1[r]}, state + (base | gout | T->tout[r]));
-        base = ((base | A.tmask) + A.gstep) & ~A.tmask;
+    for (int s = 0; s < 16; s++)
+    {
+        const double b = s < 8 ? x[s].x : x[s - 8].y;
+#pragma unroll
+        for (int blk = 0; blk < 4; blk++)
+            c[blk] = __builtin_amdgcn_mfma_f64_16x16x4f64(fm[(s * 4 + blk) * 64 + lane + z], b, c[blk], 0, 0, 0);
+    }
+#pragma unroll
+    for (int o = 0; o < 8; o++)
+    {
+        const int bp = o >> 2, r = o & 3;
+        st<kNT>(dvec2{c[bp][r], c[bp + 2][r]}, state + (at | T->tout[o]));
     }
 }
 
 // complex128: frag[(s * 4 + blk) * 64 + lane] = M[16 blk + lane % 16][4 s + lane / 16]
+template <bool kPre, bool kLds, bool kNT>
 __global__ __launch_bounds__(kThreads) void block_kernel_c128(dvec2 *__restrict__ state, const BArgs A,
                                                               const BTab *__restrict__ tabp,
                                                               const double *__restrict__ frag,
@@ -125,31 +237,50 @@ __global__ __launch_bounds__(kThreads) void block_kernel_c128(dvec2 *__restrict_
     const uint64_t gin = lane_off[lane];
     const uint64_t gout = lane_off[64 + lane];
     ZXA_CONST BTab *T = (ZXA_CONST BTab *)tabp;
-    const uint64_t wave = uint64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
+    uint64_t w = uint64_t(blockIdx.x) * kWaves + (threadIdx.x >> 6);
     const uint64_t wstride = uint64_t(gridDim.x) * kWaves;
-    uint64_t base = deposit_bits(wave, A.sorted, A.nbits);
-    for (uint64_t w = wave; w < A.nwtiles; w += wstride)
+    if (w >= A.nwtiles)
+        return;
+    uint64_t b0 = deposit_bits(w, A.sorted, A.nbits);
+    dvec2 xa[8], xb[8];
+#pragma unroll
+    for (int s = 0; s < 8; s++)
+        xa[s] = ld<kNT>(state + (b0 | gin | T->tin[s]));
+    if constexpr (!kPre)
     {
-        dvec2 x[8];
+        for (;;)
+        {
+            block_c128_tile<kLds, kNT>(xa, fm, lane, state, b0 | gout, T);
+            w += wstride;
+            if (w >= A.nwtiles)
+                return;
+            b0 = ((b0 | A.tmask) + A.gstep) & ~A.tmask;
+#pragma unroll
+            for (int s = 0; s < 8; s++)
+                xa[s] = ld<kNT>(state + (b0 | gin | T->tin[s]));
+        }
+    }
+    for (;;)
+    {
+        const bool m1 = w + wstride < A.nwtiles;
+        const uint64_t b1 = m1 ? ((b0 | A.tmask) + A.gstep) & ~A.tmask : b0;
 #pragma unroll
         for (int s = 0; s < 8; s++)
-            x[s] = __builtin_nontemporal_load(state + (base | gin | T->tin[s]));
-        dx4 c[4] = {};
+            xb[s] = ld<kNT>(state + (b1 | gin | T->tin[s]));
+        block_c128_tile<kLds, kNT>(xa, fm, lane, state, b0 | gout, T);
+        if (!m1)
+            break;
+        w += wstride;
+        const bool m2 = w + wstride < A.nwtiles;
+        const uint64_t b2 = m2 ? ((b1 | A.tmask) + A.gstep) & ~A.tmask : b1;
 #pragma unroll
-        for (int s = 0; s < 16; s++)
-        {
-            const double b = s < 8 ? x[s].x : x[s - 8].y;
-#pragma unroll
-            for (int blk = 0; blk < 4; blk++)
-                c[blk] = __builtin_amdgcn_mfma_f64_16x16x4f64(fm[(s * 4 + blk) * 64 + lane], b, c[blk], 0, 0, 0);
-        }
-#pragma unroll
-        for (int o = 0; o < 8; o++)
-        {
-            const int bp = o >> 2, r = o & 3;
-            __builtin_nontemporal_store(dvec2{c[bp][r], c[bp + 2][r]}, state + (base | gout | T->tout[o]));
-        }
-        base = ((base | A.tmask) + A.gstep) & ~A.tmask;
+        for (int s = 0; s < 8; s++)
+            xa[s] = ld<kNT>(state + (b2 | gin | T->tin[s]));
+        block_c128_tile<kLds, kNT>(xb, fm, lane, state, b1 | gout, T);
+        if (!m2)
+            break;
+        w += wstride;
+        b0 = b2;
     }
 }
 
@@ -282,21 +413,47 @@ int block5_apply(Ctx &ctx, void *state, int prec, unsigned n, const unsigned *ta
         ZXA_TRY(ctx.ring.stage(f64.data(), f64.size() * sizeof(double), ctx.stream, &dF));
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
+    // kernel variant (knob block.variant.<prec>): bit 0 prefetches the next
+    // wave tile, bit 1 re-reads the matrix from LDS at every tile, bit 2
+    // uses ordinary (cached) loads and stores instead of nontemporal ones
+    const int var = tune_get(c64 ? "block.variant.c64" : "block.variant.c128");
+    auto launch = [&](auto kernel, auto *st, const auto *frag) {
+        const int grid = grid_for_waves(ctx, kernel, A.nwtiles);
+        A.gstep = deposit_bits(uint64_t(grid) * kWaves, A.sorted, A.nbits);
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, st, A, static_cast<const BTab *>(dT),
+                           frag, static_cast<const uint64_t *>(dL));
+    };
     if (c64)
     {
-        const int grid = grid_for_waves(ctx, block_kernel_c64, A.nwtiles);
-        A.gstep = deposit_bits(uint64_t(grid) * kWaves, A.sorted, A.nbits);
-        hipLaunchKernelGGL(block_kernel_c64, dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<fvec2 *>(state),
-                           A, static_cast<const BTab *>(dT), static_cast<const float *>(dF),
-                           static_cast<const uint64_t *>(dL));
+        fvec2 *sp = static_cast<fvec2 *>(state);
+        const float *fr = static_cast<const float *>(dF);
+        switch (var & 7)
+        {
+        case 0: launch(block_kernel_c64<false, false, true>, sp, fr); break;
+        case 1: launch(block_kernel_c64<true, false, true>, sp, fr); break;
+        case 2: launch(block_kernel_c64<false, true, true>, sp, fr); break;
+        case 3: launch(block_kernel_c64<true, true, true>, sp, fr); break;
+        case 4: launch(block_kernel_c64<false, false, false>, sp, fr); break;
+        case 5: launch(block_kernel_c64<true, false, false>, sp, fr); break;
+        case 6: launch(block_kernel_c64<false, true, false>, sp, fr); break;
+        default: launch(block_kernel_c64<true, true, false>, sp, fr); break;
+        }
     }
     else
     {
-        const int grid = grid_for_waves(ctx, block_kernel_c128, A.nwtiles);
-        A.gstep = deposit_bits(uint64_t(grid) * kWaves, A.sorted, A.nbits);
-        hipLaunchKernelGGL(block_kernel_c128, dim3(grid), dim3(kThreads), 0, ctx.stream,
-                           static_cast<dvec2 *>(state), A, static_cast<const BTab *>(dT),
-                           static_cast<const double *>(dF), static_cast<const uint64_t *>(dL));
+        dvec2 *sp = static_cast<dvec2 *>(state);
+        const double *fr = static_cast<const double *>(dF);
+        switch (var & 7)
+        {
+        case 0: launch(block_kernel_c128<false, false, true>, sp, fr); break;
+        case 1: launch(block_kernel_c128<true, false, true>, sp, fr); break;
+        case 2: launch(block_kernel_c128<false, true, true>, sp, fr); break;
+        case 3: launch(block_kernel_c128<true, true, true>, sp, fr); break;
+        case 4: launch(block_kernel_c128<false, false, false>, sp, fr); break;
+        case 5: launch(block_kernel_c128<true, false, false>, sp, fr); break;
+        case 6: launch(block_kernel_c128<false, true, false>, sp, fr); break;
+        default: launch(block_kernel_c128<true, true, false>, sp, fr); break;
+        }
     }
     ZXA_HIP(hipGetLastError());
     ZXA_TRY(ctx.toc(F_BLOCK, bytes));
