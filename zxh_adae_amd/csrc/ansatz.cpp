@@ -73,12 +73,6 @@ int Ansatz::init(const AnsatzOp *ops, size_t nops, bool device)
     return alloc();
 }
 
-int Ansatz::refresh_schedule()
-{
-    if (tile_seg(prec_) != seg_)
-        return build_schedule();
-    return OK;
-}
 
 int Ansatz::alloc()
 {
@@ -110,12 +104,33 @@ int Ansatz::set_observable(const int32_t *qi, const int32_t *qj, const double *c
     return OK;
 }
 
-// The schedule depends only on the structure (and the tile geometry knob).
+// The schedule depends only on the structure: the tile geometry with the
+// fewest passes, ties going to the longest contiguous runs.
 int Ansatz::build_schedule()
 {
+    std::vector<Pass> best, p;
+    unsigned bseg = 0;
+    for (unsigned seg = 6; seg >= 3; seg--)
+    {
+        schedule_for(seg, p);
+        if (bseg == 0 || p.size() < best.size())
+        {
+            best.swap(p);
+            bseg = seg;
+        }
+    }
+    passes_.swap(best);
+    seg_ = bseg;
+    tile_passes_ = passes_.size();
+    return OK;
+}
+
+// Passes of the circuit under tile geometry seg.
+void Ansatz::schedule_for(unsigned seg, std::vector<Pass> &passes_) const
+{
     passes_.clear();
-    const unsigned low = tile_low_bits(prec_);
-    const unsigned slots = tile_upper_slots(prec_);
+    const unsigned low = tile_low_bits(prec_, int(seg));
+    const unsigned slots = tile_upper_slots(prec_, int(seg));
     std::vector<Target> pending(n_);
     std::vector<int> nrot(n_, 0);
     std::vector<size_t> window;
@@ -176,9 +191,6 @@ int Ansatz::build_schedule()
         }
     }
     close();
-    tile_passes_ = passes_.size();
-    seg_ = tile_seg(prec_);
-    return OK;
 }
 
 Ansatz::Mat Ansatz::op_matrix(size_t i, const double *params) const
@@ -304,7 +316,6 @@ void Ansatz::pass_adjoint(size_t pi, const double *params, std::vector<AdjTarget
 
 int Ansatz::run_forward(const double *params)
 {
-    ZXA_TRY(refresh_schedule());
     ZXA_TRY(state_reset(ctx_, psi_, prec_, size_t(1) << n_));
     std::vector<U3Gate> gs;
     std::vector<DiagGate> w;
@@ -312,7 +323,7 @@ int Ansatz::run_forward(const double *params)
     {
         pass_gates(p, params, gs);
         pass_window(p, params, false, w);
-        ZXA_TRY(fused_pass_apply(ctx_, psi_, prec_, n_, gs.data(), gs.size(), w.data(), w.size()));
+        ZXA_TRY(fused_pass_apply(ctx_, psi_, prec_, n_, gs.data(), gs.size(), w.data(), w.size(), int(seg_)));
     }
     return OK;
 }
@@ -363,7 +374,7 @@ int Ansatz::backward(const double *params, double *grads, double *energy)
         if (jj > 0)
             pass_window(jj - 1, params, true, w);
         ZXA_TRY(adjoint_pass_apply(ctx_, psi_, lam_, prec_, n_, ts.data(), ts.size(), w.data(), w.size(), dgrads_,
-                                   nparams_));
+                                   nparams_, int(seg_)));
     }
     if (nparams_ > 0)
         ZXA_HIP(hipMemcpyAsync(grads, dgrads_, nparams_ * sizeof(double), hipMemcpyDeviceToHost, ctx_.stream));

@@ -9,7 +9,8 @@
 // closes when a single-qubit operation meets a qubit of the pending window,
 // or when a qubit already carries two parameterized rotations.  A pass whose
 // targets exceed one tile's slots splits into several tile passes; the
-// window rides on the last.
+// window rides on the last.  The tile geometry is the circuit's own: the one
+// with the fewest tile passes, ties going to the longest contiguous runs.
 //
 // forward(params): |0> -> psi through the tile passes, then <psi|H|psi> for
 // the diagonal Z / ZZ observable (FP64 compensated).
@@ -85,8 +86,11 @@ class Ansatz
     {
         return nparams_;
     }
-    // Rebuilds the schedule if the tile geometry knob changed since.
-    int refresh_schedule();
+    // Tile geometry of the schedule (3..6 contiguous unit-address bits).
+    unsigned geometry() const
+    {
+        return seg_;
+    }
     // Pass p of the forward schedule: its U3 gates (fused matrices at params)
     // and its diagonal window (angles negated when negate).
     void pass_gates(size_t p, const double *params, std::vector<U3Gate> &gates) const;
@@ -124,6 +128,7 @@ class Ansatz
     };
     Mat op_matrix(size_t i, const double *params) const;
     int build_schedule();
+    void schedule_for(unsigned seg, std::vector<Pass> &passes) const;
     int run_forward(const double *params);
     int alloc();
 

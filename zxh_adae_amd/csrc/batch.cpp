@@ -82,7 +82,6 @@ int AnsatzBatch::finalize()
     std::vector<size_t> offs;
     for (Set &s : sets_)
     {
-        ZXA_TRY(s.a->refresh_schedule());
         offs.push_back(off);
         off += align256((size_t(1) << s.a->nbits()) * elem_bytes(prec_));
     }
@@ -122,6 +121,7 @@ int AnsatzBatch::finalize()
             BatchPassSpec sp;
             sp.psi = s.psi;
             sp.n = s.a->nbits();
+            sp.geometry = int(s.a->geometry());
             sp.targets = targets_of(gs);
             sp.window = w;
             specs.push_back(sp);
@@ -150,6 +150,7 @@ int AnsatzBatch::finalize()
             sp.psi = s.psi;
             sp.lam = s.lam;
             sp.n = s.a->nbits();
+            sp.geometry = int(s.a->geometry());
             sp.targets = targets_of(gs);
             if (j > 0)
                 s.a->pass_window(j - 1, zero.data(), true, sp.window);
@@ -179,6 +180,7 @@ int AnsatzBatch::finalize()
             sp.psi = s.psi;
             sp.lam = s.lam;
             sp.n = s.a->nbits();
+            sp.geometry = int(s.a->geometry());
             sp.grads = s.grads;
             sp.nparams = s.a->nparams();
             specs.push_back(sp);

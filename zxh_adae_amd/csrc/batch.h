@@ -9,8 +9,8 @@
 // observable launch gives every set's energy and lam = H psi, then one batched
 // adjoint launch per backward step.  The whole sequence is captured once in a
 // hipGraph and replayed; per run only the matrix tables (a pinned mirror) are
-// uploaded before the replay.  The sets' tile passes use the tile geometry
-// chosen by the autotuner (knob tile.seg.<prec>) when the batch is finalised.
+// uploaded before the replay.  Each set's tile passes use that set's own tile
+// geometry (the ansatz schedule's).
 #pragma once
 
 #include "ansatz.h"

@@ -210,9 +210,10 @@ struct U3Gate
 // 16-byte-unit address bits): a tile pass reaches every target below tile_low_bits(prec)
 // and up to tile_upper_slots(prec) more; it needs a state of at least
 // tile_min_bits(prec) bits (12 for complex128, 13 for complex64).
-unsigned tile_seg(int prec);
-unsigned tile_low_bits(int prec);
-unsigned tile_upper_slots(int prec);
+// geometry: 3..6 for that geometry, anything else for the knob's.
+unsigned tile_seg(int prec, int geometry = -1);
+unsigned tile_low_bits(int prec, int geometry = -1);
+unsigned tile_upper_slots(int prec, int geometry = -1);
 unsigned tile_min_bits(int prec);
 // Applies commuting gates on distinct targets (a fused U3 cluster).
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates);
@@ -221,7 +222,7 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gate
 // above), then a diagonal window whose selectors have at most one alpha bit,
 // in one sweep.
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
-                     const DiagGate *diag, size_t ndiag);
+                     const DiagGate *diag, size_t ndiag, int geometry = -1);
 // One target of an adjoint pass: the uncompute matrix m (= U^dagger of the
 // forward pass's fused single-qubit gate on this qubit, (re, im) x 4), up to
 // two transition operators a[k] (2x2 traceless Hermitian; zero when unused)
@@ -239,7 +240,8 @@ struct AdjTarget
 // then the diagonal window (given with the angles to apply) on both.
 // dgrads: device array of parameter gradients, accumulated in stream order.
 int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned nbits, const AdjTarget *targets,
-                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads, size_t nparams);
+                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads, size_t nparams,
+                       int geometry = -1);
 // Dense 5-qubit block: y[t] = sum_t' u[t][t'] x[t'] over every group of 32
 // amplitudes differing in the target bits (bit b of t = targets[b]); u is
 // 32x32 complex, row-major (re, im) pairs.  One sweep, matrix cores.
@@ -258,6 +260,7 @@ struct BatchPassSpec
     std::vector<DiagGate> window;
     double *grads = nullptr; // adjoint: the state's gradient array (device)
     size_t nparams = 0;      // adjoint: its length
+    int geometry = -1;       // tile geometry of the state's passes (-1: the knob's)
 };
 struct BatchLaunch
 {

@@ -1089,11 +1089,11 @@ struct Plan
     std::vector<int> slot;      // per gate: unit bit, or kIntra
 };
 
-bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl)
+bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl, int geometry = -1)
 {
     const bool c64 = prec == C64;
-    const int seg = int(tile_seg(prec));
-    const unsigned low = tile_low_bits(prec);
+    const int seg = int(tile_seg(prec, geometry));
+    const unsigned low = tile_low_bits(prec, geometry);
     const unsigned shift = c64 ? 1u : 0u;
     std::vector<int> unit_addr(kUnitBits, -1); // unit bit -> address bit
     for (int k = 0; k < seg; k++)
@@ -1106,7 +1106,7 @@ bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl
         if (s.target >= low)
             upper.push_back(s.target);
     std::sort(upper.begin(), upper.end());
-    if (upper.size() > tile_upper_slots(prec) || n < tile_min_bits(prec))
+    if (upper.size() > tile_upper_slots(prec, geometry) || n < tile_min_bits(prec))
         return false;
     // targets: layout-0 register bits, then layout-1 register bits from the
     // top, then (geometry 3) layout 2's register bit 3
@@ -1291,11 +1291,11 @@ void fill_pass(int prec, unsigned n, const Plan &pl, TPass &P, TTab &G, bool use
 
 // One tile pass over the given targets, then the diagonal window (if any).
 int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
-                  const DiagGate *diag = nullptr, size_t ndiag = 0)
+                  const DiagGate *diag = nullptr, size_t ndiag = 0, int geometry = -1)
 {
     const bool c64 = prec == C64;
     Plan pl;
-    if (!plan_tile(prec, n, gs, pl))
+    if (!plan_tile(prec, n, gs, pl, geometry))
     {
         set_error("tile pass: state narrower than a tile or too many targets");
         return EINVAL_;
@@ -1576,25 +1576,27 @@ __global__ __launch_bounds__(kThreads) void batch_collect_kernel(const BCollect 
 
 } // namespace
 
-unsigned tile_seg(int prec)
+unsigned tile_seg(int prec, int geometry)
 {
+    if (geometry >= 3 && geometry <= 6)
+        return unsigned(geometry);
     const int s = tune_get(prec == C64 ? "tile.seg.c64" : "tile.seg.c128");
     return s >= 3 && s <= 6 ? unsigned(s) : 4u;
 }
 
-unsigned tile_low_bits(int prec)
+unsigned tile_low_bits(int prec, int geometry)
 {
-    return tile_seg(prec) + (prec == C64 ? 1u : 0u);
+    return tile_seg(prec, geometry) + (prec == C64 ? 1u : 0u);
 }
 
-unsigned tile_upper_slots(int prec)
+unsigned tile_upper_slots(int prec, int geometry)
 {
-    return unsigned(kUnitBits) - tile_seg(prec);
+    return unsigned(kUnitBits) - tile_seg(prec, geometry);
 }
 
 unsigned tile_min_bits(int prec)
 {
-    return tile_low_bits(prec) + tile_upper_slots(prec);
+    return unsigned(kUnitBits) + (prec == C64 ? 1u : 0u);
 }
 
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates)
@@ -1645,7 +1647,7 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
 }
 
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates,
-                     const DiagGate *diag, size_t ndiag)
+                     const DiagGate *diag, size_t ndiag, int geometry)
 {
     if (n < tile_min_bits(prec) || n > 40)
     {
@@ -1657,7 +1659,7 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
         set_error("fused pass: diagonal window is not a phase polynomial");
         return EINVAL_;
     }
-    const unsigned low = tile_low_bits(prec);
+    const unsigned low = tile_low_bits(prec, geometry);
     std::vector<SingleGate> gs;
     int high = 0;
     for (size_t i = 0; i < ngates; i++)
@@ -1680,14 +1682,14 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
             high++;
         gs.push_back({t, &g});
     }
-    if (unsigned(high) > tile_upper_slots(prec))
+    if (unsigned(high) > tile_upper_slots(prec, geometry))
     {
         set_error("fused pass: more targets above the tile's low bits than the tile has slots");
         return EINVAL_;
     }
     if (gs.empty())
         return diag_apply(ctx, state, prec, n, diag, ndiag);
-    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag);
+    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag, geometry);
 }
 
 } // namespace zxa
@@ -1696,7 +1698,8 @@ namespace zxa
 {
 
 int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, const AdjTarget *targets,
-                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads, size_t nparams)
+                       size_t ntargets, const DiagGate *diag, size_t ndiag, double *dgrads, size_t nparams,
+                       int geometry)
 {
     const bool c64 = prec == C64;
     if (n < tile_min_bits(prec) || n > 40)
@@ -1731,7 +1734,7 @@ int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, con
         gs.push_back({q, &ug[i]});
     }
     Plan pl;
-    if (!plan_tile(prec, n, gs, pl))
+    if (!plan_tile(prec, n, gs, pl, geometry))
     {
         set_error("adjoint pass: more targets than the tile has slots");
         return EINVAL_;
@@ -1853,7 +1856,7 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
             ug[k].packed_sel = ug[k].delta = uint64_t(1) << sp.targets[k];
             gs.push_back({sp.targets[k], &ug[k]});
         }
-        if (!plan_tile(prec, sp.n, gs, plans[i]))
+        if (!plan_tile(prec, sp.n, gs, plans[i], sp.geometry))
         {
             set_error("batch: pass does not fit one tile");
             return EINVAL_;
