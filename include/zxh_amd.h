@@ -189,6 +189,11 @@ int zxh_k_sample_thresholds(const void *state, int precision, size_t m, size_t s
                             size_t shots, uint64_t *out_real);
 /* sum of |amplitude|^2 in the reference's pairwise order (f64) */
 int zxh_k_prob_total(const void *state, int precision, size_t m, double *out);
+/* out (host, count amplitudes of the state's precision) = the amplitudes at
+ * physical addresses indices[0..count): a partial get_state (zxh.cpp:579-636
+ * reads all 2^m) for probes of states too large to copy whole */
+int zxh_k_gather(const void *state, int precision, size_t local_bits, const uint64_t *indices, size_t count,
+                 void *out);
 /* device copy of bytes (a multiple of 16); the bandwidth and HBM-counter
  * calibration reference (no reference counterpart) */
 int zxh_k_copy(void *dst, const void *src, size_t bytes);

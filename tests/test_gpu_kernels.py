@@ -316,3 +316,17 @@ def test_fused_pass_rejects_multibit_and_overfull(gpu_engine):
         d.fused_pass([(pack(3, False), 1, rand_u(rng))], [], [], [])
     with pytest.raises(ZXHError):
         d.fused_pass([(pack(1 << q, False), 1 << q, rand_u(rng)) for q in range(4, 13)], [], [], [])
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_gather_reads_the_addressed_amplitudes(gpu_engine, prec):
+    from zxh_adae_amd import ZXHError
+    rng = np.random.default_rng(44)
+    dt = np.complex64 if prec == "c64" else np.complex128
+    psi = rand_state(rng, 12, dt)
+    d = dev(gpu_engine, psi, prec)
+    idx = np.concatenate([[0, 4095], rng.integers(0, 4096, 300)]).astype(np.uint64)
+    assert np.array_equal(d.gather(idx), psi[idx.astype(np.int64)])
+    assert d.gather(np.zeros(0, dtype=np.uint64)).size == 0
+    with pytest.raises(ZXHError):
+        d.gather(np.array([4096], dtype=np.uint64))

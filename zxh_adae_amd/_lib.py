@@ -80,6 +80,7 @@ _SIGS = [
     ("zxh_k_sample", C.c_int, [_vp, C.c_int, _sz, _sz, _u64, _sz, _u64p]),
     ("zxh_k_sample_thresholds", C.c_int, [_vp, C.c_int, _sz, _sz, _dp, _sz, _u64p]),
     ("zxh_k_prob_total", C.c_int, [_vp, C.c_int, _sz, _dp]),
+    ("zxh_k_gather", C.c_int, [_vp, C.c_int, _sz, _u64p, _sz, _vp]),
     ("zxh_k_copy", C.c_int, [_vp, _vp, _sz]),
     ("zxh_k_scale", C.c_int, [_vp, C.c_int, _sz, C.c_double, C.c_double]),
     ("zxh_k_expect_z", C.c_int, [_vp, C.c_int, _sz, _i32p, _i32p, _dp, _sz, _dp]),

@@ -494,6 +494,18 @@ int zxh_k_prob_total(const void *state, int precision, size_t m, double *out)
     return zxa::prob_total(*c, state, precision, unsigned(m), out);
 }
 
+int zxh_k_gather(const void *state, int precision, size_t local_bits, const uint64_t *indices, size_t count,
+                 void *out)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    if (count && (!indices || !out))
+        return fail(ZXH_EINVAL, "null pointer");
+    WITH_CTX(c);
+    return zxa::gather(*c, state, precision, unsigned(local_bits), indices, count, out);
+}
+
 int zxh_k_copy(void *dst, const void *src, size_t bytes)
 {
     if (!dst || !src)

@@ -311,6 +311,8 @@ int obs_batch_launch(Ctx &ctx, ObsBatch &b);
 // each state in the device array dstates.
 int zero_fill(Ctx &ctx, void *p, size_t bytes);
 int set_ones(Ctx &ctx, const uint64_t *dstates, int count, int prec);
+// out (host) = the amplitudes at idx[0..count) of a 2^n state (synchronous).
+int gather(Ctx &ctx, const void *state, int prec, unsigned n, const uint64_t *idx, size_t count, void *out);
 // One general pair sweep (any delta / selector).
 int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
 

@@ -110,6 +110,21 @@ __global__ void set_ones_kernel(const uint64_t *__restrict__ states, int count)
         reinterpret_cast<double2 *>(states[i])[0] = make_double2(1.0, 0.0);
 }
 
+// out[i] = amplitude idx[i] (indices checked on the host).
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void gather_kernel(const void *__restrict__ state,
+                                                          const uint64_t *__restrict__ idx, uint64_t count,
+                                                          void *__restrict__ out)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= count)
+        return;
+    if constexpr (kC64)
+        static_cast<float2 *>(out)[i] = static_cast<const float2 *>(state)[idx[i]];
+    else
+        static_cast<double2 *>(out)[i] = static_cast<const double2 *>(state)[idx[i]];
+}
+
 template <typename K>
 unsigned stream_grid(Ctx &ctx, K kernel, uint64_t n16)
 {
@@ -169,6 +184,45 @@ int zero_fill(Ctx &ctx, void *p, size_t bytes)
     const unsigned grid = unsigned(std::max<uint64_t>(1, std::min<uint64_t>((n16 + kThreads - 1) / kThreads, 1u << 16)));
     hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<float4 *>(p), n16);
     return hip_check(hipGetLastError(), "zero_kernel launch");
+}
+
+int gather(Ctx &ctx, const void *state, int prec, unsigned n, const uint64_t *idx, size_t count, void *out)
+{
+    for (size_t i = 0; i < count; i++)
+        if (idx[i] >> n)
+        {
+            set_error("gather: index outside the state");
+            return EINVAL_;
+        }
+    if (count == 0)
+        return OK;
+    const size_t esz = prec == C64 ? 8 : 16;
+    void *didx = nullptr, *dout = nullptr;
+    hipError_t e = hipMalloc(&didx, count * sizeof(uint64_t));
+    if (e == hipSuccess)
+        e = hipMalloc(&dout, count * esz);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(didx, idx, count * sizeof(uint64_t), hipMemcpyHostToDevice, ctx.stream);
+    if (e == hipSuccess)
+    {
+        const unsigned grid = unsigned((count + kThreads - 1) / kThreads);
+        if (prec == C64)
+            hipLaunchKernelGGL(gather_kernel<true>, dim3(grid), dim3(kThreads), 0, ctx.stream, state,
+                               static_cast<const uint64_t *>(didx), uint64_t(count), dout);
+        else
+            hipLaunchKernelGGL(gather_kernel<false>, dim3(grid), dim3(kThreads), 0, ctx.stream, state,
+                               static_cast<const uint64_t *>(didx), uint64_t(count), dout);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, dout, count * esz, hipMemcpyDeviceToHost, ctx.stream);
+    if (e == hipSuccess)
+        e = hipStreamSynchronize(ctx.stream);
+    if (didx)
+        (void)hipFree(didx);
+    if (dout)
+        (void)hipFree(dout);
+    return hip_check(e, "gather");
 }
 
 int set_ones(Ctx &ctx, const uint64_t *dstates, int count, int prec)

@@ -64,6 +64,14 @@ class DeviceState:
         assert a.nbytes == self.nbytes
         check(self.lib.zxh_dev_upload(self.ptr, a.ctypes.data, a.nbytes))
 
+    def gather(self, indices) -> np.ndarray:
+        """The amplitudes at the given physical addresses (no full download)."""
+        idx = np.ascontiguousarray(indices, dtype=np.uint64).reshape(-1)
+        out = np.empty(idx.size, dtype=self.dtype)
+        check(self.lib.zxh_k_gather(self.ptr, self.prec, self.nbits, idx.ctypes.data_as(_lib._u64p), idx.size,
+                                    out.ctypes.data))
+        return out
+
     def download(self) -> np.ndarray:
         out = np.empty(1 << self.nbits, dtype=self.dtype)
         check(self.lib.zxh_dev_download(out.ctypes.data, self.ptr, self.nbytes))
