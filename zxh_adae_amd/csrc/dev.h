@@ -266,6 +266,7 @@ struct BatchLaunch
 {
     int prec = 0;
     bool adj = false;
+    bool l1 = false, l2 = false; // some item uses register layout 1 / 2
     size_t nitems = 0;
     uint64_t total_tiles = 0;
     int grid = 0;

@@ -916,10 +916,20 @@ __device__ __forceinline__ void collect_rows(const double *__restrict__ partial,
     __shared__ double tot[64];
     const int lane = threadIdx.x & 63;
     const int q = threadIdx.x >> 6;
-    double s = 0.0;
-    for (uint64_t r = q; r < nrows; r += kWavesPerTile)
-        s += partial[r * 64 + lane];
-    part[q][lane] = s;
+    // four independent partial sums per thread (loads in flight), in a fixed
+    // order: deterministic
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    uint64_t r = q;
+    for (; r + 3 * kWavesPerTile < nrows; r += 4 * kWavesPerTile)
+    {
+        s0 += partial[r * 64 + lane];
+        s1 += partial[(r + kWavesPerTile) * 64 + lane];
+        s2 += partial[(r + 2 * kWavesPerTile) * 64 + lane];
+        s3 += partial[(r + 3 * kWavesPerTile) * 64 + lane];
+    }
+    for (; r < nrows; r += kWavesPerTile)
+        s0 += partial[r * 64 + lane];
+    part[q][lane] = (s0 + s1) + (s2 + s3);
     __syncthreads();
     if (threadIdx.x < 64)
         tot[lane] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
@@ -1480,12 +1490,14 @@ struct BItem
     uint32_t sorted[kUnitBits];
 };
 
-template <bool kC64, bool kAdj>
+// kL1 / kL2: the launch's items use layout 1 / 2 (any of them); items that
+// do not get identity matrices there.
+template <bool kC64, bool kAdj, bool kL1, bool kL2>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_tile_kernel(
     const BItem *__restrict__ items, int nitems, uint64_t total)
 {
     using T = typename UnitT<kC64>::T;
-    __shared__ T lds[kUnits];
+    __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
     const int t = threadIdx.x;
     const int lane = t & 63;
     for (uint64_t g = blockIdx.x; g < total; g += gridDim.x)
@@ -1523,7 +1535,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             T w[kRegs];
             load_tile<kC64>(w, pl, base | toff, G);
             AdjRest<kC64> R;
-            adj_tile<kC64, true, true>(v, w, lds, G, t, R);
+            adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
             if (I.ttab != 0)
             {
                 Epi<kC64> epi;
@@ -1539,11 +1551,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         else
         {
             apply_layout<kC64, false, 0>(v, P, G, base, toff);
-            transpose<kC64, 0, 1>(v, lds, t);
-            apply_layout<kC64, false, 1>(v, P, G, base, toff);
-            transpose<kC64, 1, 2>(v, lds, t);
-            apply_layout<kC64, false, 2>(v, P, G, base, toff);
-            transpose<kC64, 2, 0>(v, lds, t);
+            if constexpr (kL1)
+            {
+                transpose<kC64, 0, 1>(v, lds, t);
+                apply_layout<kC64, false, 1>(v, P, G, base, toff);
+                if constexpr (kL2)
+                {
+                    transpose<kC64, 1, 2>(v, lds, t);
+                    apply_layout<kC64, false, 2>(v, P, G, base, toff);
+                    transpose<kC64, 2, 0>(v, lds, t);
+                }
+                else
+                {
+                    transpose<kC64, 1, 0>(v, lds, t);
+                }
+            }
+            else if constexpr (kL2)
+            {
+                transpose<kC64, 0, 2>(v, lds, t);
+                apply_layout<kC64, false, 2>(v, P, G, base, toff);
+                transpose<kC64, 2, 0>(v, lds, t);
+            }
             if (I.ttab != 0)
             {
                 Epi<kC64> epi;
@@ -1554,6 +1582,27 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             store_tile<kC64>(v, pp, base | toff, G);
         }
     }
+}
+
+template <bool kC64, bool kAdj>
+auto batch_kernel_for(bool l1, bool l2)
+{
+    if (l1 && l2)
+        return batch_tile_kernel<kC64, kAdj, true, true>;
+    if (l1)
+        return batch_tile_kernel<kC64, kAdj, true, false>;
+    if (l2)
+        return batch_tile_kernel<kC64, kAdj, false, true>;
+    return batch_tile_kernel<kC64, kAdj, false, false>;
+}
+
+using BatchKernel = void (*)(const BItem *, int, uint64_t);
+
+BatchKernel batch_kernel(bool c64, bool adj, bool l1, bool l2)
+{
+    if (c64)
+        return adj ? batch_kernel_for<true, true>(l1, l2) : batch_kernel_for<true, false>(l1, l2);
+    return adj ? batch_kernel_for<false, true>(l1, l2) : batch_kernel_for<false, false>(l1, l2);
 }
 
 // Per item (one workgroup each): the rows of its tiles and waves summed in a
@@ -1864,6 +1913,8 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         TPass P;
         bool use[3];
         fill_pass(prec, sp.n, plans[i], P, tabs[i], use);
+        b.l1 = b.l1 || use[1];
+        b.l2 = b.l2 || use[2];
         b.slots[i] = plans[i].slot;
         BItem &it = items[i];
         std::memset(&it, 0, sizeof(it));
@@ -1965,9 +2016,7 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         ZXA_TRY(dmalloc(&b.d_collect, col.size() * sizeof(BCollect)));
         ZXA_HIP(hipMemcpy(b.d_collect, col.data(), col.size() * sizeof(BCollect), hipMemcpyHostToDevice));
     }
-    auto k = c64 ? (adj ? batch_tile_kernel<true, true> : batch_tile_kernel<true, false>)
-                 : (adj ? batch_tile_kernel<false, true> : batch_tile_kernel<false, false>);
-    b.grid = persistent_grid(ctx, k, b.total_tiles);
+    b.grid = persistent_grid(ctx, batch_kernel(c64, adj, b.l1, b.l2), b.total_tiles);
     return OK;
 }
 
@@ -2018,9 +2067,7 @@ int batch_launch(Ctx &ctx, BatchLaunch &b)
     if (!b.nitems)
         return OK;
     const bool c64 = b.prec == C64;
-    auto k = c64 ? (b.adj ? batch_tile_kernel<true, true> : batch_tile_kernel<true, false>)
-                 : (b.adj ? batch_tile_kernel<false, true> : batch_tile_kernel<false, false>);
-    hipLaunchKernelGGL(k, dim3(b.grid), dim3(kThreads), 0, ctx.stream, static_cast<const BItem *>(b.d_items),
+    hipLaunchKernelGGL(batch_kernel(c64, b.adj, b.l1, b.l2), dim3(b.grid), dim3(kThreads), 0, ctx.stream, static_cast<const BItem *>(b.d_items),
                        int(b.nitems), b.total_tiles);
     ZXA_HIP(hipGetLastError());
     if (b.adj)
