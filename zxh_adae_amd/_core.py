@@ -23,6 +23,7 @@ from . import _lib
 from ._lib import ZXHError, check
 
 __all__ = ["init", "finalize", "active", "rank", "nprocs", "PrintRes", "ZXH", "ZXHError"]
+__version__ = "0.1.0"
 
 _PREC = {"c128": _lib.ZXH_C128, "complex128": _lib.ZXH_C128, "c64": _lib.ZXH_C64, "complex64": _lib.ZXH_C64}
 
