@@ -51,8 +51,19 @@ def test_golden_circuit(gpu_engine, name, policy):
     seg = min(M, 18)
     want_real = O.oracle_sample_state(phys, seg, O.mix_seed(seed, 0), shots)
     assert np.array_equal(real, want_real)
-    # ... and the virtual bit rows of Sampling() against the reference's own
+    # ... the virtual bit rows packed on the device: exactly the oracle's
+    # affine map (same recording rules) applied to these real addresses ...
     bits = s.results_bits()
+    o = O.OracleSim(n, bool(dx), bool(eager))
+    o.gates(gates)
+    o.execute()
+    rows, b, _ = o.affine()
+    want_bits = np.zeros_like(bits)
+    for q in range(n):
+        par = np.array([bin(int(r) & int(rows[q])).count("1") & 1 for r in real], dtype=np.uint8)
+        want_bits[:, q] = par ^ b[q]
+    assert np.array_equal(bits, want_bits)
+    # ... and against the reference's own Sampling() rows
     mismatch = np.mean(np.any(bits != d["samples"], axis=1))
     assert mismatch <= 0.01, mismatch
 

@@ -332,7 +332,18 @@ int scale_apply(Ctx &ctx, void *state, int prec, unsigned nbits, double re, doub
 // amplitudes and returns the total (the reference's local_prob); search
 // resolves sorted thresholds (host array) against it.
 int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, double *total);
-int sample_search(Ctx &ctx, const void *state, int prec, const double *thresholds, size_t shots, uint64_t *out_real);
+// Optional packing of the samples into virtual result words on the device:
+// bit q = parity(real & rows[q]) ^ b[q]; out (host) gets ceil(nq / 64) words
+// per shot.
+struct PackSpec
+{
+    unsigned nq = 0;
+    const uint64_t *rows = nullptr;
+    const uint8_t *b = nullptr;
+    uint64_t *out = nullptr;
+};
+int sample_search(Ctx &ctx, const void *state, int prec, const double *thresholds, size_t shots, uint64_t *out_real,
+                  const PackSpec *pack = nullptr);
 // Both steps with caller-given thresholds.
 int sample_apply(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, const double *thresholds,
                  size_t shots, uint64_t *out_real);

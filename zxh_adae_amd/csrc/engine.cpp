@@ -643,8 +643,9 @@ int Sim::measure(size_t cnt)
     std::vector<double> thr(cnt);
     threshold_stream(stream_seed, cnt, 0.0, total, thr.data());
     real_results_.resize(cnt);
-    ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr.data(), cnt, real_results_.data()));
-    // pack_virtual_results_host (measure_result_pack.h:24-44)
+    // virtual result words (pack_virtual_results_host,
+    // measure_result_pack.h:24-44): the identity map (plain method) copies
+    // the real addresses; otherwise the device packs them after the search
     const size_t wc = (N_ + 63) / 64;
     packed_.assign(cnt * wc, 0);
     bool identity = (b_.end() == std::find(b_.begin(), b_.end(), uint8_t(1))) && N_ <= 64;
@@ -652,19 +653,18 @@ int Sim::measure(size_t cnt)
         identity = rows_[q] == (uint64_t(1) << q);
     if (identity)
     {
+        ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr.data(), cnt, real_results_.data()));
         for (size_t i = 0; i < cnt; i++)
             packed_[i] = real_results_[i];
     }
     else
     {
-        for (size_t i = 0; i < cnt; i++)
-        {
-            const uint64_t r = real_results_[i];
-            uint64_t *dst = packed_.data() + i * wc;
-            for (size_t q = 0; q < N_; q++)
-                if (((__builtin_popcountll(r & rows_[q]) & 1) ^ b_[q]) != 0)
-                    dst[q >> 6] |= uint64_t(1) << (q & 63);
-        }
+        PackSpec ps;
+        ps.nq = unsigned(N_);
+        ps.rows = rows_.data();
+        ps.b = b_.data();
+        ps.out = packed_.data();
+        ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr.data(), cnt, real_results_.data(), &ps));
     }
     result_count_ = cnt;
     return OK;

@@ -320,7 +320,34 @@ int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned s
     return OK;
 }
 
-int sample_search(Ctx &ctx, const void *state, int prec, const double *thresholds, size_t shots, uint64_t *out_real)
+// Real address -> virtual result words (pack_virtual_results_host,
+// src/zxh/src/measure/measure_result_pack.h:24-44; the reference's GPU path
+// does the same in map_real_to_virtual_words_kernel): virtual bit q of shot i
+// = parity(real[i] & rows[q]) ^ b[q], packed into wc 64-bit words per shot.
+__global__ __launch_bounds__(kThreads) void pack_kernel(const uint64_t *__restrict__ real, uint64_t shots,
+                                                        const uint64_t *__restrict__ rows,
+                                                        const uint8_t *__restrict__ b, int nq, int wc,
+                                                        uint64_t *__restrict__ out)
+{
+    const uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (i >= shots)
+        return;
+    const uint64_t r = real[i];
+    for (int w = 0; w < wc; w++)
+    {
+        uint64_t word = 0;
+        const int hi = min(64, nq - 64 * w);
+        for (int k = 0; k < hi; k++)
+        {
+            const int q = 64 * w + k;
+            word |= uint64_t((__popcll(r & rows[q]) & 1) ^ b[q]) << k;
+        }
+        out[i * uint64_t(wc) + w] = word;
+    }
+}
+
+int sample_search(Ctx &ctx, const void *state, int prec, const double *thresholds, size_t shots, uint64_t *out_real,
+                  const PackSpec *pack)
 {
     if (shots == 0)
         return OK;
@@ -336,11 +363,17 @@ int sample_search(Ctx &ctx, const void *state, int prec, const double *threshold
     const uint64_t nseg = uint64_t(1) << (m - I);
     const double *prefix = static_cast<const double *>(ctx.scratch_d.ptr);
     const double *cmax = prefix + nseg;
-    const size_t need = shots * (sizeof(double) + sizeof(uint64_t)) + 64;
+    const size_t wc = pack ? (size_t(pack->nq) + 63) / 64 : 0;
+    const size_t nq = pack ? pack->nq : 0;
+    // thresholds | real results | packed words | rows | error flag | b
+    const size_t need = shots * (sizeof(double) + sizeof(uint64_t) * (1 + wc)) + nq * sizeof(uint64_t) + 64 + nq;
     ZXA_TRY(ctx.scratch_c.ensure(need));
     double *dthr = static_cast<double *>(ctx.scratch_c.ptr);
     uint64_t *dout = reinterpret_cast<uint64_t *>(dthr + shots);
-    int *derr = reinterpret_cast<int *>(dout + shots);
+    uint64_t *dpacked = dout + shots;
+    uint64_t *drows = dpacked + shots * wc;
+    int *derr = reinterpret_cast<int *>(drows + nq);
+    uint8_t *db = reinterpret_cast<uint8_t *>(derr + 16);
     ZXA_HIP(hipMemcpyAsync(dthr, thresholds, shots * sizeof(double), hipMemcpyHostToDevice, ctx.stream));
     ZXA_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx.stream));
     const unsigned grid = static_cast<unsigned>((shots + kThreads - 1) / kThreads);
@@ -349,6 +382,16 @@ int sample_search(Ctx &ctx, const void *state, int prec, const double *threshold
                        int(I), int(LS), cmax, prefix, nseg, dthr, uint64_t(shots), dout, derr);
     ZXA_HIP(hipGetLastError());
     ZXA_TRY(ctx.toc(F_SAMPLE, 0));
+    if (pack)
+    {
+        ZXA_HIP(hipMemcpyAsync(drows, pack->rows, nq * sizeof(uint64_t), hipMemcpyHostToDevice, ctx.stream));
+        ZXA_HIP(hipMemcpyAsync(db, pack->b, nq, hipMemcpyHostToDevice, ctx.stream));
+        hipLaunchKernelGGL(pack_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, dout, uint64_t(shots), drows, db,
+                           int(nq), int(wc), dpacked);
+        ZXA_HIP(hipGetLastError());
+        ZXA_HIP(hipMemcpyAsync(pack->out, dpacked, shots * wc * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                               ctx.stream));
+    }
     int herr = 0;
     ZXA_HIP(hipMemcpyAsync(out_real, dout, shots * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx.stream));
     ZXA_HIP(hipMemcpyAsync(&herr, derr, sizeof(int), hipMemcpyDeviceToHost, ctx.stream));
