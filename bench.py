@@ -28,6 +28,8 @@ Reported beside it (same JSON line):
                 the gradient error and state infidelity against complex128
   batch_c4      512 parameter sets (16..24 qubits, 2..10 layers, 50 terms,
                 complex64) in one batched engine replayed from a hipGraph
+  zxh_workloads the reference's own 30-45-qubit benchmark circuits (QASM),
+                execute() time in lazy ZXH mode and in the plain method
   cpu_baseline  the reference's CPU path (oracle/_ref when built, else the
                 oracle port) on a bounded prefix of the same circuit, timed on
                 this host
@@ -304,6 +306,40 @@ def batch_c4(W, repeats=3, sequential=64):
             "max_grad_diff_vs_single": gdiff, "max_energy_diff_vs_single": ediff}
 
 
+ZXH_WORKLOADS = ["qft_n30", "qaoa_n30", "vqe_two_local_n30", "vqe_su2_n30", "kernel_crz_dom_n30", "kernel_rz_dom_n30",
+                 "kernel_h_inter_dom_n30", "kernel_h_intra_dom_n30", "bv_n30", "ghz_n32", "multiplier_n45"]
+
+
+def zxh_workloads(Z, repeats=3):
+    """The reference's own benchmark circuits (its QASM workloads, kept as
+    test fixtures) through the drop-in ZXH object, complex128: execute() time
+    in the default lazy ZXH mode (X/CX as affine updates, qubits expanded on
+    demand) and in the plain method (eager_expand_all), with the gate count
+    after lowering and the physical width each mode reached."""
+    from zxh_adae_amd import frontend as F
+    root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "workloads")
+    out = {}
+    for name in ZXH_WORKLOADS:
+        circ = F.load_qasm(os.path.join(root, name + ".qasm"))
+        gs = F.to_gate_tuples(F.lower_circuit(circ))
+        row = {"qubits": circ.num_qubits, "gates": len(gs)}
+        for mode, eager in (("lazy", False), ("plain", True)):
+            if eager and circ.num_qubits > 32:
+                continue
+            s = Z.ZXH(circ.num_qubits, False, eager)
+            s.gates(gs)
+            s.execute()
+            t0 = time.perf_counter()
+            for _ in range(repeats):
+                s.execute()
+            t = (time.perf_counter() - t0) / repeats
+            row[mode] = {"execute_ms": round(t * 1e3, 3), "gates/s": round(len(gs) / t, 1),
+                         "width": int(s.device_state()[1])}
+            del s
+        out[name] = row
+    return out
+
+
 def c5_circuit(W, K, prec, n, layers=40, shots=1000000, seed=5):
     """The large random circuit: per layer floor(n/5) Haar 5-qubit blocks
     (matrix cores), a 64-gate diagonal batch, 8 multi-controlled X gates;
@@ -422,7 +458,8 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--kernels", choices=["c128", "c64"], default=None,
                     help="only launch each hot kernel --steps times on a 30-qubit state (profiling)")
-    ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128"], default=None,
+    ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128", "zxh_workloads"],
+                    default=None,
                     help="run one extra alone, after the tile-geometry autotuning (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -453,6 +490,8 @@ def main():
             out = batch_c4(W)
         elif args.section == "c5_c64":
             out = c5_circuit(W, K, "c64", 33)
+        elif args.section == "zxh_workloads":
+            out = zxh_workloads(Z)
         else:
             out = c5_circuit(W, K, "c128", 32)
         print(json.dumps({args.section: out, "autotune": tuned}), flush=True)
@@ -528,6 +567,8 @@ def main():
         result["adjoint_c3"] = adjoint_c3(W)
         log("batched ansatz: 512 sets, 16..24 qubits, 2..10 layers, c64")
         result["batch_c4"] = batch_c4(W)
+        log("the reference's benchmark circuits, lazy ZXH and plain")
+        result["zxh_workloads"] = zxh_workloads(Z)
         log("large random circuit 33q c64, 40 layers, 1e6 shots")
         result["c5"] = {"c64": c5_circuit(W, K, "c64", 33)}
         log("large random circuit 32q c128, 40 layers, 1e6 shots")

@@ -18,6 +18,7 @@ unitaries in the tests):
     ry(t) -> U3(t, 0, 0)                      cz -> CP(pi)
     swap -> CX CX CX                          rzz(t) -> CX Rz CX
     ccz -> 6 CX + 7 P(+-pi/4)                 ccx -> H ccz H
+    cswap(c, a, b) -> CX(b, a) ccx(c, a, b) CX(b, a)
 Phase-type gates become P rather than the reference's Rz, so get_state can
 differ from the reference's by a global phase; samples and counts cannot.
 
@@ -38,7 +39,7 @@ from typing import List, Sequence
 
 NATIVE_GATES = ("barrier", "measure", "reset", "id", "x", "cx", "h", "z", "rz", "rx", "p", "u", "u3", "cp", "crz")
 FRONTEND_GATES = ("barrier", "measure", "reset", "id", "x", "cx", "swap", "h", "z", "cz", "rz", "rx", "ry", "p",
-                  "u1", "u2", "s", "sdg", "t", "tdg", "u", "u3", "rzz", "cp", "crz", "ccz", "ccx")
+                  "u1", "u2", "s", "sdg", "t", "tdg", "u", "u3", "rzz", "cp", "crz", "ccz", "ccx", "cswap")
 
 
 # ---------------------------------------------------------------------------
@@ -200,6 +201,9 @@ def lower_instruction(name: str, qs: Sequence[int], params: Sequence[float]):
     if name == "ccx":
         a, b, c = qs
         return [("h", (c,), ())] + _ccz(a, b, c) + [("h", (c,), ())]
+    if name == "cswap":
+        c, a, b = qs
+        return [("cx", (b, a), ())] + lower_instruction("ccx", (c, a, b), ()) + [("cx", (b, a), ())]
     raise NotImplementedError(f"Unsupported gate: {name}. allowed_gates={sorted(FRONTEND_GATES)}")
 
 
@@ -212,6 +216,34 @@ def lower_circuit(circuit) -> Circuit:
         params = [_to_float(v) for v in inst.operation.params] if name not in ("barrier", "measure") else []
         for n2, q2, p2 in lower_instruction(name, qs, params):
             out.append(n2, q2, p2)
+    return out
+
+
+# reference gate codes (gate_type_t order, defs.h:65-78; RX the recording alias)
+GATE_CODES = {"z": 2, "rz": 3, "p": 4, "crz": 5, "cp": 6, "x": 7, "cx": 8, "h": 9, "u3": 10, "u": 10, "rx": 11}
+
+
+def to_gate_tuples(circuit) -> list:
+    """A native circuit as (code, cq, q, theta, lambda, phi) tuples, the
+    form ZXH.gates() and the oracle take (U/U3 in ZXH's (theta, lambda, phi)
+    order; barriers kept, measurements and id dropped)."""
+    out = []
+    for inst in circuit.data:
+        name = _name(inst.operation)
+        qs = [_qubit_index(circuit, q) for q in inst.qubits]
+        ps = [_to_float(v) for v in inst.operation.params]
+        if name == "barrier":
+            out.append((1, 0, 0, 0.0, 0.0, 0.0))
+        elif name in ("measure", "id"):
+            continue
+        elif name in ("u", "u3"):
+            out.append((10, 0, qs[0], ps[0], ps[2], ps[1]))
+        elif name in ("cx", "cp", "crz"):
+            out.append((GATE_CODES[name], qs[0], qs[1], ps[0] if ps else 0.0, 0.0, 0.0))
+        elif name in GATE_CODES:
+            out.append((GATE_CODES[name], 0, qs[0], ps[0] if ps else 0.0, 0.0, 0.0))
+        else:
+            raise NotImplementedError(f"no gate tuple for {name} (lower the circuit first)")
     return out
 
 
