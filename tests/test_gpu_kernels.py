@@ -213,16 +213,80 @@ def random_affine(rng, n, cx_count):
     return rows, deltas
 
 
-@pytest.mark.parametrize("n,cx", [(8, 6), (13, 3), (15, 10)])
-def test_u3_batch_affine_selectors(gpu_engine, n, cx):
+@pytest.mark.parametrize("n,cx,prec", [(8, 6, "c128"), (13, 3, "c128"), (15, 10, "c128"), (16, 40, "c128"),
+                                       (20, 120, "c128"), (22, 200, "c128"), (14, 30, "c64"), (19, 90, "c64")])
+def test_u3_batch_affine_selectors(gpu_engine, n, cx, prec):
     """Selectors and pair offsets of a CX-transported map: extra alpha bits
-    (per-pair flip), beta = 1 (swapped roles), multi-bit deltas (pair kernel)."""
+    (per-pair flip), beta = 1 (swapped roles), multi-bit deltas.  From 12
+    qubits (13 for complex64) every gate runs in tile passes whose slots
+    carry the whole offsets, except complex64 offsets that use address bit 0
+    with others (one pair sweep each)."""
+    from zxh_adae_amd import kernels as K
     rng = np.random.default_rng(n)
     rows, deltas = random_affine(rng, n, cx)
     for q in range(n):
         assert bin(rows[q] & deltas[q]).count("1") % 2 == 1
     psi = rand_state(rng, n)
     descs = [(pack(rows[q], bool(rng.random() < 0.5)), deltas[q], rand_u(rng)) for q in range(n) if rng.random() < 0.7]
+    assert sum(bin(dl).count("1") > 1 for _, dl, _ in descs) >= 2
+    want = oracle_u3(psi, n, descs)
+    d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    K.set_timing(True)
+    K.reset_timing()
+    try:
+        d.u3_batch(descs)
+        pair = K.timing("u3_pair")["launches"]
+        tile = K.timing("u3_tile")["launches"]
+    finally:
+        K.set_timing(False)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+    if n >= (13 if prec == "c64" else 12):
+        split = [dl for _, dl, _ in descs if prec == "c64" and dl & 1 and dl != 1]
+        assert pair == len(split)
+        assert 1 <= tile <= len(descs)
+
+
+def test_u3_batch_of_commuting_wide_gates_in_few_passes(gpu_engine):
+    """A full layer of 20 gates after a CX chain (every offset but one has
+    several bits): 16 high pivots over 8 upper slots, so three tile passes
+    instead of 20 pair sweeps."""
+    from zxh_adae_amd import kernels as K
+    n = 20
+    rows = [1 << q for q in range(n)]
+    for q in range(n - 1):
+        rows[q + 1] ^= rows[q]  # CX(q, q+1) transports row q into row q+1
+    # rows of the map: alpha_q = bits 0..q; offsets: delta_q = e_q + e_(q+1)
+    deltas = [(1 << q) | ((1 << (q + 1)) if q + 1 < n else 0) for q in range(n)]
+    for i in range(n):
+        for j in range(n):
+            assert bin(rows[i] & deltas[j]).count("1") % 2 == int(i == j)
+    rng = np.random.default_rng(11)
+    psi = rand_state(rng, n)
+    descs = [(pack(rows[q], False), deltas[q], rand_u(rng)) for q in range(n)]
+    want = oracle_u3(psi, n, descs)
+    d = dev(gpu_engine, psi)
+    K.set_timing(True)
+    K.reset_timing()
+    try:
+        d.u3_batch(descs)
+        pair = K.timing("u3_pair")["launches"]
+        tile = K.timing("u3_tile")["launches"]
+    finally:
+        K.set_timing(False)
+    assert np.max(np.abs(d.download() - want)) < TOL128
+    assert pair == 0
+    assert tile <= 3
+
+
+def test_u3_batch_that_does_not_commute_runs_in_order(gpu_engine):
+    """Two gates on one pair offset (not one batch of a map): applied one
+    after the other, as the oracle's batch loop does."""
+    rng = np.random.default_rng(12)
+    n = 14
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << 9, False), 1 << 9, rand_u(rng)), (pack((1 << 9) | 2, True), 1 << 9, rand_u(rng)),
+             (pack(1 << 3, False), (1 << 3) | (1 << 12), rand_u(rng))]
     want = oracle_u3(psi, n, descs)
     d = dev(gpu_engine, psi)
     d.u3_batch(descs)

@@ -86,10 +86,11 @@ struct TGate
 // ascending.
 struct TPass
 {
-    uint32_t ub[kUnitBits];     // unit-address bit of each unit bit
+    uint32_t ub[kUnitBits];     // unit-address bit of each unit bit (its pivot)
     uint32_t sorted[kUnitBits]; // the same, ascending
+    uint64_t vec[kUnitBits];    // unit-address offset of each unit bit
     uint64_t ntiles;
-    uint64_t tmask; // unit-address mask of the 12 unit bits
+    uint64_t tmask; // unit-address mask of the 12 pivots
     uint64_t gstep; // the grid size deposited into the non-tile bits
 };
 
@@ -144,7 +145,7 @@ __device__ __forceinline__ uint64_t thr_off(int t, const TPass &P)
     uint64_t o = 0;
 #pragma unroll
     for (int j = 0; j < 8; j++)
-        o |= uint64_t((t >> j) & 1) << P.ub[thr_bit(L, j)];
+        o ^= ((t >> j) & 1) ? P.vec[thr_bit(L, j)] : 0;
     return o;
 }
 // Unit address of the first unit of a tile: the tile index deposited into
@@ -239,7 +240,7 @@ __device__ __forceinline__ void unit_pair(typename UnitT<kC64>::T &a, typename U
 }
 
 // Gates on the register bits of layout L (and, in layout 0, the complex64
-// in-unit gate).  base | toff is the unit address of the thread's register 0.
+// in-unit gate).  base ^ toff is the unit address of the thread's register 0.
 template <bool kC64, bool kFlip, int L>
 __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs], const TPass &P, ZXA_CONST TTab *G,
                                              uint64_t base, uint64_t toff)
@@ -252,7 +253,7 @@ __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs]
         Real<kC64> m[8];
         load_matrix<kC64>(m, g);
         const uint64_t amask = kFlip ? g.amask : 0;
-        const uint32_t pt = kFlip ? (par((base | toff) & amask) ^ g.beta) : 0u;
+        const uint32_t pt = kFlip ? (par((base ^ toff) & amask) ^ g.beta) : 0u;
         const bool a0 = kFlip && g.a0 != 0;
 #pragma unroll
         for (int r = 0; r < kRegs; r++)
@@ -269,7 +270,7 @@ __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs]
         float m[8];
         load_matrix<true>(m, g);
         const uint64_t amask = kFlip ? g.amask : 0;
-        const uint32_t pt = kFlip ? (par((base | toff) & amask) ^ g.beta) : 0u;
+        const uint32_t pt = kFlip ? (par((base ^ toff) & amask) ^ g.beta) : 0u;
 #pragma unroll
         for (int r = 0; r < kRegs; r++)
         {
@@ -407,7 +408,7 @@ __device__ __forceinline__ void load_tile(typename UnitT<kC64>::T (&v)[kRegs], c
 {
 #pragma unroll
     for (int r = 0; r < kRegs; r++)
-        v[r] = __builtin_nontemporal_load(p + (at | G->roff[r]));
+        v[r] = __builtin_nontemporal_load(p + (at ^ G->roff[r]));
 }
 
 // The store recomputes its addresses (at is laundered): keeping the loads'
@@ -419,7 +420,7 @@ __device__ __forceinline__ void store_tile(const typename UnitT<kC64>::T (&v)[kR
     asm volatile("" : "+v"(at));
 #pragma unroll
     for (int r = 0; r < kRegs; r++)
-        __builtin_nontemporal_store(v[r], p + (at | G->roff[r]));
+        __builtin_nontemporal_store(v[r], p + (at ^ G->roff[r]));
 }
 
 // Gates, transposes and the diagonal epilogue of one tile held in v.
@@ -490,32 +491,32 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     uint64_t t0 = blockIdx.x;
     uint64_t b0 = deposit(t0, P.sorted);
     ZXA_CONST TTab *G = tables(tabp);
-    load_tile<kC64>(a, p, b0 | toff, G);
+    load_tile<kC64>(a, p, b0 ^ toff, G);
     uint64_t t1 = t0 + grid;
     bool m1 = t1 < P.ntiles;
     uint64_t b1 = next_base(b0, P);
-    load_tile<kC64>(b, p, (m1 ? b1 : b0) | toff, G);
+    load_tile<kC64>(b, p, (m1 ? b1 : b0) ^ toff, G);
     tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t0, b0);
     for (;;)
     {
         G = tables(tabp);
-        store_tile<kC64>(a, p, b0 | toff, G);
+        store_tile<kC64>(a, p, b0 ^ toff, G);
         if (!m1)
             break;
         const uint64_t t2 = t1 + grid;
         const bool m2 = t2 < P.ntiles;
         const uint64_t b2 = next_base(b1, P);
-        load_tile<kC64>(a, p, (m2 ? b2 : b1) | toff, G);
+        load_tile<kC64>(a, p, (m2 ? b2 : b1) ^ toff, G);
         tile_work<kC64, kL1, kL2, kDiag, kFlip>(b, lds, P, G, tphase, epi, t, toff, t1, b1);
 
         G = tables(tabp);
-        store_tile<kC64>(b, p, b1 | toff, G);
+        store_tile<kC64>(b, p, b1 ^ toff, G);
         if (!m2)
             break;
         const uint64_t t3 = t2 + grid;
         const bool m3 = t3 < P.ntiles;
         const uint64_t b3 = next_base(b2, P);
-        load_tile<kC64>(b, p, (m3 ? b3 : b2) | toff, G);
+        load_tile<kC64>(b, p, (m3 ? b3 : b2) ^ toff, G);
         tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t2, b2);
 
         t0 = t2;
@@ -549,46 +550,46 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     uint64_t t0 = blockIdx.x;
     uint64_t b0 = deposit(t0, P.sorted);
     ZXA_CONST TTab *G = tables(tabp);
-    load_tile<kC64>(a, p, b0 | toff, G);
+    load_tile<kC64>(a, p, b0 ^ toff, G);
     uint64_t t1 = t0 + grid;
     bool m1 = t1 < P.ntiles;
     uint64_t b1 = m1 ? next_base(b0, P) : b0;
-    load_tile<kC64>(b, p, b1 | toff, G);
+    load_tile<kC64>(b, p, b1 ^ toff, G);
     uint64_t t2 = t1 + grid;
     bool m2 = m1 && t2 < P.ntiles;
     uint64_t b2 = m2 ? next_base(b1, P) : b1;
-    load_tile<kC64>(c, p, b2 | toff, G);
+    load_tile<kC64>(c, p, b2 ^ toff, G);
     tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t0, b0);
     for (;;)
     {
         G = tables(tabp);
-        store_tile<kC64>(a, p, b0 | toff, G);
+        store_tile<kC64>(a, p, b0 ^ toff, G);
         if (!m1)
             break;
         const uint64_t t3 = t2 + grid;
         const bool m3 = m2 && t3 < P.ntiles;
         const uint64_t b3 = m3 ? next_base(b2, P) : b2;
-        load_tile<kC64>(a, p, b3 | toff, G);
+        load_tile<kC64>(a, p, b3 ^ toff, G);
         tile_work<kC64, kL1, kL2, kDiag, kFlip>(b, lds, P, G, tphase, epi, t, toff, t1, b1);
 
         G = tables(tabp);
-        store_tile<kC64>(b, p, b1 | toff, G);
+        store_tile<kC64>(b, p, b1 ^ toff, G);
         if (!m2)
             break;
         const uint64_t t4 = t3 + grid;
         const bool m4 = m3 && t4 < P.ntiles;
         const uint64_t b4 = m4 ? next_base(b3, P) : b3;
-        load_tile<kC64>(b, p, b4 | toff, G);
+        load_tile<kC64>(b, p, b4 ^ toff, G);
         tile_work<kC64, kL1, kL2, kDiag, kFlip>(c, lds, P, G, tphase, epi, t, toff, t2, b2);
 
         G = tables(tabp);
-        store_tile<kC64>(c, p, b2 | toff, G);
+        store_tile<kC64>(c, p, b2 ^ toff, G);
         if (!m3)
             break;
         const uint64_t t5 = t4 + grid;
         const bool m5 = m4 && t5 < P.ntiles;
         const uint64_t b5 = m5 ? next_base(b4, P) : b4;
-        load_tile<kC64>(c, p, b5 | toff, G);
+        load_tile<kC64>(c, p, b5 ^ toff, G);
         tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t3, b3);
 
         t0 = t3;
@@ -891,14 +892,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     {
         ZXA_CONST TTab *G = tables(tabp);
         T v[kRegs], w[kRegs];
-        load_tile<kC64>(v, pp, base | toff, G);
-        load_tile<kC64>(w, pl, base | toff, G);
+        load_tile<kC64>(v, pp, base ^ toff, G);
+        load_tile<kC64>(w, pl, base ^ toff, G);
         AdjRest<kC64> R;
         adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
         if constexpr (kDiag)
             epi.template apply_n<true>(v, w, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), t);
-        store_tile<kC64>(v, pp, base | toff, G);
-        store_tile<kC64>(w, pl, base | toff, G);
+        store_tile<kC64>(v, pp, base ^ toff, G);
+        store_tile<kC64>(w, pl, base ^ toff, G);
         acc += R.total(t & 63);
         base = next_base(base, P);
     }
@@ -1078,6 +1079,9 @@ int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip,
     return launch_l<kC64, false, false>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
 }
 
+// A gate's target is the address bit its tile slot stands for: the bit of
+// its pair offset for a single-bit offset, else the pivot chosen for it
+// (pivot_passes), and the slot's address offset is then the whole offset.
 struct SingleGate
 {
     unsigned target;
@@ -1095,7 +1099,8 @@ struct SingleGate
 // across lanes where it can).
 struct Plan
 {
-    std::vector<unsigned> ebit; // tile element bit -> address bit
+    std::vector<unsigned> ebit; // tile element bit -> address bit (pivot)
+    std::vector<uint64_t> evec; // tile element bit -> address offset
     std::vector<int> slot;      // per gate: unit bit, or kIntra
 };
 
@@ -1147,11 +1152,16 @@ bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl
     for (unsigned e : pl.ebit)
         if (e >= n)
             return false;
+    pl.evec.clear();
+    for (unsigned e : pl.ebit)
+        pl.evec.push_back(uint64_t(1) << e);
     pl.slot.clear();
     for (const SingleGate &s : gs)
     {
         const int e = int(std::find(pl.ebit.begin(), pl.ebit.end(), s.target) - pl.ebit.begin());
         pl.slot.push_back(c64 ? (e == 0 ? kIntra : e - 1) : e);
+        if (__builtin_popcountll(s.g->delta) > 1)
+            pl.evec[size_t(e)] = s.g->delta;
     }
     return true;
 }
@@ -1281,13 +1291,14 @@ void fill_pass(int prec, unsigned n, const Plan &pl, TPass &P, TTab &G, bool use
     {
         P.ub[k] = pos[k + shift] - shift;
         P.sorted[k] = P.ub[k];
+        P.vec[k] = pl.evec[k + shift] >> shift;
         P.tmask |= uint64_t(1) << P.ub[k];
     }
     std::sort(P.sorted, P.sorted + kUnitBits);
     for (int r = 0; r < kRegs; r++)
         for (int i = 0; i < 4; i++)
             if ((r >> i) & 1)
-                G.roff[r] |= uint64_t(1) << P.ub[reg_bit(0, i)];
+                G.roff[r] ^= P.vec[reg_bit(0, i)];
     P.ntiles = uint64_t(1) << (n - shift - kUnitBits);
     for (TGate &tg : G.g)
     {
@@ -1317,13 +1328,20 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     fill_pass(prec, n, pl, P, G, use);
     const unsigned shift = c64 ? 1u : 0u; // element bit -> unit-address bit
     bool flip = false;
+    // the side of a pair's first element is the selector's parity over that
+    // element's address.  With single-bit slot offsets the target bit is
+    // clear there and drops out of the selector; a wider offset may cover
+    // other gates' target bits, so then every gate keeps its whole selector
+    bool wide = false;
+    for (const SingleGate &s : gs)
+        wide = wide || __builtin_popcountll(s.g->delta) > 1;
     for (size_t gi = 0; gi < gs.size(); gi++)
     {
         const SingleGate &s = gs[gi];
         const int slot = pl.slot[gi];
         TGate &tg = G.g[slot];
         const uint64_t alpha = s.g->packed_sel & ((uint64_t(1) << 63) - 1);
-        const uint64_t rest = alpha & ~s.g->delta;
+        const uint64_t rest = wide ? alpha : alpha & ~s.g->delta;
         uint32_t beta = uint32_t(s.g->packed_sel >> 63);
         const double *u = s.g->u;
         double m[8];
@@ -1352,7 +1370,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
             uint64_t off = 0;
             for (int i = 0; i < 4; i++)
                 if ((r >> i) & 1)
-                    off |= uint64_t(1) << P.ub[reg_bit(lay, i)];
+                    off ^= P.vec[reg_bit(lay, i)];
             tg.rpar |= uint32_t(__builtin_popcountll(off & tg.amask) & 1) << r;
         }
         if (rest != 0)
@@ -1521,6 +1539,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         {
             P.ub[k] = I.ub[k];
             P.sorted[k] = I.sorted[k];
+            P.vec[k] = uint64_t(1) << I.ub[k];
         }
         const uint64_t local = g - I.first;
         const uint64_t base = deposit(local, P.sorted);
@@ -1528,12 +1547,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         ZXA_CONST TTab *G = (ZXA_CONST TTab *)I.tab;
         T *pp = reinterpret_cast<T *>(I.psi);
         T v[kRegs];
-        load_tile<kC64>(v, pp, base | toff, G);
+        load_tile<kC64>(v, pp, base ^ toff, G);
         if constexpr (kAdj)
         {
             T *pl = reinterpret_cast<T *>(I.lam);
             T w[kRegs];
-            load_tile<kC64>(w, pl, base | toff, G);
+            load_tile<kC64>(w, pl, base ^ toff, G);
             AdjRest<kC64> R;
             adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
             if (I.ttab != 0)
@@ -1544,8 +1563,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                     v, w, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * Epi<kC64>::kStride),
                     t);
             }
-            store_tile<kC64>(v, pp, base | toff, G);
-            store_tile<kC64>(w, pl, base | toff, G);
+            store_tile<kC64>(v, pp, base ^ toff, G);
+            store_tile<kC64>(w, pl, base ^ toff, G);
             reinterpret_cast<double *>(I.partial)[(local * kWavesPerTile + (t >> 6)) * 64 + lane] = R.total(lane);
         }
         else
@@ -1579,7 +1598,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
                 epi.apply(v, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * Epi<kC64>::kStride),
                           t);
             }
-            store_tile<kC64>(v, pp, base | toff, G);
+            store_tile<kC64>(v, pp, base ^ toff, G);
         }
     }
 }
@@ -1648,49 +1667,108 @@ unsigned tile_min_bits(int prec)
     return unsigned(kUnitBits) + (prec == C64 ? 1u : 0u);
 }
 
+// Pivots of a set of pair offsets: each offset reduced by the earlier ones
+// (elimination over GF(2)) and its top bit taken, so that the offsets
+// restricted to the pivots form an invertible matrix.  A tile whose slots
+// carry the offsets and whose outer bits skip the pivots is then a coset of
+// the offsets' span, and the tiles cover the state once.  Taking the top bit
+// keeps the contiguous low bits free wherever an offset reaches above them.
+// False if the offsets are linearly dependent.
+bool pivot_bits(const std::vector<const U3Gate *> &gs, unsigned low, std::vector<unsigned> &piv, unsigned *nhigh)
+{
+    std::vector<uint64_t> w;
+    piv.clear();
+    *nhigh = 0;
+    for (const U3Gate *g : gs)
+    {
+        uint64_t x = g->delta;
+        for (size_t i = 0; i < w.size(); i++)
+            if ((x >> piv[i]) & 1)
+                x ^= w[i];
+        if (x == 0)
+            return false;
+        const unsigned p = 63u - unsigned(__builtin_clzll(x));
+        w.push_back(x);
+        piv.push_back(p);
+        if (p >= low)
+            (*nhigh)++;
+    }
+    return true;
+}
+
+// A batch acts on distinct virtual qubits when every gate's selector sees
+// its own pair offset and no other (alpha_i . delta_j = [i == j]): then the
+// gates commute and each is a single-qubit gate in the coordinates the
+// offsets span, so any grouping into passes gives the batch's product.
+bool independent_batch(const U3Gate *gates, size_t ngates)
+{
+    for (size_t i = 0; i < ngates; i++)
+    {
+        const uint64_t alpha = gates[i].packed_sel & ((uint64_t(1) << 63) - 1);
+        for (size_t j = 0; j < ngates; j++)
+            if (unsigned(__builtin_popcountll(alpha & gates[j].delta) & 1) != unsigned(i == j))
+                return false;
+    }
+    return true;
+}
+
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates)
 {
+    const bool c64 = prec == C64;
     const unsigned low = tile_low_bits(prec);
-    std::vector<SingleGate> singles;
+    const unsigned slots = tile_upper_slots(prec);
+    if (!independent_batch(gates, ngates))
+    {
+        // gates that do not commute: one sweep each, in order
+        for (size_t i = 0; i < ngates; i++)
+            ZXA_TRY(u3_pair_apply(ctx, state, prec, n, gates[i]));
+        return OK;
+    }
+    std::vector<const U3Gate *> single, wide;
     for (size_t i = 0; i < ngates; i++)
     {
         const U3Gate &g = gates[i];
         const uint64_t alpha = g.packed_sel & ((uint64_t(1) << 63) - 1);
-        const bool one_bit = g.delta != 0 && (g.delta & (g.delta - 1)) == 0 && (g.delta >> n) == 0;
-        const bool paired = (__builtin_popcountll(alpha & g.delta) & 1) == 1;
-        if (n >= tile_min_bits(prec) && n <= 40 && one_bit && paired && (alpha >> n) == 0)
-            singles.push_back({unsigned(__builtin_ctzll(g.delta)), &g});
+        const bool in_state = g.delta != 0 && (g.delta >> n) == 0 && (alpha >> n) == 0;
+        // complex64 units hold address bit 0: an offset may use it only alone
+        const bool in_units = !c64 || (g.delta & 1u) == 0 || g.delta == 1;
+        if (n >= tile_min_bits(prec) && n <= 40 && in_state && in_units)
+            ((g.delta & (g.delta - 1)) == 0 ? single : wide).push_back(&g);
         else
             ZXA_TRY(u3_pair_apply(ctx, state, prec, n, g));
     }
-    if (singles.empty())
-        return OK;
-    std::sort(singles.begin(), singles.end(),
-              [](const SingleGate &a, const SingleGate &b) { return a.target < b.target; });
-    for (size_t i = 1; i < singles.size(); i++)
-        if (singles[i].target == singles[i - 1].target)
+    // single-bit offsets in ascending order (the low ones ride with the
+    // first pass), then the wider ones; a pass takes gates while their
+    // pivots above the tile's low bits fit its upper slots
+    std::sort(single.begin(), single.end(), [](const U3Gate *a, const U3Gate *b) { return a->delta < b->delta; });
+    std::vector<const U3Gate *> order = single;
+    order.insert(order.end(), wide.begin(), wide.end());
+    std::vector<unsigned> piv;
+    unsigned nhigh = 0;
+    size_t at = 0;
+    while (at < order.size())
+    {
+        std::vector<const U3Gate *> pass;
+        while (at < order.size())
         {
-            set_error("u3_apply: two gates on one target in one batch");
+            pass.push_back(order[at]);
+            if (!pivot_bits(pass, low, piv, &nhigh) || nhigh > slots)
+            {
+                pass.pop_back();
+                break;
+            }
+            at++;
+        }
+        if (pass.empty())
+        {
+            set_error("u3_apply: dependent pair offsets in one batch");
             return EINVAL_;
         }
-    // the low targets ride with the first pass; the others fill the slots
-    std::vector<SingleGate> lows, highs;
-    for (const SingleGate &s : singles)
-        (s.target < low ? lows : highs).push_back(s);
-    size_t hi = 0;
-    bool first = true;
-    while (first || hi < highs.size())
-    {
-        std::vector<SingleGate> pass;
-        if (first)
-            pass = lows;
-        const size_t take = std::min<size_t>(tile_upper_slots(prec), highs.size() - hi);
-        for (size_t k = 0; k < take; k++)
-            pass.push_back(highs[hi + k]);
-        hi += take;
-        first = false;
-        if (!pass.empty())
-            ZXA_TRY(run_tile_pass(ctx, state, prec, n, pass));
+        pivot_bits(pass, low, piv, &nhigh);
+        std::vector<SingleGate> gs;
+        for (size_t k = 0; k < pass.size(); k++)
+            gs.push_back({piv[k], pass[k]});
+        ZXA_TRY(run_tile_pass(ctx, state, prec, n, gs));
     }
     return OK;
 }
