@@ -310,13 +310,14 @@ ZXH_WORKLOADS = ["qft_n30", "qaoa_n30", "vqe_two_local_n30", "vqe_su2_n30", "ker
                  "kernel_h_inter_dom_n30", "kernel_h_intra_dom_n30", "bv_n30", "ghz_n32", "multiplier_n45"]
 
 
-def zxh_workloads(Z, repeats=3):
+def zxh_workloads(Z, repeats=3, breakdown=False):
     """The reference's own benchmark circuits (its QASM workloads, kept as
     test fixtures) through the drop-in ZXH object, complex128: execute() time
     in the default lazy ZXH mode (X/CX as affine updates, qubits expanded on
     demand) and in the plain method (eager_expand_all), with the gate count
     after lowering and the physical width each mode reached."""
     from zxh_adae_amd import frontend as F
+    from zxh_adae_amd import kernels as K
     root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "workloads")
     out = {}
     for name in ZXH_WORKLOADS:
@@ -335,6 +336,19 @@ def zxh_workloads(Z, repeats=3):
             t = (time.perf_counter() - t0) / repeats
             row[mode] = {"execute_ms": round(t * 1e3, 3), "gates/s": round(len(gs) / t, 1),
                          "width": int(s.device_state()[1])}
+            if breakdown:
+                # one more run with HIP-event timing per kernel family
+                s.reset_timing()
+                s.set_timing(True)
+                s.execute()
+                K.sync()
+                s.set_timing(False)
+                fams = {}
+                for fam in K.FAMILIES:
+                    tm = s.timing(fam)
+                    if tm["launches"]:
+                        fams[fam] = {"launches": tm["launches"], "ms": round(tm["ms"], 3)}
+                row[mode]["families"] = fams
             del s
         out[name] = row
     return out
@@ -478,7 +492,7 @@ def main():
         return
     from zxh_adae_amd import autotune
     if args.section:
-        prec = "c128" if args.section == "c5_c128" else "c64"
+        prec = "c128" if args.section in ("c5_c128", "zxh_workloads") else "c64"
         st = K.DeviceState(NQ, prec)
         st.scale(0.0)
         tuned = autotune.tune_tile(st)
@@ -491,7 +505,7 @@ def main():
         elif args.section == "c5_c64":
             out = c5_circuit(W, K, "c64", 33)
         elif args.section == "zxh_workloads":
-            out = zxh_workloads(Z)
+            out = zxh_workloads(Z, breakdown=True)
         else:
             out = c5_circuit(W, K, "c128", 32)
         print(json.dumps({args.section: out, "autotune": tuned}), flush=True)

@@ -430,15 +430,23 @@ int Sim::flush_u3(std::vector<Acc> &pend, size_t &count)
         Acc &a = pend[q];
         if (!a.active)
             continue;
-        uint64_t d = 0;
-        if (!solve(q, &d))
-        {
-            set_error("flush_u3_batch requires no-expand U3 descriptors");
-            return ESTATE;
-        }
         U3Gate g;
-        g.packed_sel = local_selector(q);
-        g.delta = d & low_mask(lb);
+        if (a.fixed)
+        {
+            g.packed_sel = a.sel;
+            g.delta = a.delta;
+        }
+        else
+        {
+            uint64_t d = 0;
+            if (!solve(q, &d))
+            {
+                set_error("flush_u3_batch requires no-expand U3 descriptors");
+                return ESTATE;
+            }
+            g.packed_sel = local_selector(q);
+            g.delta = d & low_mask(lb);
+        }
         const std::complex<double> u[4] = {a.u00, a.u01, a.u10, a.u11};
         for (int i = 0; i < 4; i++)
         {
@@ -477,92 +485,188 @@ int Sim::execute()
     pass_diag_.clear();
     pass_support_ = 0;
     pass_high_ = 0;
-    std::vector<DiagGate> window;
+
+    // Scheduling.  The reference policy batches each run of consecutive U3
+    // gates and flushes the diagonal window when a run starts (zxh.cpp:440-470).
+    // The fused policy keeps a batch open across later gates: pending are
+    // the diagonal gates to apply before the batch (pre), the batch (one
+    // product per qubit and run, each with the descriptor of its own place in
+    // the circuit) and the diagonal gates to apply after it (post), and the
+    // order pre, batch, post is a valid reordering of the circuit throughout:
+    //  - X and CX as affine updates change only the map;
+    //  - a diagonal gate joins pre when it commutes with every member
+    //    (selector . delta = 0 over GF(2)), else post;
+    //  - a U3 gate joins the batch when it commutes with every gate in post
+    //    and every member of another run (alpha . delta' = alpha' . delta = 0),
+    //    or continues its own run's product on its qubit.  Otherwise pre and
+    //    the members of earlier runs are applied, and post becomes pre (no
+    //    diagonal gate arrives inside a run, so the current run's members
+    //    follow post in the circuit and commute with it).
+    // Each run keeps its own product per qubit, as the reference forms it,
+    // so the 1e-5 identity skip sees the same products.
+    const bool stream = policy_ == POLICY_FUSED;
+    std::vector<DiagGate> pre, post;
     std::vector<Acc> pend(N_);
     size_t pend_count = 0;
+    std::vector<size_t> owner(N_, 0);
+    size_t run = 0;
+    bool in_run = false;
 
-    size_t gi = 0;
-    while (gi < gates_.size())
+    // absorb_u3_gate + left_multiply_u3 (zxh.cpp:144-156, 363-388): the
+    // accumulator restarts from the identity once it has come within 1e-5 of it
+    auto absorb = [&](size_t q, const std::complex<double> (&u)[4], uint64_t sel, uint64_t delta) {
+        Acc &a = pend[q];
+        const bool was = a.active;
+        if (!was)
+        {
+            a = Acc();
+            a.active = true;
+            a.fixed = true;
+            a.sel = sel;
+            a.delta = delta;
+        }
+        const std::complex<double> o00 = a.u00, o01 = a.u01, o10 = a.u10, o11 = a.u11;
+        a.u00 = u[0] * o00 + u[1] * o10;
+        a.u01 = u[0] * o01 + u[1] * o11;
+        a.u10 = u[2] * o00 + u[3] * o10;
+        a.u11 = u[2] * o01 + u[3] * o11;
+        a.active = !(close_val(a.u00, {1.0, 0.0}) && close_val(a.u01, {0.0, 0.0}) && close_val(a.u10, {0.0, 0.0}) &&
+                     close_val(a.u11, {1.0, 0.0}));
+        if (!was && a.active)
+            pend_count++;
+        else if (was && !a.active)
+            pend_count--;
+        owner[q] = run;
+    };
+    const uint64_t amask = (uint64_t(1) << 63) - 1;
+    auto par = [](uint64_t x) { return (__builtin_popcountll(x) & 1) != 0; };
+    auto diag_commutes = [&](const DiagGate &d, uint64_t delta) {
+        return !par(d.s0 & amask & delta) && (d.s1 == kNoSel || !par(d.s1 & amask & delta));
+    };
+    auto diag_joins_pre = [&](const DiagGate &d) {
+        for (size_t k = 0; k < N_; k++)
+            if (pend[k].active && !diag_commutes(d, pend[k].delta))
+                return false;
+        return true;
+    };
+    auto push_diag = [&](const DiagGate &d) {
+        if (stream && !diag_joins_pre(d))
+            post.push_back(d);
+        else
+            pre.push_back(d);
+    };
+    // everything pending, in order
+    auto flush_all = [&]() -> int {
+        ZXA_TRY(flush_diag(pre));
+        ZXA_TRY(flush_u3(pend, pend_count));
+        return flush_diag(post);
+    };
+    // pre and the members of earlier runs; post becomes pre
+    auto flush_earlier = [&]() -> int {
+        std::vector<std::pair<size_t, Acc>> keep;
+        for (size_t k = 0; k < N_; k++)
+            if (pend[k].active && owner[k] == run)
+            {
+                keep.push_back({k, pend[k]});
+                pend[k] = Acc();
+                pend_count--;
+            }
+        ZXA_TRY(flush_diag(pre));
+        ZXA_TRY(flush_u3(pend, pend_count));
+        for (const auto &kv : keep)
+        {
+            pend[kv.first] = kv.second;
+            pend_count++;
+        }
+        pre.swap(post);
+        post.clear();
+        return OK;
+    };
+
+    for (size_t gi = 0; gi < gates_.size(); gi++)
     {
         const GateRec &g = gates_[gi];
-        if (g.type == G_H || g.type == G_U3)
+        const bool is_u3 = g.type == G_H || g.type == G_U3;
+        if (!is_u3 && in_run)
+        {
+            // end of a run: the reference flushes its batch here
+            in_run = false;
+            if (!stream)
+                ZXA_TRY(flush_u3(pend, pend_count));
+        }
+        if (is_u3)
         {
             // U3 cluster (zxh.cpp:440-470)
-            ZXA_TRY(flush_diag(window));
-            while (gi < gates_.size() && (gates_[gi].type == G_H || gates_[gi].type == G_U3))
+            if (!in_run)
             {
-                const GateRec &ug = gates_[gi];
-                const double th = ug.type == G_H ? kPi / 2.0 : ug.theta;
-                const double la = ug.type == G_H ? kPi : ug.lambda;
-                const double ph = ug.type == G_H ? 0.0 : ug.phi;
-                std::complex<double> u[4];
-                u3_matrix(th, la, ph, u);
-                uint64_t delta = 0;
-                if (!solve(ug.q, &delta))
-                {
-                    // flush the batch, then the eager, expanding apply_u3
-                    ZXA_TRY(flush_u3(pend, pend_count));
-                    ZXA_TRY(flush_pass());
-                    int rc = OK;
-                    const uint64_t d = solve_expand(ug.q, &rc);
-                    if (rc)
-                        return rc;
-                    U3Gate eg;
-                    eg.packed_sel = local_selector(ug.q);
-                    eg.delta = d & low_mask(active_bits());
-                    for (int i = 0; i < 4; i++)
-                    {
-                        eg.u[2 * i] = u[i].real();
-                        eg.u[2 * i + 1] = u[i].imag();
-                    }
-                    ZXA_TRY(u3_pair_apply(ctx_, dstate_, prec_, active_bits(), eg));
-                }
-                else
-                {
-                    // absorb_u3_gate + left_multiply_u3 (zxh.cpp:144-156, 363-388)
-                    Acc &a = pend[ug.q];
-                    const bool was = a.active;
-                    if (!was)
-                    {
-                        a = Acc();
-                        a.active = true;
-                    }
-                    const std::complex<double> o00 = a.u00, o01 = a.u01, o10 = a.u10, o11 = a.u11;
-                    a.u00 = u[0] * o00 + u[1] * o10;
-                    a.u01 = u[0] * o01 + u[1] * o11;
-                    a.u10 = u[2] * o00 + u[3] * o10;
-                    a.u11 = u[2] * o01 + u[3] * o11;
-                    a.active = !(close_val(a.u00, {1.0, 0.0}) && close_val(a.u01, {0.0, 0.0}) &&
-                                 close_val(a.u10, {0.0, 0.0}) && close_val(a.u11, {1.0, 0.0}));
-                    if (!was && a.active)
-                        pend_count++;
-                    else if (was && !a.active)
-                        pend_count--;
-                }
-                gi++;
+                in_run = true;
+                run = gi;
+                if (!stream)
+                    ZXA_TRY(flush_diag(pre));
             }
-            ZXA_TRY(flush_u3(pend, pend_count));
+            const double th = g.type == G_H ? kPi / 2.0 : g.theta;
+            const double la = g.type == G_H ? kPi : g.lambda;
+            const double ph = g.type == G_H ? 0.0 : g.phi;
+            std::complex<double> u[4];
+            u3_matrix(th, la, ph, u);
+            uint64_t delta = 0;
+            if (!solve(g.q, &delta))
+            {
+                // flush the batch, then the eager, expanding apply_u3
+                ZXA_TRY(flush_all());
+                ZXA_TRY(flush_pass());
+                int rc = OK;
+                const uint64_t d = solve_expand(g.q, &rc);
+                if (rc)
+                    return rc;
+                U3Gate eg;
+                eg.packed_sel = local_selector(g.q);
+                eg.delta = d & low_mask(active_bits());
+                for (int i = 0; i < 4; i++)
+                {
+                    eg.u[2 * i] = u[i].real();
+                    eg.u[2 * i + 1] = u[i].imag();
+                }
+                ZXA_TRY(u3_pair_apply(ctx_, dstate_, prec_, active_bits(), eg));
+                continue;
+            }
+            delta &= low_mask(active_bits());
+            const uint64_t sel = local_selector(g.q);
+            if (stream)
+            {
+                const uint64_t alpha = sel & amask;
+                bool ok = !(pend[g.q].active && owner[g.q] != run);
+                for (const DiagGate &d : post)
+                    ok = ok && diag_commutes(d, delta);
+                for (size_t k = 0; k < N_ && ok; k++)
+                    if (pend[k].active && owner[k] != run)
+                        ok = !par(alpha & pend[k].delta) && !par(pend[k].sel & amask & delta);
+                if (!ok)
+                    ZXA_TRY(flush_earlier());
+            }
+            // absorb_u3_gate + left_multiply_u3 (zxh.cpp:144-156, 363-388)
+            absorb(g.q, u, sel, delta);
             continue;
         }
         switch (g.type)
         {
         case G_BARRIER:
-            ZXA_TRY(flush_diag(window));
+            ZXA_TRY(flush_all());
             break;
         case G_Z:
         case G_RZ: {
             const double th = g.type == G_Z ? kPi : g.theta;
-            window.push_back({local_selector(g.q), kNoSel, th});
+            push_diag({local_selector(g.q), kNoSel, th});
             gphase_ *= phase(-0.5 * th);
             break;
         }
         case G_P:
-            window.push_back({local_selector(g.q), kNoSel, g.theta});
+            push_diag({local_selector(g.q), kNoSel, g.theta});
             break;
         case G_X:
             if (disable_x_)
             {
-                ZXA_TRY(flush_diag(window));
+                ZXA_TRY(flush_all());
                 ZXA_TRY(flush_pass());
                 int rc = OK;
                 const uint64_t d = solve_expand(g.q, &rc);
@@ -577,7 +681,7 @@ int Sim::execute()
         case G_CX:
             if (disable_x_)
             {
-                ZXA_TRY(flush_diag(window));
+                ZXA_TRY(flush_all());
                 ZXA_TRY(flush_pass());
                 int rc = OK;
                 const uint64_t d = solve_expand(g.q, &rc);
@@ -596,20 +700,19 @@ int Sim::execute()
             const uint64_t scq = local_selector(g.cq), sq = local_selector(g.q);
             if (sel_class(scq) == 2 || sel_class(sq) == 2)
             {
-                ZXA_TRY(flush_diag(window));
+                ZXA_TRY(flush_all());
                 ZXA_TRY(flush_pass());
                 ZXA_TRY(cp_apply(ctx_, dstate_, prec_, active_bits(), scq, sq, g.theta));
             }
             else
-                window.push_back({scq, sq, g.theta});
+                push_diag({scq, sq, g.theta});
             break;
         }
         default:
             break;
         }
-        gi++;
     }
-    ZXA_TRY(flush_diag(window));
+    ZXA_TRY(flush_all());
     ZXA_TRY(flush_pass());
     ZXA_HIP(hipStreamSynchronize(ctx_.stream));
     executed_ = true;

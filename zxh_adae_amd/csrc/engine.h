@@ -113,6 +113,10 @@ class Sim
     {
         bool active = false;
         std::complex<double> u00{1.0, 0.0}, u01{0.0, 0.0}, u10{0.0, 0.0}, u11{1.0, 0.0};
+        // a product hoisted from a later run keeps the descriptor of its own
+        // place in the circuit (else it is taken from the map at the flush)
+        bool fixed = false;
+        uint64_t sel = 0, delta = 0;
     };
 
     bool solve(size_t q, uint64_t *x) const;
