@@ -145,6 +145,25 @@ def test_diag_single_bit_selectors(gpu_engine, n, count, prec):
     assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
 
 
+@pytest.mark.parametrize("n,count", [(12, 40), (14, 300), (17, 120), (20, 64)])
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_diag_two_bit_single_selectors_polynomial(gpu_engine, n, count, prec):
+    """P / Rz-type gates on two-bit selectors (an Rzz in lazy mode) mixed with
+    single-bit CP: still a degree-2 phase polynomial, swept by the fast path."""
+    rng = np.random.default_rng(n * 31 + count)
+    psi = rand_state(rng, n)
+    s0, s1, th = diag_window(rng, n, count)
+    for i in range(count):
+        if s1[i] == SEL_NONE and rng.random() < 0.7:
+            u, v = (int(b) for b in rng.choice(n, 2, replace=False))
+            s0[i] = pack((1 << u) | (1 << v), bool(rng.random() < 0.4))
+    want = oracle_diag(psi, n, s0, s1, th)
+    d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    d.diag(s0, s1, th)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
 @pytest.mark.parametrize("n", [6, 12, 14])
 def test_diag_multi_bit_selectors(gpu_engine, n):
     rng = np.random.default_rng(7 + n)

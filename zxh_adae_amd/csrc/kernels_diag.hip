@@ -641,13 +641,18 @@ int diag_general(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *ga
 
 } // namespace
 
+// Degree <= 2 in the address bits: a two-selector gate whose selectors have
+// at most one bit each, or a one-selector gate (P, Rz, Z) whose selector has
+// at most two bits ([x_u ^ x_v] = x_u + x_v - 2 x_u x_v; the Rz of an
+// Rzz lowered as CX Rz CX with X and CX as affine updates is one).
 bool window_is_polynomial(unsigned n, const DiagGate *gates, size_t ngates)
 {
     const uint64_t amask = (uint64_t(1) << 63) - 1;
     for (size_t i = 0; i < ngates; i++)
     {
         const uint64_t a0 = gates[i].s0 & amask;
-        if (__builtin_popcountll(a0) > 1 || (n < 64 && (a0 >> n) != 0))
+        const int most = gates[i].s1 == kNoSel ? 2 : 1;
+        if (__builtin_popcountll(a0) > most || (n < 64 && (a0 >> n) != 0))
             return false;
         if (gates[i].s1 != kNoSel)
         {
@@ -659,7 +664,9 @@ bool window_is_polynomial(unsigned n, const DiagGate *gates, size_t ngates)
     return true;
 }
 
-// theta * [S0][S1] with [x_j ^ beta] = beta + (1 - 2 beta) x_j.
+// theta * [S0][S1] with [x_j ^ beta] = beta + (1 - 2 beta) x_j, and for a
+// one-selector gate on two bits theta * [x_u ^ x_v ^ beta]
+// = theta * (beta + (1 - 2 beta) (x_u + x_v - 2 x_u x_v)).
 void window_polynomial(unsigned n, const DiagGate *gates, size_t ngates, Poly &P)
 {
     struct Lin
@@ -682,6 +689,18 @@ void window_polynomial(unsigned n, const DiagGate *gates, size_t ngates, Poly &P
     for (size_t i = 0; i < ngates; i++)
     {
         const DiagGate &g = gates[i];
+        const uint64_t a0 = g.s0 & ((uint64_t(1) << 63) - 1);
+        if (g.s1 == kNoSel && __builtin_popcountll(a0) == 2)
+        {
+            const int u = __builtin_ctzll(a0), v = 63 - __builtin_clzll(a0);
+            const double beta = (g.s0 >> 63) ? 1.0 : 0.0;
+            const double sl = g.theta * (1.0 - 2.0 * beta);
+            P.K += g.theta * beta;
+            P.L[u] += sl;
+            P.L[v] += sl;
+            P.Q[size_t(u) * n + v] += -2.0 * sl;
+            continue;
+        }
         const Lin a = lin_of(g.s0);
         const Lin b = g.s1 == kNoSel ? Lin{-1, 1.0, 0.0} : lin_of(g.s1);
         const double th = g.theta;

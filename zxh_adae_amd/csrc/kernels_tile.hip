@@ -1738,32 +1738,38 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
             ZXA_TRY(u3_pair_apply(ctx, state, prec, n, g));
     }
     // single-bit offsets in ascending order (the low ones ride with the
-    // first pass), then the wider ones; a pass takes gates while their
-    // pivots above the tile's low bits fit its upper slots
+    // first pass), then the wider ones.  A pass takes every remaining gate
+    // whose pivot fits: the pivots above the tile's low bits within its
+    // upper slots, and no offset reaching above the low bits on a pivot
+    // below the first whole 128-byte line of the contiguous run (its lanes
+    // would read partial lines of two places).  A pass's first gate always
+    // fits, so every pass takes at least one.
     std::sort(single.begin(), single.end(), [](const U3Gate *a, const U3Gate *b) { return a->delta < b->delta; });
-    std::vector<const U3Gate *> order = single;
-    order.insert(order.end(), wide.begin(), wide.end());
+    std::vector<const U3Gate *> rest = single;
+    rest.insert(rest.end(), wide.begin(), wide.end());
+    const unsigned whole = 3u + (c64 ? 1u : 0u);
     std::vector<unsigned> piv;
     unsigned nhigh = 0;
-    size_t at = 0;
-    while (at < order.size())
+    while (!rest.empty())
     {
-        std::vector<const U3Gate *> pass;
-        while (at < order.size())
+        std::vector<const U3Gate *> pass, later;
+        for (const U3Gate *g : rest)
         {
-            pass.push_back(order[at]);
-            if (!pivot_bits(pass, low, piv, &nhigh) || nhigh > slots)
+            pass.push_back(g);
+            const bool fits = pivot_bits(pass, low, piv, &nhigh) && nhigh <= slots &&
+                              !(piv.back() < std::min(low, whole) && (g->delta >> low) != 0);
+            if (!fits)
             {
                 pass.pop_back();
-                break;
+                later.push_back(g);
             }
-            at++;
         }
         if (pass.empty())
         {
             set_error("u3_apply: dependent pair offsets in one batch");
             return EINVAL_;
         }
+        rest.swap(later);
         pivot_bits(pass, low, piv, &nhigh);
         std::vector<SingleGate> gs;
         for (size_t k = 0; k < pass.size(); k++)

@@ -1,9 +1,11 @@
 // poly.h -- a diagonal window as a phase polynomial over the address bits.
 //
-// When every selector of the window has at most one alpha bit,
+// When every selector of a two-selector gate has at most one alpha bit and
+// every one-selector gate's selector at most two,
 //     theta(x) = sum_g theta_g [S0_g(x)] [S1_g(x)]
 //              = K + sum_j L_j x_j + sum_{j<k} Q_jk x_j x_k ,
-// using [x_j ^ beta] = beta + (1 - 2 beta) x_j.  Both the diagonal sweep and
+// using [x_j ^ beta] = beta + (1 - 2 beta) x_j and
+// [x_u ^ x_v] = x_u + x_v - 2 x_u x_v.  Both the diagonal sweep and
 // the diagonal epilogue of the fused tile pass evaluate theta through this
 // form, split into per-lane / per-row / per-block factors.
 #pragma once
@@ -27,8 +29,9 @@ struct Poly
     }
 };
 
-// True when every selector of the window has at most one alpha bit, all
-// inside the n address bits (the polynomial form exists).
+// True when the polynomial form exists: the selectors have at most one
+// alpha bit (two for a gate with one selector), all inside the n address
+// bits.
 bool window_is_polynomial(unsigned n, const DiagGate *gates, size_t ngates);
 // Builds the polynomial; requires window_is_polynomial.
 void window_polynomial(unsigned n, const DiagGate *gates, size_t ngates, Poly &out);
