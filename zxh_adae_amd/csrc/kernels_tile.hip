@@ -239,6 +239,27 @@ __device__ __forceinline__ void unit_pair(typename UnitT<kC64>::T &a, typename U
     }
 }
 
+// (a, b) -> M (a, b) with the matrix ml for the first amplitude of each unit
+// and mh for the second (complex64; complex128 units hold one amplitude).
+template <bool kC64>
+__device__ __forceinline__ void unit_pair2(typename UnitT<kC64>::T &a, typename UnitT<kC64>::T &b,
+                                           const Real<kC64> (&ml)[8], const Real<kC64> (&mh)[8])
+{
+    if constexpr (kC64)
+    {
+        const Pair<float> lo = pair2<float>(a.x, a.y, b.x, b.y, ml, false);
+        const Pair<float> hi = pair2<float>(a.z, a.w, b.z, b.w, mh, false);
+        a = fvec4{lo.ax, lo.ay, hi.ax, hi.ay};
+        b = fvec4{lo.bx, lo.by, hi.bx, hi.by};
+    }
+    else
+    {
+        const Pair<double> o = pair2<double>(a.x, a.y, b.x, b.y, ml, false);
+        a = dvec2{o.ax, o.ay};
+        b = dvec2{o.bx, o.by};
+    }
+}
+
 // Gates on the register bits of layout L (and, in layout 0, the complex64
 // in-unit gate).  base ^ toff is the unit address of the thread's register 0.
 template <bool kC64, bool kFlip, int L>
@@ -255,6 +276,30 @@ __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs]
         const uint64_t amask = kFlip ? g.amask : 0;
         const uint32_t pt = kFlip ? (par((base ^ toff) & amask) ^ g.beta) : 0u;
         const bool a0 = kFlip && g.a0 != 0;
+        if (kFlip && g.rpar == 0u)
+        {
+            // the side does not change across the thread's registers (rpar
+            // is uniform): select the matrix once, for both unit halves
+            // (the operands are laundered: the compiler would otherwise turn
+            // the select into a run-time index into m)
+            Real<kC64> ml[8], mh[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+            {
+                Real<kC64> keep = m[e], swap = m[e ^ 6];
+                asm volatile("" : "+v"(keep), "+v"(swap));
+                ml[e] = pt ? swap : keep;
+                mh[e] = (pt ^ uint32_t(a0)) ? swap : keep;
+            }
+#pragma unroll
+            for (int r = 0; r < kRegs; r++)
+            {
+                if ((r >> i) & 1)
+                    continue;
+                unit_pair2<kC64>(v[r], v[r | (1 << i)], ml, mh);
+            }
+            continue;
+        }
 #pragma unroll
         for (int r = 0; r < kRegs; r++)
         {
@@ -1335,13 +1380,58 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     bool wide = false;
     for (const SingleGate &s : gs)
         wide = wide || __builtin_popcountll(s.g->delta) > 1;
+    bool any_rest = false;
+    for (const SingleGate &s : gs)
+    {
+        const uint64_t alpha = s.g->packed_sel & ((uint64_t(1) << 63) - 1);
+        any_rest = any_rest || (wide ? alpha : alpha & ~s.g->delta) != 0;
+    }
+    if (any_rest)
+    {
+        // Filler unit bits (neither contiguous nor a slot) take the offsets
+        // of the gates whose selector sees them: v ^= sum_j [alpha_j . v]
+        // delta_j leaves the tile's span (so the tiles) unchanged and makes
+        // alpha_k . v = 0 for every gate k (alpha_k . delta_j = [k == j]),
+        // so a gate's side no longer changes across those bits; where it
+        // changes across none of a layout's register bits the kernel picks
+        // the matrix once per thread instead of once per pair.
+        const int seg = int(tile_seg(prec, geometry));
+        std::vector<bool> is_slot(kUnitBits, false);
+        for (int sl : pl.slot)
+            if (sl != kIntra)
+                is_slot[size_t(sl)] = true;
+        for (int k = seg; k < kUnitBits; k++)
+        {
+            if (is_slot[size_t(k)])
+                continue;
+            const uint64_t v0 = P.vec[k];
+            uint64_t v = v0;
+            for (size_t gi = 0; gi < gs.size(); gi++)
+            {
+                if (pl.slot[gi] == kIntra)
+                    continue;
+                const uint64_t au = (gs[gi].g->packed_sel & ((uint64_t(1) << 63) - 1)) >> shift;
+                if (__builtin_popcountll(au & v0) & 1)
+                    v ^= gs[gi].g->delta >> shift;
+            }
+            P.vec[k] = v;
+        }
+        for (int r = 0; r < kRegs; r++)
+        {
+            G.roff[r] = 0;
+            for (int i = 0; i < 4; i++)
+                if ((r >> i) & 1)
+                    G.roff[r] ^= P.vec[reg_bit(0, i)];
+        }
+    }
     for (size_t gi = 0; gi < gs.size(); gi++)
     {
         const SingleGate &s = gs[gi];
         const int slot = pl.slot[gi];
         TGate &tg = G.g[slot];
         const uint64_t alpha = s.g->packed_sel & ((uint64_t(1) << 63) - 1);
-        const uint64_t rest = wide ? alpha : alpha & ~s.g->delta;
+        // filler offsets may now cover target bits too: whole selectors
+        const uint64_t rest = any_rest ? alpha : alpha & ~s.g->delta;
         uint32_t beta = uint32_t(s.g->packed_sel >> 63);
         const double *u = s.g->u;
         double m[8];
@@ -1365,8 +1455,17 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         tg.a0 = c64 ? uint32_t(rest & 1u) : 0u;
         tg.beta = beta;
         const int lay = slot == kIntra ? 0 : layout_of_unit_bit(slot);
+        // only the registers holding a pair's first element (the gate's own
+        // register bit clear) are read, so rpar == 0 when the side is the
+        // same across all of them
+        int own = -1;
+        for (int i = 0; i < 4; i++)
+            if (slot != kIntra && reg_bit(lay, i) == slot)
+                own = i;
         for (int r = 0; r < kRegs; r++)
         {
+            if (own >= 0 && ((r >> own) & 1))
+                continue;
             uint64_t off = 0;
             for (int i = 0; i < 4; i++)
                 if ((r >> i) & 1)
@@ -1712,11 +1811,52 @@ bool independent_batch(const U3Gate *gates, size_t ngates)
     return true;
 }
 
+// Passes of a batch for one tile geometry.  Single-bit offsets in ascending
+// order (the low ones ride with the first pass), then the wider ones.  A pass
+// takes every remaining gate whose pivot fits: the pivots above the tile's
+// low bits within its upper slots, and no offset reaching above the low bits
+// on a pivot below the first whole 128-byte line of the contiguous run (its
+// lanes would read partial lines of two places).  A pass's first gate
+// always fits, so every pass takes at least one.
+bool plan_passes(int prec, int geometry, std::vector<const U3Gate *> rest,
+                 std::vector<std::vector<SingleGate>> &passes)
+{
+    const bool c64 = prec == C64;
+    const unsigned low = tile_low_bits(prec, geometry);
+    const unsigned slots = tile_upper_slots(prec, geometry);
+    const unsigned whole = 3u + (c64 ? 1u : 0u);
+    std::vector<unsigned> piv;
+    unsigned nhigh = 0;
+    passes.clear();
+    while (!rest.empty())
+    {
+        std::vector<const U3Gate *> pass, later;
+        for (const U3Gate *g : rest)
+        {
+            pass.push_back(g);
+            const bool fits = pivot_bits(pass, low, piv, &nhigh) && nhigh <= slots &&
+                              !(piv.back() < std::min(low, whole) && (g->delta >> low) != 0);
+            if (!fits)
+            {
+                pass.pop_back();
+                later.push_back(g);
+            }
+        }
+        if (pass.empty())
+            return false;
+        rest.swap(later);
+        pivot_bits(pass, low, piv, &nhigh);
+        std::vector<SingleGate> gs;
+        for (size_t k = 0; k < pass.size(); k++)
+            gs.push_back({piv[k], pass[k]});
+        passes.push_back(gs);
+    }
+    return true;
+}
+
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates)
 {
     const bool c64 = prec == C64;
-    const unsigned low = tile_low_bits(prec);
-    const unsigned slots = tile_upper_slots(prec);
     if (!independent_batch(gates, ngates))
     {
         // gates that do not commute: one sweep each, in order
@@ -1737,45 +1877,31 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
         else
             ZXA_TRY(u3_pair_apply(ctx, state, prec, n, g));
     }
-    // single-bit offsets in ascending order (the low ones ride with the
-    // first pass), then the wider ones.  A pass takes every remaining gate
-    // whose pivot fits: the pivots above the tile's low bits within its
-    // upper slots, and no offset reaching above the low bits on a pivot
-    // below the first whole 128-byte line of the contiguous run (its lanes
-    // would read partial lines of two places).  A pass's first gate always
-    // fits, so every pass takes at least one.
+    if (single.empty() && wide.empty())
+        return OK;
     std::sort(single.begin(), single.end(), [](const U3Gate *a, const U3Gate *b) { return a->delta < b->delta; });
-    std::vector<const U3Gate *> rest = single;
-    rest.insert(rest.end(), wide.begin(), wide.end());
-    const unsigned whole = 3u + (c64 ? 1u : 0u);
-    std::vector<unsigned> piv;
-    unsigned nhigh = 0;
-    while (!rest.empty())
+    std::vector<const U3Gate *> order = single;
+    order.insert(order.end(), wide.begin(), wide.end());
+    // the configured geometry, unless another needs fewer passes (wide
+    // offsets take pivots above the low bits, which the smaller contiguous
+    // runs give more slots for); among equal counts the longer runs
+    const int own = int(tile_seg(prec));
+    std::vector<std::vector<SingleGate>> best, cand;
+    int geo = own;
+    if (!plan_passes(prec, own, order, best))
     {
-        std::vector<const U3Gate *> pass, later;
-        for (const U3Gate *g : rest)
-        {
-            pass.push_back(g);
-            const bool fits = pivot_bits(pass, low, piv, &nhigh) && nhigh <= slots &&
-                              !(piv.back() < std::min(low, whole) && (g->delta >> low) != 0);
-            if (!fits)
-            {
-                pass.pop_back();
-                later.push_back(g);
-            }
-        }
-        if (pass.empty())
-        {
-            set_error("u3_apply: dependent pair offsets in one batch");
-            return EINVAL_;
-        }
-        rest.swap(later);
-        pivot_bits(pass, low, piv, &nhigh);
-        std::vector<SingleGate> gs;
-        for (size_t k = 0; k < pass.size(); k++)
-            gs.push_back({piv[k], pass[k]});
-        ZXA_TRY(run_tile_pass(ctx, state, prec, n, gs));
+        set_error("u3_apply: dependent pair offsets in one batch");
+        return EINVAL_;
     }
+    if (!wide.empty())
+        for (int g = 6; g >= 3; g--)
+            if (g != own && plan_passes(prec, g, order, cand) && cand.size() < best.size())
+            {
+                best.swap(cand);
+                geo = g;
+            }
+    for (const std::vector<SingleGate> &gs : best)
+        ZXA_TRY(run_tile_pass(ctx, state, prec, n, gs, nullptr, 0, geo));
     return OK;
 }
 
