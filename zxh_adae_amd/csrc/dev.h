@@ -316,6 +316,9 @@ int set_ones(Ctx &ctx, const uint64_t *dstates, int count, int prec);
 int gather(Ctx &ctx, const void *state, int prec, unsigned n, const uint64_t *idx, size_t count, void *out);
 // One general pair sweep (any delta / selector).
 int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
+// The U3 of a qubit expansion: delta is the state's top bit (nbits - 1) and
+// that half of the state is zero (it reads the lower half only).
+int u3_expand_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
 
 int x_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t delta, uint64_t packed_sel);
 int cx_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t delta, uint64_t sel_cq, uint64_t sel_q);

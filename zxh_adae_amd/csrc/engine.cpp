@@ -627,7 +627,11 @@ int Sim::execute()
                     eg.u[2 * i] = u[i].real();
                     eg.u[2 * i + 1] = u[i].imag();
                 }
-                ZXA_TRY(u3_pair_apply(ctx_, dstate_, prec_, active_bits(), eg));
+                // the new bit is the top one and its half still zero
+                if (eg.delta == uint64_t(1) << (active_bits() - 1))
+                    ZXA_TRY(u3_expand_apply(ctx_, dstate_, prec_, active_bits(), eg));
+                else
+                    ZXA_TRY(u3_pair_apply(ctx_, dstate_, prec_, active_bits(), eg));
                 continue;
             }
             delta &= low_mask(active_bits());

@@ -76,6 +76,37 @@ __global__ __launch_bounds__(kThreads) void u3_pair_kernel(void *__restrict__ st
     }
 }
 
+// The U3 that expands a qubit: its pair offset is the new top bit, whose half
+// of the state is still zero, so each pair reads one amplitude a and writes
+// (m00 a, m10 a) (a on the even side) or (m11 a, m01 a) (a on the odd side):
+// one read and two writes per pair instead of two and two.
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void u3_expand_kernel(void *__restrict__ state, uint64_t half,
+                                                             uint64_t packed, double u00x, double u00y, double u01x,
+                                                             double u01y, double u10x, double u10y, double u11x,
+                                                             double u11y)
+{
+    using T = typename Elem<kC64>::T;
+    using R = typename Elem<kC64>::R;
+    T *p = static_cast<T *>(state);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < half; k += stride)
+    {
+        const bool odd = sel_eval(packed, k);
+        const T a = p[k];
+        // (this amplitude's new value, its partner's) = (m00, m10) a or (m11, m01) a
+        const R sx = R(odd ? u11x : u00x), sy = R(odd ? u11y : u00y);
+        const R tx = R(odd ? u01x : u10x), ty = R(odd ? u01y : u10y);
+        T na, nb;
+        na.x = sx * a.x - sy * a.y;
+        na.y = sx * a.y + sy * a.x;
+        nb.x = tx * a.x - ty * a.y;
+        nb.y = tx * a.y + ty * a.x;
+        p[k] = na;
+        p[k + half] = nb;
+    }
+}
+
 // X / CX explicit permutations (x_block_kernel_cuda / cx_block_kernel_cuda,
 // kernel_cuda.cu:442-477): swap i (S_q(i) = 0, and S_cq(i) = 1 for CX) with
 // i ^ delta.
@@ -180,6 +211,26 @@ int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate &g)
         hipLaunchKernelGGL(u3_pair_kernel<false>, dim3(grid_for(npairs)), dim3(kThreads), 0, ctx.stream, state,
                            npairs, g.delta, pivot, g.packed_sel, g.u[0], g.u[1], g.u[2], g.u[3], g.u[4], g.u[5],
                            g.u[6], g.u[7]);
+    ZXA_HIP(hipGetLastError());
+    return ctx.toc(F_U3PAIR, bytes);
+}
+
+int u3_expand_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate &g)
+{
+    if (n == 0 || g.delta != (uint64_t(1) << (n - 1)))
+    {
+        set_error("u3 expand: the pair offset is not the state's top bit");
+        return EINVAL_;
+    }
+    const uint64_t half = uint64_t(1) << (n - 1);
+    const uint64_t bytes = 3ull * half * elem_bytes(prec);
+    ZXA_TRY(ctx.tic());
+    if (prec == C64)
+        hipLaunchKernelGGL(u3_expand_kernel<true>, dim3(grid_for(half)), dim3(kThreads), 0, ctx.stream, state, half,
+                           g.packed_sel, g.u[0], g.u[1], g.u[2], g.u[3], g.u[4], g.u[5], g.u[6], g.u[7]);
+    else
+        hipLaunchKernelGGL(u3_expand_kernel<false>, dim3(grid_for(half)), dim3(kThreads), 0, ctx.stream, state, half,
+                           g.packed_sel, g.u[0], g.u[1], g.u[2], g.u[3], g.u[4], g.u[5], g.u[6], g.u[7]);
     ZXA_HIP(hipGetLastError());
     return ctx.toc(F_U3PAIR, bytes);
 }
