@@ -310,7 +310,7 @@ ZXH_WORKLOADS = ["qft_n30", "qaoa_n30", "vqe_two_local_n30", "vqe_su2_n30", "ker
                  "kernel_h_inter_dom_n30", "kernel_h_intra_dom_n30", "bv_n30", "ghz_n32", "multiplier_n45"]
 
 
-def zxh_workloads(Z, repeats=3, breakdown=False):
+def zxh_workloads(Z, repeats=3, breakdown=False, names=None):
     """The reference's own benchmark circuits (its QASM workloads, kept as
     test fixtures) through the drop-in ZXH object, complex128: execute() time
     in the default lazy ZXH mode (X/CX as affine updates, qubits expanded on
@@ -320,7 +320,7 @@ def zxh_workloads(Z, repeats=3, breakdown=False):
     from zxh_adae_amd import kernels as K
     root = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden", "workloads")
     out = {}
-    for name in ZXH_WORKLOADS:
+    for name in names or ZXH_WORKLOADS:
         circ = F.load_qasm(os.path.join(root, name + ".qasm"))
         gs = F.to_gate_tuples(F.lower_circuit(circ))
         row = {"qubits": circ.num_qubits, "gates": len(gs)}
@@ -475,7 +475,13 @@ def main():
     ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128", "zxh_workloads"],
                     default=None,
                     help="run one extra alone, after the tile-geometry autotuning (profiling)")
+    ap.add_argument("--workloads", default=None,
+                    help="with --section zxh_workloads: comma-separated circuit names (default: all)")
     args = ap.parse_args()
+    unknown = [w for w in (args.workloads.split(",") if args.workloads else []) if w not in ZXH_WORKLOADS]
+    if unknown:
+        print(f"unknown workloads {unknown}; known: {ZXH_WORKLOADS}", file=sys.stderr)
+        sys.exit(2)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine is single-GPU by design",
               file=sys.stderr)
@@ -505,7 +511,7 @@ def main():
         elif args.section == "c5_c64":
             out = c5_circuit(W, K, "c64", 33)
         elif args.section == "zxh_workloads":
-            out = zxh_workloads(Z, breakdown=True)
+            out = zxh_workloads(Z, breakdown=True, names=args.workloads.split(",") if args.workloads else None)
         else:
             out = c5_circuit(W, K, "c128", 32)
         print(json.dumps({args.section: out, "autotune": tuned}), flush=True)

@@ -277,3 +277,28 @@ def test_open_batch_on_reference_workloads(gpu_engine, name):
         counts[policy] = _swept_bytes(s)
     print(name, counts, counts["reference"] / counts["fused"])
     assert counts["fused"] * 3 <= counts["reference"]
+
+
+SOUP_KINDS = (W.H, W.U3, W.P, W.CP, W.RZ, W.Z, W.X, W.CX, W.CRZ, W.RX)
+
+
+@pytest.mark.parametrize("n,count,seed", [(13, 400, 71), (14, 600, 72), (16, 500, 73), (17, 300, 74)])
+@pytest.mark.parametrize("mode", ["lazy", "nox", "exp"])
+def test_random_soups_through_the_scheduler(gpu_engine, n, count, seed, mode):
+    """Barrier-free random circuits over every gate kind, wide enough for tile
+    passes: the open batch, wide pair offsets, per-pair flips, cleaned
+    fillers and two-bit polynomial windows all meet here.  Both policies
+    against the oracle (state, width and map); complex64 within 2e-5."""
+    Z = gpu_engine
+    dx, eager = {"lazy": (False, False), "nox": (True, False), "exp": (False, True)}[mode]
+    gs = W.random_mixed(n, count, seed, kinds=SOUP_KINDS)
+    o = O.OracleSim(n, dx, eager)
+    o.gates(gs)
+    o.execute()
+    want = o.get_state()
+    for policy in POLICIES:
+        s = make_sim(Z, n, dx, eager, gs, policy=policy)
+        assert s.required_M() == o.required_M()
+        assert np.max(np.abs(s.get_state_array() - want)) < TOL128, policy
+    s = make_sim(Z, n, dx, eager, gs, prec="c64")
+    assert np.max(np.abs(s.get_state_array() - want)) < 2e-5
