@@ -280,6 +280,31 @@ def test_open_batch_on_reference_workloads(gpu_engine, name):
 
 
 SOUP_KINDS = (W.H, W.U3, W.P, W.CP, W.RZ, W.Z, W.X, W.CX, W.CRZ, W.RX)
+DIAG_SOUP_KINDS = (W.H, W.U3, W.P, W.CP, W.RZ, W.Z, W.CRZ, W.RX)
+
+
+@pytest.mark.parametrize("n,count,seed", [(13, 500, 81), (16, 700, 82), (19, 400, 83)])
+def test_plain_soups_with_fans(gpu_engine, n, count, seed):
+    """Plain method, single-qubit and diagonal gates only: every diagonal
+    gate that sits between two gates on its qubits becomes a fan, in passes
+    that keep circuit order (low and high targets mixed).  Both policies and
+    both geometries' extremes against the oracle."""
+    from zxh_adae_amd import autotune as A
+    Z = gpu_engine
+    gs = W.random_mixed(n, count, seed, kinds=DIAG_SOUP_KINDS)
+    o = O.OracleSim(n, False, True)
+    o.gates(gs)
+    o.execute()
+    want = o.get_state()
+    old = A.tune_get("tile.seg.c128")
+    try:
+        for seg in (3, 6):
+            A.tune_set("tile.seg.c128", seg)
+            for policy in POLICIES:
+                s = make_sim(Z, n, False, True, gs, policy=policy)
+                assert np.max(np.abs(s.get_state_array() - want)) < TOL128, (seg, policy)
+    finally:
+        A.tune_set("tile.seg.c128", old)
 
 
 @pytest.mark.parametrize("n,count,seed", [(13, 400, 71), (14, 600, 72), (16, 500, 73), (17, 300, 74)])
@@ -302,3 +327,57 @@ def test_random_soups_through_the_scheduler(gpu_engine, n, count, seed, mode):
         assert np.max(np.abs(s.get_state_array() - want)) < TOL128, policy
     s = make_sim(Z, n, dx, eager, gs, prec="c64")
     assert np.max(np.abs(s.get_state_array() - want)) < 2e-5
+
+
+def textbook_qft(n, seed=None):
+    """The QFT in the order of the reference's qft workload: for t from the
+    top qubit down, the controlled phases from every higher qubit onto t,
+    then H(t).  Each H then depends on the phases before it (no two commute
+    past each other).  With a seed, random U3 gates replace the H gates."""
+    rng = np.random.default_rng(seed) if seed is not None else None
+    gs = []
+    for t in range(n - 1, -1, -1):
+        for k in range(n - 1, t, -1):
+            gs.append((W.CP, k, t, math.pi / 2 ** (k - t), 0, 0))
+        if rng is None:
+            gs.append((W.H, 0, t, 0, 0, 0))
+        else:
+            a, b, c = (float(x) for x in rng.uniform(-math.pi, math.pi, 3))
+            gs.append((W.U3, 0, t, a, b, c))
+    return gs
+
+
+@pytest.mark.parametrize("n,seed,seg", [(16, None, 6), (20, None, 4), (18, 5, 3), (21, 6, 6), (22, None, 5)])
+def test_fans_let_dependent_gates_share_a_pass(gpu_engine, n, seed, seg):
+    """Plain method, complex128: the controlled phases between two H gates
+    that involve the later one's target become that gate's fan, so up to one
+    pass per tile-slot group instead of one pass per gate.  Parity against
+    the oracle and the reference policy; complex64 (no fans) as well."""
+    from zxh_adae_amd import autotune as A
+    Z = gpu_engine
+    gs = W.hadamard_all(n) + [(W.BARRIER, 0, 0, 0, 0, 0)] + textbook_qft(n, seed)
+    o = O.OracleSim(n, False, True)
+    o.gates(gs)
+    o.execute()
+    want = o.get_state()
+    passes = {}
+    old = A.tune_get("tile.seg.c128")
+    try:
+        A.tune_set("tile.seg.c128", seg)
+        for policy in POLICIES:
+            s = Z.ZXH(n, False, True, precision="c128", policy=policy)
+            s.gates(gs)
+            s.set_timing(True)
+            s.execute()
+            s.set_timing(False)
+            assert np.max(np.abs(s.get_state_array() - want)) < TOL128, policy
+            passes[policy] = sum(s.timing(f)["launches"] for f in ("u3_tile", "fused", "diag"))
+    finally:
+        A.tune_set("tile.seg.c128", old)
+    print(n, seed, seg, passes)
+    # the H layer, then one pass per slot group of the QFT (the low targets
+    # fill the contiguous run in circuit order)
+    assert passes["fused"] <= 2 + -(-n // (12 - seg)) + 1
+    assert passes["fused"] * 2 < passes["reference"]
+    c64 = make_sim(Z, n, False, True, gs, prec="c64")
+    assert np.max(np.abs(c64.get_state_array() - want)) < 2e-5

@@ -161,6 +161,7 @@ void Ctx::release()
     scratch_d.release();
     scratch_e.release();
     scratch_f.release();
+    scratch_g.release();
     (void)collect();
     for (hipEvent_t e : pool_)
         (void)hipEventDestroy(e);

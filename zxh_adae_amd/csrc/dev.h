@@ -134,6 +134,7 @@ struct Ctx
     DevBuf scratch_d; // sampler per-segment tails
     DevBuf scratch_e; // observable per-workgroup partial sums
     DevBuf scratch_f; // adjoint per-wave transition accumulators
+    DevBuf scratch_g; // tile-pass per-tile fan factors
     bool timing = false;
     FamilyTiming fam[F_COUNT];
     // sampler state between prepare and search (tree in scratch_b, tails in scratch_d)
@@ -215,14 +216,30 @@ unsigned tile_seg(int prec, int geometry = -1);
 unsigned tile_low_bits(int prec, int geometry = -1);
 unsigned tile_upper_slots(int prec, int geometry = -1);
 unsigned tile_min_bits(int prec);
+// True when single-bit targets in this (circuit) order fit one staged tile
+// pass of a 2^n complex128 state: the order a pass with fans keeps.
+bool tile_staged_fits(int prec, unsigned n, const std::vector<unsigned> &targets, int geometry = -1);
 // Applies commuting gates on distinct targets (a fused U3 cluster).
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates);
-// One tile pass applying commuting single-bit U3 gates (alpha == delta, any
-// number of targets below tile_low_bits and at most tile_upper_slots at or
-// above), then a diagonal window whose selectors have at most one alpha bit,
-// in one sweep.
+// A phase applied just before a gate of a tile pass to the amplitudes whose
+// target bit is one: exp(i (c + sum_k theta[k] x_k)), theta over the state's
+// address bits (theta[target] is ignored).  A diagonal window that lies
+// between two single-bit gates and involves the later one's target becomes
+// that gate's fan, so the gates can share a pass.
+struct GateFan
+{
+    bool on = false;
+    double c = 0.0;
+    std::vector<double> theta;
+};
+// One tile pass applying single-bit U3 gates (alpha == delta, any number of
+// targets below tile_low_bits and at most tile_upper_slots at or above),
+// then a diagonal window whose selectors have at most one alpha bit, in one
+// sweep.  Without fans the gates must commute; with fans (complex128, every
+// target at or above tile_low_bits) they are applied in the given order,
+// each after its fan.
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
-                     const DiagGate *diag, size_t ndiag, int geometry = -1);
+                     const DiagGate *diag, size_t ndiag, int geometry = -1, const GateFan *fans = nullptr);
 // One target of an adjoint pass: the uncompute matrix m (= U^dagger of the
 // forward pass's fused single-qubit gate on this qubit, (re, im) x 4), up to
 // two transition operators a[k] (2x2 traceless Hermitian; zero when unused)

@@ -347,12 +347,72 @@ int Sim::flush_pass()
     if (pass_u3_.empty() && pass_diag_.empty())
         return OK;
     int rc = fused_pass_apply(ctx_, dstate_, prec_, active_bits(), pass_u3_.data(), pass_u3_.size(),
-                              pass_diag_.data(), pass_diag_.size());
+                              pass_diag_.data(), pass_diag_.size(), -1,
+                              pass_has_fan_ ? pass_fan_.data() : nullptr);
     pass_u3_.clear();
+    pass_fan_.clear();
     pass_diag_.clear();
     pass_support_ = 0;
     pass_high_ = 0;
+    pass_has_fan_ = false;
     return rc;
+}
+
+// Moves the pending window's terms that involve address bit t into a fan
+// for the next gate on t (applied just before it), when they are all of
+// the form theta [x_t] L with L = 1, a constant selector or [x_k ^ beta]:
+// theta x_t (beta + (1 - 2 beta) x_k).  False (nothing moved) otherwise.
+bool Sim::take_fan(unsigned t, GateFan &fan)
+{
+    const uint64_t amask = (uint64_t(1) << 63) - 1;
+    const uint64_t bit = uint64_t(1) << t;
+    const unsigned n = active_bits();
+    fan = GateFan();
+    fan.theta.assign(n, 0.0);
+    std::vector<DiagGate> keep;
+    for (const DiagGate &d : pass_diag_)
+    {
+        const uint64_t a0 = d.s0 & amask;
+        const uint64_t a1 = d.s1 == kNoSel ? 0 : (d.s1 & amask);
+        if (!((a0 | a1) & bit))
+        {
+            keep.push_back(d);
+            continue;
+        }
+        // the selector on t, and the other factor
+        const bool on0 = (a0 & bit) != 0;
+        const uint64_t st = on0 ? d.s0 : d.s1;
+        if ((st & amask) != bit || (st >> 63) != 0)
+            return false;
+        if (d.s1 == kNoSel)
+        {
+            fan.c += d.theta;
+            continue;
+        }
+        const uint64_t so = on0 ? d.s1 : d.s0;
+        const uint64_t ao = so & amask;
+        const double beta = (so >> 63) ? 1.0 : 0.0;
+        if (ao == bit || ao == 0)
+        {
+            // [x_t][x_t] = x_t; a constant selector is its beta
+            fan.c += ao == bit ? (beta == 0.0 ? d.theta : 0.0) : d.theta * beta;
+            continue;
+        }
+        if ((ao & (ao - 1)) != 0 || (ao >> n) != 0)
+            return false;
+        fan.c += d.theta * beta;
+        fan.theta[size_t(__builtin_ctzll(ao))] += d.theta * (1.0 - 2.0 * beta);
+    }
+    fan.on = true;
+    pass_diag_.swap(keep);
+    pass_support_ = 0;
+    for (const DiagGate &d : pass_diag_)
+    {
+        pass_support_ |= d.s0 & amask;
+        if (d.s1 != kNoSel)
+            pass_support_ |= d.s1 & amask;
+    }
+    return true;
 }
 
 // Adds single-bit U3 batches to the pending pass when exact: a gate joins
@@ -373,12 +433,21 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
         ZXA_TRY(flush_pass());
         return u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size());
     }
-    for (const U3Gate &g : descs)
-        if (g.delta & pass_support_)
-        {
-            ZXA_TRY(flush_pass());
-            break;
-        }
+    const unsigned low = tile_low_bits(prec_);
+    const unsigned n = active_bits();
+    // Complex128 (X and CX as affine updates): a gate whose target the
+    // pending window touches may still join the pass, the window's terms on
+    // its target becoming its fan.  A pass with fans keeps its gates in
+    // circuit order (tile_staged_fits).  Otherwise a gate the window touches
+    // starts a new pass.
+    const bool can_fan = prec_ == C128 && !disable_x_;
+    auto order_with = [&](unsigned t) {
+        std::vector<unsigned> v;
+        for (const U3Gate &p : pass_u3_)
+            v.push_back(unsigned(__builtin_ctzll(p.delta)));
+        v.push_back(t);
+        return v;
+    };
     for (const U3Gate &g0 : descs)
     {
         U3Gate g = g0;
@@ -389,28 +458,55 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
             std::memcpy(g.u, sw, sizeof(sw));
             g.packed_sel = g.delta;
         }
-        U3Gate *same = nullptr;
-        for (U3Gate &o : pass_u3_)
-            if (o.delta == g.delta)
-                same = &o;
-        if (same)
+        const unsigned t = unsigned(__builtin_ctzll(g.delta));
+        int same = -1;
+        for (size_t k = 0; k < pass_u3_.size(); k++)
+            if (pass_u3_[k].delta == g.delta)
+                same = int(k);
+        // a later gate's fan on t lies between the two gates on t
+        bool hits = (g.delta & pass_support_) != 0;
+        if (same >= 0)
+            for (size_t k = size_t(same) + 1; k < pass_fan_.size(); k++)
+                hits = hits || (pass_fan_[k].on && pass_fan_[k].theta[t] != 0.0);
+        GateFan fan;
+        if (hits)
         {
+            bool ok = can_fan && same < 0 && tile_staged_fits(prec_, n, order_with(t));
+            ok = ok && take_fan(t, fan);
+            if (!ok)
+            {
+                ZXA_TRY(flush_pass());
+                same = -1;
+                fan = GateFan();
+            }
+        }
+        if (same >= 0)
+        {
+            U3Gate *o = &pass_u3_[size_t(same)];
             const std::complex<double> a[4] = {{g.u[0], g.u[1]}, {g.u[2], g.u[3]}, {g.u[4], g.u[5]}, {g.u[6], g.u[7]}};
-            const std::complex<double> b[4] = {{same->u[0], same->u[1]}, {same->u[2], same->u[3]},
-                                               {same->u[4], same->u[5]}, {same->u[6], same->u[7]}};
+            const std::complex<double> b[4] = {{o->u[0], o->u[1]}, {o->u[2], o->u[3]},
+                                               {o->u[4], o->u[5]}, {o->u[6], o->u[7]}};
             const std::complex<double> c[4] = {a[0] * b[0] + a[1] * b[2], a[0] * b[1] + a[1] * b[3],
                                                a[2] * b[0] + a[3] * b[2], a[2] * b[1] + a[3] * b[3]};
             for (int i = 0; i < 4; i++)
             {
-                same->u[2 * i] = c[i].real();
-                same->u[2 * i + 1] = c[i].imag();
+                o->u[2 * i] = c[i].real();
+                o->u[2 * i + 1] = c[i].imag();
             }
             continue;
         }
-        const bool high = g.delta >> tile_low_bits(prec_) != 0;
-        if (high && unsigned(pass_high_) >= tile_upper_slots(prec_))
+        const bool high = t >= low;
+        const bool fits = (pass_has_fan_ || fan.on) ? tile_staged_fits(prec_, n, order_with(t))
+                                                    : !(high && unsigned(pass_high_) >= tile_upper_slots(prec_));
+        if (!fits)
+        {
+            // (a fan was taken only where the staged order fits)
             ZXA_TRY(flush_pass());
+            fan = GateFan();
+        }
         pass_u3_.push_back(g);
+        pass_fan_.push_back(fan);
+        pass_has_fan_ = pass_has_fan_ || fan.on;
         if (high)
             pass_high_++;
     }
@@ -482,9 +578,11 @@ int Sim::execute()
     }
 
     pass_u3_.clear();
+    pass_fan_.clear();
     pass_diag_.clear();
     pass_support_ = 0;
     pass_high_ = 0;
+    pass_has_fan_ = false;
 
     // Scheduling.  The reference policy batches each run of consecutive U3
     // gates and flushes the diagonal window when a run starts (zxh.cpp:440-470).

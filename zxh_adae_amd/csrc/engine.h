@@ -156,12 +156,16 @@ class Sim
     std::vector<uint64_t> real_results_;
     std::vector<uint64_t> packed_;
     size_t result_count_ = 0;
-    // pending fused pass: single-bit U3 gates, then a diagonal window
+    // pending fused pass: single-bit U3 gates (each after its fan, if any),
+    // then a diagonal window
     int policy_ = POLICY_FUSED;
     std::vector<U3Gate> pass_u3_;
+    std::vector<GateFan> pass_fan_;
     std::vector<DiagGate> pass_diag_;
     uint64_t pass_support_ = 0;
     int pass_high_ = 0;
+    bool pass_has_fan_ = false;
+    bool take_fan(unsigned t, GateFan &fan);
     Ctx ctx_;
 };
 

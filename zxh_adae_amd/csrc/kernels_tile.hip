@@ -76,8 +76,20 @@ struct TGate
     uint32_t a0;    // complex64: selector bit 0 (differs between a unit's two amplitudes)
     uint32_t beta;
     uint32_t rpar;  // bit r: parity of amask over register r's offset in the gate's layout
-    uint32_t pad;
+    uint32_t fan;   // 1: the pair's second element takes a phase first (see TFan)
 };
+
+// A gate's fan (complex128 passes): before the gate, every amplitude whose
+// target bit is one is multiplied by exp(i phi(x)), phi linear in the other
+// address bits.  exp(i phi) = F(tile) * prod_{thread bits j set} thr[j] *
+// reg[r]: F per tile (fan_phase_kernel), thr over the thread bits and reg
+// over the register offsets of the gate's layout.
+struct TFan
+{
+    double thr[8][2];
+    double reg[kRegs][2];
+};
+constexpr int kFanStride = 2 * (kUnitBits + 1); // doubles of per-tile factors per tile
 
 // Kernel arguments of a pass.  Unit bits 0..seg-1 are unit-address bits
 // 0..seg-1 (seg = 3..6, the tile geometry); the host assigns the others
@@ -92,6 +104,7 @@ struct TPass
     uint64_t ntiles;
     uint64_t tmask; // unit-address mask of the 12 pivots
     uint64_t gstep; // the grid size deposited into the non-tile bits
+    const double *fphase; // fan passes: per-tile factors, kFanStride doubles per tile
 };
 
 // Per-pass tables in device memory.  The kernel re-reads them through the
@@ -103,6 +116,7 @@ struct TTab
 {
     uint64_t roff[kRegs]; // layout-0 register offsets (unit addresses)
     TGate g[kUnitBits + 1];
+    TFan fan[kUnitBits + 1];
 };
 
 // Register bits of the three layouts (unit bits); the thread bits are the
@@ -262,15 +276,45 @@ __device__ __forceinline__ void unit_pair2(typename UnitT<kC64>::T &a, typename 
 
 // Gates on the register bits of layout L (and, in layout 0, the complex64
 // in-unit gate).  base ^ toff is the unit address of the thread's register 0.
-template <bool kC64, bool kFlip, int L>
+template <bool kC64, bool kFlip, int L, bool kFan = false>
 __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs], const TPass &P, ZXA_CONST TTab *G,
-                                             uint64_t base, uint64_t toff)
+                                             uint64_t base, uint64_t toff, int t = 0, ZXA_CONST double *fp = nullptr)
 {
 #pragma unroll
     for (int i = 0; i < 4; i++)
     {
         const int k = reg_bit(L, i);
         ZXA_CONST TGate &g = G->g[k];
+        if constexpr (kFan && !kC64)
+        {
+            if (g.fan != 0u)
+            {
+                // the gate's fan: exp(i phi) on the second element of each pair
+                ZXA_CONST TFan &f = G->fan[k];
+                double fx = fp[2 * k], fy = fp[2 * k + 1];
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                {
+                    const bool on = ((t >> j) & 1) != 0;
+                    const double ax = on ? f.thr[j][0] : 1.0, ay = on ? f.thr[j][1] : 0.0;
+                    const double nx = fx * ax - fy * ay;
+                    fy = fx * ay + fy * ax;
+                    fx = nx;
+                }
+#pragma unroll
+                for (int r = 0; r < kRegs; r++)
+                {
+                    if ((r >> i) & 1)
+                        continue;
+                    const double px = fx * f.reg[r][0] - fy * f.reg[r][1];
+                    const double py = fx * f.reg[r][1] + fy * f.reg[r][0];
+                    dvec2 &b = v[r | (1 << i)];
+                    const double bx = b.x * px - b.y * py;
+                    const double by = b.x * py + b.y * px;
+                    b = dvec2{bx, by};
+                }
+            }
+        }
         Real<kC64> m[8];
         load_matrix<kC64>(m, g);
         const uint64_t amask = kFlip ? g.amask : 0;
@@ -469,20 +513,23 @@ __device__ __forceinline__ void store_tile(const typename UnitT<kC64>::T (&v)[kR
 }
 
 // Gates, transposes and the diagonal epilogue of one tile held in v.
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false>
 __device__ __forceinline__ void tile_work(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds,
                                           const TPass &P, ZXA_CONST TTab *G, const double *tphase,
                                           const Epi<kC64> &epi, int t, uint64_t toff, uint64_t tile, uint64_t base)
 {
-    apply_layout<kC64, kFlip, 0>(v, P, G, base, toff);
+    ZXA_CONST double *fp = nullptr;
+    if constexpr (kFan)
+        fp = (ZXA_CONST double *)(P.fphase + tile * kFanStride);
+    apply_layout<kC64, kFlip, 0, kFan>(v, P, G, base, toff, t, fp);
     if constexpr (kL1)
     {
         transpose<kC64, 0, 1>(v, lds, t);
-        apply_layout<kC64, kFlip, 1>(v, P, G, base, thr_off<1>(t, P));
+        apply_layout<kC64, kFlip, 1, kFan>(v, P, G, base, thr_off<1>(t, P), t, fp);
         if constexpr (kL2)
         {
             transpose<kC64, 1, 2>(v, lds, t);
-            apply_layout<kC64, kFlip, 2>(v, P, G, base, thr_off<2>(t, P));
+            apply_layout<kC64, kFlip, 2, kFan>(v, P, G, base, thr_off<2>(t, P), t, fp);
             transpose<kC64, 2, 0>(v, lds, t);
         }
         else
@@ -493,7 +540,7 @@ __device__ __forceinline__ void tile_work(typename UnitT<kC64>::T (&v)[kRegs], t
     else if constexpr (kL2)
     {
         transpose<kC64, 0, 2>(v, lds, t);
-        apply_layout<kC64, kFlip, 2>(v, P, G, base, thr_off<2>(t, P));
+        apply_layout<kC64, kFlip, 2, kFan>(v, P, G, base, thr_off<2>(t, P), t, fp);
         transpose<kC64, 2, 0>(v, lds, t);
     }
     if constexpr (kDiag)
@@ -517,7 +564,7 @@ __device__ __forceinline__ ZXA_CONST TTab *tables(const TTab *tabp)
 // that waiting for a tile's loads never waits for the previous stores.  The
 // prefetch is unconditional (past the last tile it re-reads the current one):
 // with no branch around it the compiler's wait counts stay exact.
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void tile_kernel(
     void *__restrict__ state, const TPass P, const TTab *__restrict__ tabp, const double *__restrict__ ttab,
     const double *__restrict__ tphase)
@@ -541,7 +588,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     bool m1 = t1 < P.ntiles;
     uint64_t b1 = next_base(b0, P);
     load_tile<kC64>(b, p, (m1 ? b1 : b0) ^ toff, G);
-    tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t0, b0);
+    tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t0, b0);
     for (;;)
     {
         G = tables(tabp);
@@ -552,7 +599,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const bool m2 = t2 < P.ntiles;
         const uint64_t b2 = next_base(b1, P);
         load_tile<kC64>(a, p, (m2 ? b2 : b1) ^ toff, G);
-        tile_work<kC64, kL1, kL2, kDiag, kFlip>(b, lds, P, G, tphase, epi, t, toff, t1, b1);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(b, lds, P, G, tphase, epi, t, toff, t1, b1);
 
         G = tables(tabp);
         store_tile<kC64>(b, p, b1 ^ toff, G);
@@ -562,7 +609,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const bool m3 = t3 < P.ntiles;
         const uint64_t b3 = next_base(b2, P);
         load_tile<kC64>(b, p, (m3 ? b3 : b2) ^ toff, G);
-        tile_work<kC64, kL1, kL2, kDiag, kFlip>(a, lds, P, G, tphase, epi, t, toff, t2, b2);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t2, b2);
 
         t0 = t2;
         b0 = b2;
@@ -1084,13 +1131,13 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t tiles)
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
 }
 
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip>
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false>
 int launch_t(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const double *ttab, const double *tphase)
 {
-    // register buffers per workgroup (knob tile.bufs.<prec>); the flip
-    // variants keep two (their tables take the registers of a third)
-    auto k = tile_kernel<kC64, kL1, kL2, kDiag, kFlip>;
-    if constexpr (!kFlip)
+    // register buffers per workgroup (knob tile.bufs.<prec>); the flip and
+    // fan variants keep two (their tables take the registers of a third)
+    auto k = tile_kernel<kC64, kL1, kL2, kDiag, kFlip, kFan>;
+    if constexpr (!kFlip && !kFan)
         if (tune_get(kC64 ? "tile.bufs.c64" : "tile.bufs.c128") == 3)
             k = tile_kernel3<kC64, kL1, kL2, kDiag, false>;
     const int grid = persistent_grid(ctx, k, P0.ntiles);
@@ -1100,23 +1147,41 @@ int launch_t(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const doub
     return hip_check(hipGetLastError(), "tile_kernel launch");
 }
 
-template <bool kC64, bool kDiag, bool kFlip>
+template <bool kC64, bool kDiag, bool kFlip, bool kFan = false>
 int launch_l(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, const TTab *tab, const double *ttab,
              const double *tphase)
 {
     if (l1 && l2)
-        return launch_t<kC64, true, true, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
+        return launch_t<kC64, true, true, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
     if (l1)
-        return launch_t<kC64, true, false, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
+        return launch_t<kC64, true, false, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
     if (l2)
-        return launch_t<kC64, false, true, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
-    return launch_t<kC64, false, false, kDiag, kFlip>(ctx, state, P, tab, ttab, tphase);
+        return launch_t<kC64, false, true, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
+    return launch_t<kC64, false, false, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
 }
 
+// fan: complex128 passes without flips only (P.fphase set)
 template <bool kC64>
 int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip, const TTab *tab,
-             const double *ttab, const double *tphase)
+             const double *ttab, const double *tphase, bool fan = false)
 {
+    if constexpr (!kC64)
+        if (fan)
+        {
+            if (flip)
+            {
+                set_error("tile pass: fans with per-pair selectors");
+                return EINVAL_;
+            }
+            if (ttab)
+                return launch_l<false, true, false, true>(ctx, state, P, l1, l2, tab, ttab, tphase);
+            return launch_l<false, false, false, true>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
+        }
+    if (fan)
+    {
+        set_error("tile pass: fans need a complex128 state");
+        return EINVAL_;
+    }
     if (ttab)
         return launch_l<kC64, true, false>(ctx, state, P, l1, l2, tab, ttab, tphase);
     if (flip)
@@ -1149,7 +1214,52 @@ struct Plan
     std::vector<int> slot;      // per gate: unit bit, or kIntra
 };
 
-bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl, int geometry = -1)
+// Staged unit bits (complex128, single-bit targets in circuit order): the
+// gates take unit bits in the kernel's processing order, layout 0's
+// register bits 8..11, then layout 1's 4..7, then layout 2's 0..3.  A target
+// at or above the low bits needs a unit bit at or above seg, a low target one
+// below: the contiguous run may hold its address bits in any order, since a
+// wave still reads the same 64 contiguous units.  Earliest fit is optimal;
+// false if the order cannot be kept.  The unused run bits take the unused
+// low address bits; other unit bits stay -1.
+bool staged_units(int prec, int geometry, const std::vector<unsigned> &targets, std::vector<int> &unit_addr)
+{
+    if (prec == C64)
+        return false;
+    const int seg = int(tile_seg(prec, geometry));
+    static const int order[kUnitBits] = {8, 9, 10, 11, 4, 5, 6, 7, 0, 1, 2, 3};
+    unit_addr.assign(kUnitBits, -1);
+    std::vector<bool> low_used(size_t(seg), false);
+    int pos = 0;
+    for (unsigned t : targets)
+    {
+        const bool up = t >= unsigned(seg);
+        while (pos < kUnitBits && (order[pos] >= seg) != up)
+            pos++;
+        if (pos == kUnitBits)
+            return false;
+        unit_addr[order[pos]] = int(t);
+        if (!up)
+            low_used[t] = true;
+        pos++;
+    }
+    int next = 0;
+    for (int k = 0; k < seg; k++)
+    {
+        if (unit_addr[k] >= 0)
+            continue;
+        while (low_used[size_t(next)])
+            next++;
+        unit_addr[k] = next;
+        low_used[size_t(next)] = true;
+    }
+    return true;
+}
+
+// staged: the gates' order is the circuit's (fan passes), unit bits from
+// staged_units.
+bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl, int geometry = -1,
+               bool staged = false)
 {
     const bool c64 = prec == C64;
     const int seg = int(tile_seg(prec, geometry));
@@ -1165,19 +1275,34 @@ bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl
     for (const SingleGate &s : gs)
         if (s.target >= low)
             upper.push_back(s.target);
-    std::sort(upper.begin(), upper.end());
-    if (upper.size() > tile_upper_slots(prec, geometry) || n < tile_min_bits(prec))
+    if (n < tile_min_bits(prec))
         return false;
-    // targets: layout-0 register bits, then layout-1 register bits from the
-    // top, then (geometry 3) layout 2's register bit 3
-    std::vector<int> order;
-    for (int k : {8, 9, 10, 11, 7, 6, 5, 4, 3})
-        if (k >= seg)
-            order.push_back(k);
-    for (size_t u = 0; u < upper.size(); u++)
+    if (staged)
     {
-        unit_addr[order[u]] = int(upper[u]);
-        used[upper[u]] = true;
+        std::vector<unsigned> targets;
+        for (const SingleGate &s : gs)
+            targets.push_back(s.target);
+        if (!staged_units(prec, geometry, targets, unit_addr))
+            return false;
+        for (unsigned t : upper)
+            used[t] = true;
+    }
+    else
+    {
+        std::sort(upper.begin(), upper.end());
+        if (upper.size() > tile_upper_slots(prec, geometry))
+            return false;
+        // targets: layout-0 register bits, then layout-1 register bits from
+        // the top, then (geometry 3) layout 2's register bit 3
+        std::vector<int> order;
+        for (int k : {8, 9, 10, 11, 7, 6, 5, 4, 3})
+            if (k >= seg)
+                order.push_back(k);
+        for (size_t u = 0; u < upper.size(); u++)
+        {
+            unit_addr[order[u]] = int(upper[u]);
+            used[upper[u]] = true;
+        }
     }
     unsigned next = low;
     for (int f = seg; f < kUnitBits; f++)
@@ -1355,15 +1480,121 @@ void fill_pass(int prec, unsigned n, const Plan &pl, TPass &P, TTab &G, bool use
         use[slot == kIntra ? 0 : layout_of_unit_bit(slot)] = true;
 }
 
+// Per-tile factors of the fans: out[tile * kFanStride + 2 s] = exp(i (c_s +
+// sum_i theta_s[outer i] * tile bit i)), one thread per tile; tab holds, per
+// slot s < kUnitBits + 1, c_s then the nh outer coefficients.
+__global__ __launch_bounds__(kThreads) void fan_phase_kernel(const double *__restrict__ tab, int nh, uint64_t ntiles,
+                                                             double *__restrict__ out)
+{
+    const uint64_t tile = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (tile >= ntiles)
+        return;
+    for (int sl = 0; sl < kUnitBits + 1; sl++)
+    {
+        const double *row = tab + size_t(sl) * (1 + nh);
+        double ph = row[0];
+        for (int i = 0; i < nh; i++)
+            if ((tile >> i) & 1u)
+                ph += row[1 + i];
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        out[tile * kFanStride + 2 * sl] = cs;
+        out[tile * kFanStride + 2 * sl + 1] = sn;
+    }
+}
+
+// Fan tables of a complex128 pass (single-bit slots, planned staged): each
+// fan gate's thread and register factors in its layout, and the per-tile
+// factors over the outer bits (fan_phase_kernel into scratch_g).
+int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const GateFan *fans, const Plan &pl,
+               TPass &P, TTab &G)
+{
+    std::vector<int> outer;
+    for (unsigned b = 0; b < n; b++)
+        if (std::find(pl.ebit.begin(), pl.ebit.end(), b) == pl.ebit.end())
+            outer.push_back(int(b));
+    const int nh = int(outer.size());
+    std::vector<double> tab(size_t(kUnitBits + 1) * (1 + nh), 0.0);
+    auto cis = [](double a, double *o) {
+        o[0] = std::cos(a);
+        o[1] = std::sin(a);
+    };
+    for (size_t gi = 0; gi < gs.size(); gi++)
+    {
+        if (!fans[gi].on)
+            continue;
+        const int k = pl.slot[gi];
+        const int lay = layout_of_unit_bit(k);
+        int own = -1;
+        for (int i = 0; i < 4; i++)
+            if (reg_bit(lay, i) == k)
+                own = i;
+        const std::vector<double> &th = fans[gi].theta;
+        TFan &f = G.fan[k];
+        for (int j = 0; j < 8; j++)
+            cis(th[P.ub[thr_bit(lay, j)]], f.thr[j]);
+        for (int r = 0; r < kRegs; r++)
+        {
+            double a = 0.0;
+            for (int i = 0; i < 4; i++)
+                if (i != own && ((r >> i) & 1))
+                    a += th[P.ub[reg_bit(lay, i)]];
+            cis(a, f.reg[r]);
+        }
+        G.g[k].fan = 1u;
+        double *row = tab.data() + size_t(k) * (1 + nh);
+        row[0] = fans[gi].c;
+        for (int i = 0; i < nh; i++)
+            row[1 + i] = th[size_t(outer[i])];
+    }
+    void *dt = nullptr;
+    ZXA_TRY(ctx.ring.stage(tab.data(), tab.size() * sizeof(double), ctx.stream, &dt));
+    ZXA_TRY(ctx.scratch_g.ensure(P.ntiles * kFanStride * sizeof(double)));
+    const unsigned grid = unsigned((P.ntiles + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(fan_phase_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<const double *>(dt),
+                       nh, P.ntiles, static_cast<double *>(ctx.scratch_g.ptr));
+    ZXA_HIP(hipGetLastError());
+    P.fphase = static_cast<const double *>(ctx.scratch_g.ptr);
+    return OK;
+}
+
 // One tile pass over the given targets, then the diagonal window (if any).
 int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
-                  const DiagGate *diag = nullptr, size_t ndiag = 0, int geometry = -1)
+                  const DiagGate *diag = nullptr, size_t ndiag = 0, int geometry = -1,
+                  const GateFan *fans = nullptr)
 {
     const bool c64 = prec == C64;
-    Plan pl;
-    if (!plan_tile(prec, n, gs, pl, geometry))
+    bool any_fan = false;
+    if (fans)
+        for (size_t gi = 0; gi < gs.size(); gi++)
+            any_fan = any_fan || fans[gi].on;
+    if (any_fan)
     {
-        set_error("tile pass: state narrower than a tile or too many targets");
+        if (c64)
+        {
+            set_error("tile pass: fans need a complex128 state");
+            return EINVAL_;
+        }
+        for (size_t gi = 0; gi < gs.size(); gi++)
+        {
+            const uint64_t alpha = gs[gi].g->packed_sel & ((uint64_t(1) << 63) - 1);
+            if (alpha != gs[gi].g->delta || (gs[gi].g->delta & (gs[gi].g->delta - 1)) != 0)
+            {
+                set_error("tile pass: fans need single-bit selectors equal to the pair offsets");
+                return EINVAL_;
+            }
+            if (fans[gi].on && fans[gi].theta.size() != n)
+            {
+                set_error("tile pass: a fan needs one coefficient per address bit");
+                return EINVAL_;
+            }
+        }
+    }
+    Plan pl;
+    if (!plan_tile(prec, n, gs, pl, geometry, any_fan))
+    {
+        set_error(any_fan ? "tile pass: the gates of a pass with fans do not fit the kernel's order"
+                          : "tile pass: state narrower than a tile or too many targets");
         return EINVAL_;
     }
     const std::vector<unsigned> &pos = pl.ebit;
@@ -1486,6 +1717,8 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         }
         ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase));
     }
+    if (any_fan)
+        ZXA_TRY(setup_fans(ctx, n, gs, fans, pl, P, G));
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     const TTab *tab = static_cast<const TTab *>(dtab);
@@ -1494,8 +1727,8 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     if (c64)
         ZXA_TRY(launch_p<true>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
     else
-        ZXA_TRY(launch_p<false>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
-    ZXA_TRY(ctx.toc(ndiag > 0 ? F_FUSED : F_U3TILE, bytes));
+        ZXA_TRY(launch_p<false>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase, any_fan));
+    ZXA_TRY(ctx.toc(ndiag > 0 || any_fan ? F_FUSED : F_U3TILE, bytes));
     return ctx.ring.done(ctx.stream);
 }
 
@@ -1766,6 +1999,17 @@ unsigned tile_min_bits(int prec)
     return unsigned(kUnitBits) + (prec == C64 ? 1u : 0u);
 }
 
+bool tile_staged_fits(int prec, unsigned n, const std::vector<unsigned> &targets, int geometry)
+{
+    std::vector<int> ua;
+    if (n < tile_min_bits(prec) || n > 40)
+        return false;
+    for (unsigned t : targets)
+        if (t >= n)
+            return false;
+    return staged_units(prec, geometry, targets, ua);
+}
+
 // Pivots of a set of pair offsets: each offset reduced by the earlier ones
 // (elimination over GF(2)) and its top bit taken, so that the offsets
 // restricted to the pivots form an invertible matrix.  A tile whose slots
@@ -1882,31 +2126,31 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
     std::sort(single.begin(), single.end(), [](const U3Gate *a, const U3Gate *b) { return a->delta < b->delta; });
     std::vector<const U3Gate *> order = single;
     order.insert(order.end(), wide.begin(), wide.end());
-    // the configured geometry, unless another needs fewer passes (wide
-    // offsets take pivots above the low bits, which the smaller contiguous
-    // runs give more slots for); among equal counts the longer runs
+    // single-bit batches: the configured geometry.  Wide batches: the
+    // geometry with the fewest passes (wide offsets take pivots above the
+    // low bits, which shorter contiguous runs give more slots for), among
+    // equal counts the longest runs
     const int own = int(tile_seg(prec));
     std::vector<std::vector<SingleGate>> best, cand;
-    int geo = own;
-    if (!plan_passes(prec, own, order, best))
+    int geo = -1;
+    for (int g = wide.empty() ? own : 6; g >= (wide.empty() ? own : 3); g--)
+        if (plan_passes(prec, g, order, cand) && (geo < 0 || cand.size() < best.size()))
+        {
+            best.swap(cand);
+            geo = g;
+        }
+    if (geo < 0)
     {
         set_error("u3_apply: dependent pair offsets in one batch");
         return EINVAL_;
     }
-    if (!wide.empty())
-        for (int g = 6; g >= 3; g--)
-            if (g != own && plan_passes(prec, g, order, cand) && cand.size() < best.size())
-            {
-                best.swap(cand);
-                geo = g;
-            }
     for (const std::vector<SingleGate> &gs : best)
         ZXA_TRY(run_tile_pass(ctx, state, prec, n, gs, nullptr, 0, geo));
     return OK;
 }
 
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates,
-                     const DiagGate *diag, size_t ndiag, int geometry)
+                     const DiagGate *diag, size_t ndiag, int geometry, const GateFan *fans)
 {
     if (n < tile_min_bits(prec) || n > 40)
     {
@@ -1948,7 +2192,7 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
     }
     if (gs.empty())
         return diag_apply(ctx, state, prec, n, diag, ndiag);
-    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag, geometry);
+    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag, geometry, fans);
 }
 
 } // namespace zxa
