@@ -81,12 +81,13 @@ struct TGate
 
 // A gate's fan (complex128 passes): before the gate, every amplitude whose
 // target bit is one is multiplied by exp(i phi(x)), phi linear in the other
-// address bits.  exp(i phi) = F(tile) * prod_{thread bits j set} thr[j] *
-// reg[r]: F per tile (fan_phase_kernel), thr over the thread bits and reg
-// over the register offsets of the gate's layout.
+// address bits.  exp(i phi) = F(tile) * nib[t & 15] * nib[16 + (t >> 4)] *
+// reg[r]: F per tile (fan_phase_kernel), nib over the low and high four
+// thread bits and reg over the register offsets of the gate's layout.  The
+// kernel copies nib to LDS (a thread's entries differ from its neighbours').
 struct TFan
 {
-    double thr[8][2];
+    double nib[32][2];
     double reg[kRegs][2];
 };
 constexpr int kFanStride = 2 * (kUnitBits + 1); // doubles of per-tile factors per tile
@@ -278,7 +279,8 @@ __device__ __forceinline__ void unit_pair2(typename UnitT<kC64>::T &a, typename 
 // in-unit gate).  base ^ toff is the unit address of the thread's register 0.
 template <bool kC64, bool kFlip, int L, bool kFan = false>
 __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs], const TPass &P, ZXA_CONST TTab *G,
-                                             uint64_t base, uint64_t toff, int t = 0, ZXA_CONST double *fp = nullptr)
+                                             uint64_t base, uint64_t toff, int t = 0, ZXA_CONST double *fp = nullptr,
+                                             const dvec2 *fl = nullptr)
 {
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -291,16 +293,10 @@ __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs]
             {
                 // the gate's fan: exp(i phi) on the second element of each pair
                 ZXA_CONST TFan &f = G->fan[k];
-                double fx = fp[2 * k], fy = fp[2 * k + 1];
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                {
-                    const bool on = ((t >> j) & 1) != 0;
-                    const double ax = on ? f.thr[j][0] : 1.0, ay = on ? f.thr[j][1] : 0.0;
-                    const double nx = fx * ax - fy * ay;
-                    fy = fx * ay + fy * ax;
-                    fx = nx;
-                }
+                const dvec2 lo = fl[k * 32 + (t & 15)], hi = fl[k * 32 + 16 + (t >> 4)];
+                const double tx = lo.x * hi.x - lo.y * hi.y, ty = lo.x * hi.y + lo.y * hi.x;
+                const double ux = fp[2 * k], uy = fp[2 * k + 1];
+                const double fx = ux * tx - uy * ty, fy = ux * ty + uy * tx;
 #pragma unroll
                 for (int r = 0; r < kRegs; r++)
                 {
@@ -516,20 +512,21 @@ __device__ __forceinline__ void store_tile(const typename UnitT<kC64>::T (&v)[kR
 template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false>
 __device__ __forceinline__ void tile_work(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds,
                                           const TPass &P, ZXA_CONST TTab *G, const double *tphase,
-                                          const Epi<kC64> &epi, int t, uint64_t toff, uint64_t tile, uint64_t base)
+                                          const Epi<kC64> &epi, int t, uint64_t toff, uint64_t tile, uint64_t base,
+                                          const dvec2 *fl = nullptr)
 {
     ZXA_CONST double *fp = nullptr;
     if constexpr (kFan)
         fp = (ZXA_CONST double *)(P.fphase + tile * kFanStride);
-    apply_layout<kC64, kFlip, 0, kFan>(v, P, G, base, toff, t, fp);
+    apply_layout<kC64, kFlip, 0, kFan>(v, P, G, base, toff, t, fp, fl);
     if constexpr (kL1)
     {
         transpose<kC64, 0, 1>(v, lds, t);
-        apply_layout<kC64, kFlip, 1, kFan>(v, P, G, base, thr_off<1>(t, P), t, fp);
+        apply_layout<kC64, kFlip, 1, kFan>(v, P, G, base, thr_off<1>(t, P), t, fp, fl);
         if constexpr (kL2)
         {
             transpose<kC64, 1, 2>(v, lds, t);
-            apply_layout<kC64, kFlip, 2, kFan>(v, P, G, base, thr_off<2>(t, P), t, fp);
+            apply_layout<kC64, kFlip, 2, kFan>(v, P, G, base, thr_off<2>(t, P), t, fp, fl);
             transpose<kC64, 2, 0>(v, lds, t);
         }
         else
@@ -540,7 +537,7 @@ __device__ __forceinline__ void tile_work(typename UnitT<kC64>::T (&v)[kRegs], t
     else if constexpr (kL2)
     {
         transpose<kC64, 0, 2>(v, lds, t);
-        apply_layout<kC64, kFlip, 2, kFan>(v, P, G, base, thr_off<2>(t, P), t, fp);
+        apply_layout<kC64, kFlip, 2, kFan>(v, P, G, base, thr_off<2>(t, P), t, fp, fl);
         transpose<kC64, 2, 0>(v, lds, t);
     }
     if constexpr (kDiag)
@@ -571,6 +568,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 {
     using T = typename UnitT<kC64>::T;
     __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
+    // fan passes: every slot's thread-nibble factors (TFan::nib)
+    __shared__ dvec2 fl[kFan ? (kUnitBits + 1) * 32 : 1];
     const int t = threadIdx.x;
     T *p = static_cast<T *>(state);
     const uint64_t toff = thr_off<0>(t, P);
@@ -578,6 +577,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     if constexpr (kDiag)
         epi.init(ttab, t);
     const uint64_t grid = gridDim.x;
+    if constexpr (kFan)
+    {
+        for (int e = t; e < (kUnitBits + 1) * 32; e += kThreads)
+            fl[e] = dvec2{tabp->fan[e >> 5].nib[e & 31][0], tabp->fan[e >> 5].nib[e & 31][1]};
+        __syncthreads();
+    }
 
     T a[kRegs], b[kRegs];
     uint64_t t0 = blockIdx.x;
@@ -588,7 +593,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     bool m1 = t1 < P.ntiles;
     uint64_t b1 = next_base(b0, P);
     load_tile<kC64>(b, p, (m1 ? b1 : b0) ^ toff, G);
-    tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t0, b0);
+    tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t0, b0, fl);
     for (;;)
     {
         G = tables(tabp);
@@ -599,7 +604,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const bool m2 = t2 < P.ntiles;
         const uint64_t b2 = next_base(b1, P);
         load_tile<kC64>(a, p, (m2 ? b2 : b1) ^ toff, G);
-        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(b, lds, P, G, tphase, epi, t, toff, t1, b1);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(b, lds, P, G, tphase, epi, t, toff, t1, b1, fl);
 
         G = tables(tabp);
         store_tile<kC64>(b, p, b1 ^ toff, G);
@@ -609,7 +614,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const bool m3 = t3 < P.ntiles;
         const uint64_t b3 = next_base(b2, P);
         load_tile<kC64>(b, p, (m3 ? b3 : b2) ^ toff, G);
-        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t2, b2);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t2, b2, fl);
 
         t0 = t2;
         b0 = b2;
@@ -1531,8 +1536,15 @@ int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const Ga
                 own = i;
         const std::vector<double> &th = fans[gi].theta;
         TFan &f = G.fan[k];
-        for (int j = 0; j < 8; j++)
-            cis(th[P.ub[thr_bit(lay, j)]], f.thr[j]);
+        for (int h = 0; h < 2; h++)
+            for (int e = 0; e < 16; e++)
+            {
+                double a = 0.0;
+                for (int j = 0; j < 4; j++)
+                    if ((e >> j) & 1)
+                        a += th[P.ub[thr_bit(lay, 4 * h + j)]];
+                cis(a, f.nib[16 * h + e]);
+            }
         for (int r = 0; r < kRegs; r++)
         {
             double a = 0.0;
