@@ -360,8 +360,10 @@ int Sim::flush_pass()
 
 // Moves the pending window's terms that involve address bit t into a fan
 // for the next gate on t (applied just before it), when they are all of
-// the form theta [x_t] L with L = 1, a constant selector or [x_k ^ beta]:
-// theta x_t (beta + (1 - 2 beta) x_k).  False (nothing moved) otherwise.
+// the form theta [x_t] L with L = 1, a constant selector or [x_k ^ beta]
+// (theta x_t (beta + (1 - 2 beta) x_k)), or one two-bit selector
+// theta [x_t ^ x_k ^ beta] (its part without x_t stays in the window).
+// False (nothing moved) otherwise.
 bool Sim::take_fan(unsigned t, GateFan &fan)
 {
     const uint64_t amask = (uint64_t(1) << 63) - 1;
@@ -377,6 +379,19 @@ bool Sim::take_fan(unsigned t, GateFan &fan)
         if (!((a0 | a1) & bit))
         {
             keep.push_back(d);
+            continue;
+        }
+        if (d.s1 == kNoSel && a0 != bit)
+        {
+            // one selector on t and k: theta [x_t ^ x_k ^ beta] = the fan
+            // theta (1 - 2 beta) x_t (1 - 2 x_k) plus theta [x_k ^ beta]
+            const uint64_t ak = a0 & ~bit;
+            if ((ak & (ak - 1)) != 0 || (ak >> n) != 0)
+                return false;
+            const double sg = (d.s0 >> 63) ? -1.0 : 1.0;
+            fan.c += d.theta * sg;
+            fan.theta[size_t(__builtin_ctzll(ak))] += -2.0 * d.theta * sg;
+            keep.push_back({ak | (d.s0 & ~amask), kNoSel, d.theta});
             continue;
         }
         // the selector on t, and the other factor
