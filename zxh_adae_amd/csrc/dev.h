@@ -336,6 +336,11 @@ int u3_pair_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate 
 // The U3 of a qubit expansion: delta is the state's top bit (nbits - 1) and
 // that half of the state is zero (it reads the lower half only).
 int u3_expand_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate &g);
+// count (at most four) expansions in a row from a state of base bits: gate
+// i expands bit base + i (delta = 1 << (base + i), selector below bit
+// base + i + 1); the region above bit base is zero.  Same products as count
+// u3_expand_apply calls; one read of the old state, one write of the new.
+int u3_expand_group_apply(Ctx &ctx, void *state, int prec, unsigned base, const U3Gate *gates, size_t count);
 
 int x_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t delta, uint64_t packed_sel);
 int cx_apply(Ctx &ctx, void *state, int prec, unsigned nbits, uint64_t delta, uint64_t sel_cq, uint64_t sel_q);

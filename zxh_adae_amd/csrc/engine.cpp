@@ -312,6 +312,7 @@ int Sim::flush_diag(std::vector<DiagGate> &window)
 {
     if (window.empty())
         return OK;
+    ZXA_TRY(flush_group());
     int rc = OK;
     if (fusable() && window_is_polynomial(active_bits(), window.data(), window.size()))
     {
@@ -342,10 +343,20 @@ bool Sim::fusable() const
 }
 
 // Runs the pending fused pass: its U3 gates, then its diagonal window.
+int Sim::flush_group()
+{
+    if (exp_group_.empty())
+        return OK;
+    const int rc = u3_expand_group_apply(ctx_, dstate_, prec_, exp_base_, exp_group_.data(), exp_group_.size());
+    exp_group_.clear();
+    return rc;
+}
+
 int Sim::flush_pass()
 {
     if (pass_u3_.empty() && pass_diag_.empty())
         return OK;
+    ZXA_TRY(flush_group());
     int rc = fused_pass_apply(ctx_, dstate_, prec_, active_bits(), pass_u3_.data(), pass_u3_.size(),
                               pass_diag_.data(), pass_diag_.size(), -1,
                               pass_has_fan_ ? pass_fan_.data() : nullptr);
@@ -446,6 +457,7 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
     if (!simple)
     {
         ZXA_TRY(flush_pass());
+        ZXA_TRY(flush_group());
         return u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size());
     }
     const unsigned low = tile_low_bits(prec_);
@@ -618,6 +630,8 @@ int Sim::execute()
     // Each run keeps its own product per qubit, as the reference forms it,
     // so the 1e-5 identity skip sees the same products.
     const bool stream = policy_ == POLICY_FUSED;
+    // expansions applied at once (u3_expand_group_apply)
+    constexpr size_t kGroupMax = 4;
     std::vector<DiagGate> pre, post;
     std::vector<Acc> pend(N_);
     size_t pend_count = 0;
@@ -663,6 +677,10 @@ int Sim::execute()
         return true;
     };
     auto push_diag = [&](const DiagGate &d) {
+        // a selector that is identically zero (alpha = 0, beta = 0: a qubit not
+        // yet expanded) makes the gate's phase zero everywhere
+        if (stream && (d.s0 == 0 || (d.s1 != kNoSel && d.s1 == 0)))
+            return;
         if (stream && !diag_joins_pre(d))
             post.push_back(d);
         else
@@ -725,9 +743,18 @@ int Sim::execute()
             uint64_t delta = 0;
             if (!solve(g.q, &delta))
             {
-                // flush the batch, then the eager, expanding apply_u3
-                ZXA_TRY(flush_all());
-                ZXA_TRY(flush_pass());
+                // flush the batch, then the eager, expanding apply_u3.  While
+                // nothing else is pending, up to four expansions in a row wait
+                // to be applied at once: one read of the old state and one
+                // write of the new instead of a sweep per bit
+                const bool idle = pend_count == 0 && pre.empty() && post.empty() && pass_u3_.empty() &&
+                                  pass_diag_.empty();
+                if (!idle)
+                {
+                    ZXA_TRY(flush_all());
+                    ZXA_TRY(flush_pass());
+                }
+                const unsigned before = active_bits();
                 int rc = OK;
                 const uint64_t d = solve_expand(g.q, &rc);
                 if (rc)
@@ -741,7 +768,19 @@ int Sim::execute()
                     eg.u[2 * i + 1] = u[i].imag();
                 }
                 // the new bit is the top one and its half still zero
-                if (eg.delta == uint64_t(1) << (active_bits() - 1))
+                const bool top = eg.delta == uint64_t(1) << (active_bits() - 1);
+                if (stream && top && !exp_group_.empty() &&
+                    (exp_group_.size() == kGroupMax || exp_base_ + exp_group_.size() != before))
+                    ZXA_TRY(flush_group());
+                if (stream && top)
+                {
+                    if (exp_group_.empty())
+                        exp_base_ = before;
+                    exp_group_.push_back(eg);
+                    continue;
+                }
+                ZXA_TRY(flush_group());
+                if (top)
                     ZXA_TRY(u3_expand_apply(ctx_, dstate_, prec_, active_bits(), eg));
                 else
                     ZXA_TRY(u3_pair_apply(ctx_, dstate_, prec_, active_bits(), eg));
@@ -789,6 +828,7 @@ int Sim::execute()
                 const uint64_t d = solve_expand(g.q, &rc);
                 if (rc)
                     return rc;
+                ZXA_TRY(flush_group());
                 ZXA_TRY(x_apply(ctx_, dstate_, prec_, active_bits(), d & low_mask(active_bits()),
                                 local_selector(g.q)));
             }
@@ -804,6 +844,7 @@ int Sim::execute()
                 const uint64_t d = solve_expand(g.q, &rc);
                 if (rc)
                     return rc;
+                ZXA_TRY(flush_group());
                 ZXA_TRY(cx_apply(ctx_, dstate_, prec_, active_bits(), d & low_mask(active_bits()),
                                  local_selector(g.cq), local_selector(g.q)));
             }
@@ -815,10 +856,13 @@ int Sim::execute()
             break;
         case G_CP: {
             const uint64_t scq = local_selector(g.cq), sq = local_selector(g.q);
+            if (stream && (scq == 0 || sq == 0))
+                break; // phase zero everywhere (see push_diag)
             if (sel_class(scq) == 2 || sel_class(sq) == 2)
             {
                 ZXA_TRY(flush_all());
                 ZXA_TRY(flush_pass());
+                ZXA_TRY(flush_group());
                 ZXA_TRY(cp_apply(ctx_, dstate_, prec_, active_bits(), scq, sq, g.theta));
             }
             else
@@ -831,6 +875,7 @@ int Sim::execute()
     }
     ZXA_TRY(flush_all());
     ZXA_TRY(flush_pass());
+    ZXA_TRY(flush_group());
     ZXA_HIP(hipStreamSynchronize(ctx_.stream));
     executed_ = true;
     return OK;

@@ -166,6 +166,11 @@ class Sim
     int pass_high_ = 0;
     bool pass_has_fan_ = false;
     bool take_fan(unsigned t, GateFan &fan);
+    // expansions in a row while nothing else is pending, applied at once
+    // before the next physical operation
+    std::vector<U3Gate> exp_group_;
+    unsigned exp_base_ = 0;
+    int flush_group();
     Ctx ctx_;
 };
 

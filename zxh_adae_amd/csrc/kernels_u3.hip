@@ -107,6 +107,61 @@ __global__ __launch_bounds__(kThreads) void u3_expand_kernel(void *__restrict__ 
     }
 }
 
+// A run of J (2..4) expansions at once.  Gate i expands address bit
+// base + i on the state of base + i bits; its selector may use every lower
+// bit.  One thread per old amplitude a[k] builds the 2^J new amplitudes
+// k + y 2^base in registers, doubling per gate as the expansions one at a
+// time do: with odd = S_i(k + y 2^base) for the y built so far, the
+// amplitude v becomes m00 v (bit base + i zero) and m10 v (one), or m11 v
+// and m01 v when odd.  Thread k alone reads a[k], so writing in place is safe.
+struct ExpGate
+{
+    uint64_t packed;
+    double u[8];
+};
+
+template <bool kC64, int J>
+__global__ __launch_bounds__(kThreads) void u3_expand_group_kernel(void *__restrict__ state, unsigned base,
+                                                                   uint64_t old, const ExpGate *__restrict__ gates)
+{
+    using T = typename Elem<kC64>::T;
+    using R = typename Elem<kC64>::R;
+    T *p = static_cast<T *>(state);
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < old; k += stride)
+    {
+        R vx[1 << J], vy[1 << J];
+        const T a = p[k];
+        vx[0] = a.x;
+        vy[0] = a.y;
+#pragma unroll
+        for (int i = 0; i < J; i++)
+        {
+#pragma unroll
+            for (int y = 0; y < (1 << i); y++)
+            {
+                const bool odd = sel_eval(gates[i].packed, k | (uint64_t(y) << base));
+                const int a0 = odd ? 6 : 0, a1 = odd ? 2 : 4;
+                const R s0x = R(gates[i].u[a0]), s0y = R(gates[i].u[a0 + 1]);
+                const R s1x = R(gates[i].u[a1]), s1y = R(gates[i].u[a1 + 1]);
+                const R x = vx[y], w = vy[y];
+                vx[y | (1 << i)] = s1x * x - s1y * w;
+                vy[y | (1 << i)] = s1x * w + s1y * x;
+                vx[y] = s0x * x - s0y * w;
+                vy[y] = s0x * w + s0y * x;
+            }
+        }
+#pragma unroll
+        for (int y = 0; y < (1 << J); y++)
+        {
+            T out;
+            out.x = vx[y];
+            out.y = vy[y];
+            p[k + (uint64_t(y) << base)] = out;
+        }
+    }
+}
+
 // X / CX explicit permutations (x_block_kernel_cuda / cx_block_kernel_cuda,
 // kernel_cuda.cu:442-477): swap i (S_q(i) = 0, and S_cq(i) = 1 for CX) with
 // i ^ delta.
@@ -233,6 +288,61 @@ int u3_expand_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate &g
                            g.packed_sel, g.u[0], g.u[1], g.u[2], g.u[3], g.u[4], g.u[5], g.u[6], g.u[7]);
     ZXA_HIP(hipGetLastError());
     return ctx.toc(F_U3PAIR, bytes);
+}
+
+int u3_expand_group_apply(Ctx &ctx, void *state, int prec, unsigned base, const U3Gate *gates, size_t count)
+{
+    if (count == 0)
+        return OK;
+    if (count == 1)
+        return u3_expand_apply(ctx, state, prec, base + 1, gates[0]);
+    if (count > 4 || base + count > 40)
+    {
+        set_error("u3 expand group: at most four expansions, within 40 bits");
+        return EINVAL_;
+    }
+    std::vector<ExpGate> tab(count);
+    for (size_t i = 0; i < count; i++)
+    {
+        const unsigned b = base + unsigned(i);
+        const uint64_t alpha = gates[i].packed_sel & ((uint64_t(1) << 63) - 1);
+        if (gates[i].delta != (uint64_t(1) << b) || (alpha >> (b + 1)) != 0)
+        {
+            set_error("u3 expand group: gate i must expand bit base + i and select below it");
+            return EINVAL_;
+        }
+        tab[i].packed = gates[i].packed_sel;
+        std::memcpy(tab[i].u, gates[i].u, sizeof(tab[i].u));
+    }
+    void *dt = nullptr;
+    ZXA_TRY(ctx.ring.stage(tab.data(), tab.size() * sizeof(ExpGate), ctx.stream, &dt));
+    const ExpGate *dg = static_cast<const ExpGate *>(dt);
+    const uint64_t old = uint64_t(1) << base;
+    // bytes: the old amplitudes read, the final ones written
+    const uint64_t bytes = (old + (old << count)) * elem_bytes(prec);
+    const dim3 grid(grid_for(old)), block(kThreads);
+    ZXA_TRY(ctx.tic());
+    if (prec == C64)
+    {
+        if (count == 2)
+            hipLaunchKernelGGL((u3_expand_group_kernel<true, 2>), grid, block, 0, ctx.stream, state, base, old, dg);
+        else if (count == 3)
+            hipLaunchKernelGGL((u3_expand_group_kernel<true, 3>), grid, block, 0, ctx.stream, state, base, old, dg);
+        else
+            hipLaunchKernelGGL((u3_expand_group_kernel<true, 4>), grid, block, 0, ctx.stream, state, base, old, dg);
+    }
+    else
+    {
+        if (count == 2)
+            hipLaunchKernelGGL((u3_expand_group_kernel<false, 2>), grid, block, 0, ctx.stream, state, base, old, dg);
+        else if (count == 3)
+            hipLaunchKernelGGL((u3_expand_group_kernel<false, 3>), grid, block, 0, ctx.stream, state, base, old, dg);
+        else
+            hipLaunchKernelGGL((u3_expand_group_kernel<false, 4>), grid, block, 0, ctx.stream, state, base, old, dg);
+    }
+    ZXA_HIP(hipGetLastError());
+    ZXA_TRY(ctx.toc(F_U3PAIR, bytes));
+    return ctx.ring.done(ctx.stream);
 }
 
 int x_apply(Ctx &ctx, void *state, int prec, unsigned n, uint64_t delta, uint64_t sel)
