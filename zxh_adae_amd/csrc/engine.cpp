@@ -610,6 +610,7 @@ int Sim::execute()
     pass_support_ = 0;
     pass_high_ = 0;
     pass_has_fan_ = false;
+    exp_group_.clear();
 
     // Scheduling.  The reference policy batches each run of consecutive U3
     // gates and flushes the diagonal window when a run starts (zxh.cpp:440-470).
