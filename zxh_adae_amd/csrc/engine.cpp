@@ -747,8 +747,11 @@ int Sim::execute()
                 // flush the batch, then the eager, expanding apply_u3.  While
                 // nothing else is pending, up to four expansions in a row wait
                 // to be applied at once: one read of the old state and one
-                // write of the new instead of a sweep per bit
-                const bool idle = pend_count == 0 && pre.empty() && post.empty() && pass_u3_.empty() &&
+                // write of the new instead of a sweep per bit.  Diagonal gates
+                // waiting in pre may stay: they act on the old bits only, so
+                // they commute with an expansion (both halves of each new pair
+                // get the same phase) and are applied after the group
+                const bool idle = pend_count == 0 && (stream || pre.empty()) && post.empty() && pass_u3_.empty() &&
                                   pass_diag_.empty();
                 if (!idle)
                 {
