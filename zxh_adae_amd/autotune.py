@@ -82,7 +82,7 @@ def layer_passes(n: int, seg: int, c64: bool):
     return chunks
 
 
-TILE_TOLERANCE = 0.03
+TILE_TOLERANCE = 0.05
 
 
 def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dict[str, object]:
@@ -93,7 +93,8 @@ def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dic
     the layer's total time: the geometry with the longest runs whose layer
     is within TILE_TOLERANCE of the fastest (longer runs keep passes with
     fewer targets fast, and a smaller difference is within run-to-run
-    variation).  Sets that geometry; the state is changed by the passes."""
+    variation: at 3% the complex128 choice flipped between runs).  Sets that
+    geometry; the state is changed by the passes."""
     c64 = state.prec == _lib.ZXH_C64
     knob = "tile.seg.c64" if c64 else "tile.seg.c128"
     n = state.nbits
