@@ -134,8 +134,11 @@ int zxh_state_device_ptr(const zxh_sim *s, void **dptr, size_t *m_out);
 /* per-kernel-family timing with HIP events (0 = off) */
 int zxh_set_timing(zxh_sim *s, int on);
 /* family: 0 diag, 1 u3 tile pass, 2 u3 pair (general), 3 perm, 4 cp,
- * 5 sample, 6 reset, 7 copy, 8 scale.  Fills launches, total milliseconds
- * and algorithmic bytes. */
+ * 5 sample, 6 reset, 7 copy, 8 scale, 9 fused tile pass (u3 + diagonal
+ * window), 10 observable, 11 adjoint, 12 dense block.  Fills launches, total
+ * milliseconds and algorithmic bytes.  Family 13 only counts the launches of
+ * families 1 and 9 that ran on the LDS-free upper kernel (time and bytes 0;
+ * they are in those families). */
 int zxh_get_timing(zxh_sim *s, int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
 int zxh_reset_timing(zxh_sim *s);
 

@@ -391,6 +391,61 @@ def test_u3_batch_under_each_tile_geometry(gpu_engine, tile_seg, n, targets, pre
     assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
 
 
+@pytest.fixture
+def upper_knobs(gpu_engine):
+    """Restores the tile geometry and upper-kernel knobs after a test."""
+    from zxh_adae_amd import autotune
+    knobs = ("tile.upper.c128", "tile.upper.c64", "tile.seg.c128", "tile.seg.c64")
+    old = {k: autotune.tune_get(k) for k in knobs}
+    yield autotune
+    for k, v in old.items():
+        autotune.tune_set(k, v)
+
+
+@pytest.mark.parametrize("prec,seg,n,targets,ndiag,upper", [
+    ("c128", 6, 18, [12, 13, 14, 15, 16, 17], 0, 1), ("c128", 6, 20, [19, 18, 14, 15, 16, 17], 0, 1),
+    ("c128", 6, 20, [8, 9, 10, 11, 18, 19], 40, 1), ("c128", 6, 22, [16, 21], 0, 1),
+    ("c128", 6, 20, [3, 15, 19], 0, 0), ("c128", 4, 20, [10, 12, 14, 16, 18, 19], 64, 1),
+    ("c64", 6, 20, [0, 14, 15, 16, 17, 18, 19], 0, 1), ("c64", 4, 19, [0, 9, 11, 13, 18], 30, 1),
+    ("c64", 3, 19, [1, 5, 9, 13, 18], 0, 0)])
+def test_upper_kernel_matches_the_tile_kernel(upper_knobs, prec, seg, n, targets, ndiag, upper):
+    """Passes whose targets all sit on the tile's unit bits 6..11 (and the
+    complex64 in-unit bit) may run on the LDS-free upper kernel: forced on
+    (knob 0) it gives the tile kernel's amplitudes exactly (forced off, knob
+    64), with and without a diagonal epilogue, and matches the oracle; a pass
+    with a target inside the contiguous run stays on the tile kernel."""
+    from zxh_adae_amd import kernels as K
+    A = upper_knobs
+    A.tune_set(f"tile.seg.{prec}", seg)
+    rng = np.random.default_rng(n * 31 + seg + ndiag)
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in targets]
+    s0, s1, th = diag_window(rng, n, ndiag) if ndiag else ([], [], [])
+    want = oracle_u3(psi, n, descs)
+    if ndiag:
+        want = oracle_diag(want, n, s0, s1, th)
+    got, count = {}, {}
+    for knob in (64, 0):
+        A.tune_set(f"tile.upper.{prec}", knob)
+        d = dev(upper_knobs, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+        K.set_timing(True)
+        K.reset_timing()
+        try:
+            if ndiag:
+                d.fused_pass(descs, s0, s1, th)
+            else:
+                d.u3_batch(descs)
+            count[knob] = K.timing("u3_upper")["launches"]
+            passes = K.timing("u3_tile")["launches"] + K.timing("fused")["launches"]
+        finally:
+            K.set_timing(False)
+        got[knob] = d.download()
+        assert passes == 1
+    assert count == {64: 0, 0: upper}
+    assert np.array_equal(got[0], got[64])
+    assert np.max(np.abs(got[0].astype(np.complex128) - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
 def test_fused_pass_rejects_multibit_and_overfull(gpu_engine):
     from zxh_adae_amd import ZXHError
     rng = np.random.default_rng(1)

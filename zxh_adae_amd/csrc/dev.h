@@ -89,7 +89,8 @@ enum Family
     F_OBS = 10,
     F_ADJ = 11,
     F_BLOCK = 12,
-    F_COUNT = 13
+    F_UPPER = 13, // count only: the F_U3TILE / F_FUSED launches that ran upper_kernel
+    F_COUNT = 14
 };
 
 struct FamilyTiming
@@ -179,6 +180,10 @@ struct Ctx
 //   diag.nt.c128 / diag.nt.c64     nontemporal loads/stores in the flat sweep
 //   tile.seg.c128 / tile.seg.c64   contiguous unit-address bits of a tile (3..6)
 //   tile.bufs.c128 / tile.bufs.c64 register buffers per tile-pass workgroup (2, 3)
+//   tile.upper.c128 / .c64         a pass with every target on unit bits 6..11
+//                                  whose highest target unit-address bit is at
+//                                  least this runs on the LDS-free upper kernel
+//                                  (64: never)
 //   block.variant.c128 / .c64      dense-block kernel variant (bit 0 prefetch,
 //                                  bit 1 matrix from LDS, bit 2 cached access)
 int tune_get(const char *name);

@@ -30,6 +30,7 @@
 #include "poly.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <type_traits>
@@ -700,6 +701,105 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     }
 }
 
+// Exchanges lane bit 4 (k32 false: v_permlane16_swap, the odd 16-lane rows of
+// a with the even rows of b) or lane bit 5 (k32 true: v_permlane32_swap, the
+// upper 32 lanes of a with the lower 32 of b) with the register bit that
+// tells a from b.  Each exchange is its own inverse.
+template <bool k32, typename T>
+__device__ __forceinline__ void lane_swap(T &a, T &b)
+{
+    static_assert(sizeof(T) == 16, "16-byte units");
+    struct W
+    {
+        uint32_t w[4];
+    };
+    W x = __builtin_bit_cast(W, a), y = __builtin_bit_cast(W, b);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        if constexpr (k32)
+        {
+            const auto r = __builtin_amdgcn_permlane32_swap(x.w[i], y.w[i], false, false);
+            x.w[i] = r[0];
+            y.w[i] = r[1];
+        }
+        else
+        {
+            const auto r = __builtin_amdgcn_permlane16_swap(x.w[i], y.w[i], false, false);
+            x.w[i] = r[0];
+            y.w[i] = r[1];
+        }
+    }
+    a = __builtin_bit_cast(T, x);
+    b = __builtin_bit_cast(T, y);
+}
+
+// The layout-0 thread whose units lane j of wave w holds in upper_kernel:
+// unit bits 0..3 in lane bits 0..3, unit bits 4, 5 in the wave index, unit
+// bits 6, 7 in lane bits 4, 5.
+__device__ __forceinline__ int upper_thread(int t)
+{
+    const int j = t & 63;
+    return (j & 15) | ((t >> 6) << 4) | ((j >> 4) << 6);
+}
+
+// Passes whose gates all sit on unit bits 6..11 (complex64: and the in-unit
+// bit), without per-pair selectors or fans: one tile per workgroup and no
+// LDS, so the workgroups per CU are bound by registers instead of by a 64 KiB
+// tile buffer (more tiles in flight per CU).  Each thread holds the units of
+// layout-0 thread upper_thread(t), registers = unit bits 8..11 as in layout 0.
+// After the layout-0 gates the lane swaps make unit bits 6, 7 register bits 0,
+// 1 (and unit bits 8, 9 lane bits 4, 5) for their gates, and are undone
+// before the diagonal epilogue and the store.  The gates run in the tile
+// kernel's order (layout 0, then unit bits 6, 7) with the same arithmetic.
+template <bool kC64, bool kDiag>
+__global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ state, const TPass P,
+                                                         const TTab *__restrict__ tabp,
+                                                         const double *__restrict__ ttab,
+                                                         const double *__restrict__ tphase)
+{
+    using T = typename UnitT<kC64>::T;
+    const int tl = upper_thread(threadIdx.x);
+    T *p = static_cast<T *>(state);
+    const uint64_t tile = blockIdx.x;
+    const uint64_t at = deposit(tile, P.sorted) ^ thr_off<0>(tl, P);
+    ZXA_CONST TTab *G = (ZXA_CONST TTab *)tabp;
+    T v[kRegs];
+    load_tile<kC64>(v, p, at, G);
+    apply_layout<kC64, false, 0>(v, P, G, 0, 0);
+#pragma unroll
+    for (int r = 0; r < kRegs; r += 2)
+        lane_swap<false>(v[r], v[r + 1]);
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (!(r & 2))
+            lane_swap<true>(v[r], v[r + 2]);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+    {
+        Real<kC64> m[8];
+        load_matrix<kC64>(m, G->g[6 + i]);
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+            if (!((r >> i) & 1))
+                unit_pair<kC64, false>(v[r], v[r | (1 << i)], m, false, false);
+    }
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        if (!(r & 2))
+            lane_swap<true>(v[r], v[r + 2]);
+#pragma unroll
+    for (int r = 0; r < kRegs; r += 2)
+        lane_swap<false>(v[r], v[r + 1]);
+    if constexpr (kDiag)
+    {
+        Epi<kC64> epi;
+        epi.init(ttab, tl);
+        epi.apply(v, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), tl);
+    }
+    store_tile<kC64>(v, p, at, G);
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint pass: the reverse of one forward tile pass, on two states at once.
 // psi is the ansatz state and lam = (later gates)^dagger H psi, both taken at
@@ -1192,6 +1292,27 @@ int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip,
     if (flip)
         return launch_l<kC64, false, true>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
     return launch_l<kC64, false, false>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
+}
+
+// One upper_kernel workgroup per tile (ttab and tphase set: the diagonal
+// epilogue).
+int launch_u(Ctx &ctx, bool c64, void *state, const TPass &P, const TTab *tab, const double *ttab,
+             const double *tphase)
+{
+    const dim3 grid(unsigned(P.ntiles));
+    if (c64 && ttab)
+        hipLaunchKernelGGL((upper_kernel<true, true>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab, ttab,
+                           tphase);
+    else if (c64)
+        hipLaunchKernelGGL((upper_kernel<true, false>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab, nullptr,
+                           nullptr);
+    else if (ttab)
+        hipLaunchKernelGGL((upper_kernel<false, true>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab, ttab,
+                           tphase);
+    else
+        hipLaunchKernelGGL((upper_kernel<false, false>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab,
+                           nullptr, nullptr);
+    return hip_check(hipGetLastError(), "upper_kernel launch");
 }
 
 // A gate's target is the address bit its tile slot stands for: the bit of
@@ -1734,13 +1855,33 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     const TTab *tab = static_cast<const TTab *>(dtab);
+    // passes whose targets all sit on unit bits 6..11 run on upper_kernel (one
+    // tile per workgroup, no LDS) when their highest target unit-address bit
+    // is at least tile.upper.<prec>: spread over more than 2^k units, a tile's
+    // runs are fetched faster with more tiles in flight per CU than the LDS
+    // allows the tile kernel
+    bool upper = !flip && !any_fan && P.ntiles <= uint64_t(INT_MAX);
+    unsigned high = 0;
+    for (size_t gi = 0; gi < gs.size() && upper; gi++)
+    {
+        const int slot = pl.slot[gi];
+        if (slot == kIntra)
+            continue;
+        upper = slot >= 6;
+        high = std::max(high, P.ub[slot]);
+    }
+    upper = upper && high >= unsigned(tune_get(c64 ? "tile.upper.c64" : "tile.upper.c128"));
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
-    if (c64)
+    if (upper)
+        ZXA_TRY(launch_u(ctx, c64, state, P, tab, ttab, tphase));
+    else if (c64)
         ZXA_TRY(launch_p<true>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
     else
         ZXA_TRY(launch_p<false>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase, any_fan));
     ZXA_TRY(ctx.toc(ndiag > 0 || any_fan ? F_FUSED : F_U3TILE, bytes));
+    if (upper && ctx.timing)
+        ctx.fam[F_UPPER].launches++;
     return ctx.ring.done(ctx.stream);
 }
 
