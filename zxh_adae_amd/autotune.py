@@ -82,7 +82,7 @@ def layer_passes(n: int, seg: int, c64: bool):
     return chunks
 
 
-TILE_TOLERANCE = 0.05
+TILE_TOLERANCE = 0.03
 
 
 def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dict[str, object]:
@@ -92,9 +92,9 @@ def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dic
     per pass (fewer passes) at shorter contiguous runs, so the choice is by
     the layer's total time: the geometry with the longest runs whose layer
     is within TILE_TOLERANCE of the fastest (longer runs keep passes with
-    fewer targets fast, and a smaller difference is within run-to-run
-    variation: at 3% the complex128 choice flipped between runs).  Sets that
-    geometry; the state is changed by the passes."""
+    fewer targets fast, and a difference of a few percent is within
+    run-to-run variation).  Sets that geometry; the state is changed by the
+    passes."""
     c64 = state.prec == _lib.ZXH_C64
     knob = "tile.seg.c64" if c64 else "tile.seg.c128"
     n = state.nbits
