@@ -446,6 +446,47 @@ def test_upper_kernel_matches_the_tile_kernel(upper_knobs, prec, seg, n, targets
     assert np.max(np.abs(got[0].astype(np.complex128) - want)) < (TOL128 if prec == "c128" else TOL64)
 
 
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+def test_autotuners_follow_their_rules(upper_knobs, prec):
+    """tune_diag keeps the fastest diagonal variant; tune_tile keeps the
+    geometry with the longest contiguous runs whose layer is within
+    TILE_TOLERANCE of the fastest, with the pass count of layer_passes.  Both
+    set their knobs, and a U3 batch under the tuned geometry matches the
+    oracle."""
+    from zxh_adae_amd import kernels as K
+    A = upper_knobs
+    n = 20
+    diag_knobs = (f"diag.ept.{prec}", f"diag.nt.{prec}")
+    old = {k: A.tune_get(k) for k in diag_knobs}
+    try:
+        st = K.DeviceState(n, prec)
+        st.scale(0.0)
+        d = A.tune_diag(st)
+        key = f"ept{d['best']['ept']}_nt{d['best']['nt']}"
+        assert d["ms"][key] == min(d["ms"].values())
+        assert (A.tune_get(diag_knobs[0]), A.tune_get(diag_knobs[1])) == (d["best"]["ept"], d["best"]["nt"])
+        t = A.tune_tile(st)
+        ms = {int(k[3:]): v for k, v in t["layer_ms"].items()}
+        assert sorted(ms) == sorted(A.TILE_SEGS)
+        bound = min(ms.values()) * (1.0 + A.TILE_TOLERANCE) + 1e-4  # layer_ms is rounded to 0.1 us
+        assert ms[t["best"]] <= bound
+        assert all(ms[s] > min(ms.values()) * (1.0 + A.TILE_TOLERANCE) - 1e-4 for s in ms if s > t["best"])
+        assert A.tune_get(f"tile.seg.{prec}") == t["best"]
+        for s in A.TILE_SEGS:
+            assert t["passes"][f"seg{s}"] == len(A.layer_passes(n, s, prec == "c64"))
+        st.free()
+        rng = np.random.default_rng(5)
+        psi = rand_state(rng, n)
+        descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in range(n)]
+        want = oracle_u3(psi, n, descs)
+        d2 = dev(upper_knobs, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+        d2.u3_batch(descs)
+        assert np.max(np.abs(d2.download().astype(np.complex128) - want)) < (TOL128 if prec == "c128" else TOL64)
+    finally:
+        for k, v in old.items():
+            A.tune_set(k, v)
+
+
 def test_fused_pass_rejects_multibit_and_overfull(gpu_engine):
     from zxh_adae_amd import ZXHError
     rng = np.random.default_rng(1)
