@@ -295,6 +295,10 @@ int zxh_batch_set_info(zxh_batch *b, size_t set, unsigned *nbits, size_t *passes
  *                                    minus this above them)
  *   "tile.bufs.c128", "tile.bufs.c64": 2 or 3 register buffers per tile-pass
  *                                      workgroup (tiles in flight)
+ *   "tile.upper.c128", "tile.upper.c64": a pass whose targets all sit on tile
+ *       unit bits 6..11 runs on the LDS-free upper kernel when its highest
+ *       target unit-address bit is at least this (0..64, 64 = never;
+ *       default 18); results are bit-identical either way
  *   "block.variant.c128", "block.variant.c64": dense-block kernel variant,
  *       bit 0 prefetch the next wave tile, bit 1 matrix re-read from LDS,
  *       bit 2 cached (not nontemporal) loads and stores (0..7) */

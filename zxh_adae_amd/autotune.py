@@ -8,9 +8,10 @@ Knobs swept:
   diag.ept.<prec>  0 = persistent diagonal sweep; 1, 2, 4 = flat grid with that
                    many 16-byte units per thread
   diag.nt.<prec>   nontemporal loads/stores in the flat sweep
-  tile.seg.<prec>  contiguous unit-address bits of a tile pass (4, 5, 6: every
-                   wave access moves 256 B, 512 B or 1 KiB contiguous; a pass
-                   then reaches 8, 7 or 6 targets above them)
+  tile.seg.<prec>  contiguous unit-address bits of a tile pass (3, 4, 5, 6:
+                   every wave access moves 128 B, 256 B, 512 B or 1 KiB
+                   contiguous; a pass then reaches 9, 8, 7 or 6 targets above
+                   them)
 """
 from __future__ import annotations
 
