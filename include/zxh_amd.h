@@ -301,7 +301,9 @@ int zxh_batch_set_info(zxh_batch *b, size_t set, unsigned *nbits, size_t *passes
  *       default 18); results are bit-identical either way
  *   "block.variant.c128", "block.variant.c64": dense-block kernel variant,
  *       bit 0 prefetch the next wave tile, bit 1 matrix re-read from LDS,
- *       bit 2 cached (not nontemporal) loads and stores (0..7) */
+ *       bit 2 cached (not nontemporal) loads and stores (0..7)
+ *   "block.tiles.c128", "block.tiles.c64": wave tiles per wave of a block
+ *       sweep (1..4096), or 0 for a persistent grid (default 0) */
 int zxh_tune_set(const char *name, int value);
 int zxh_tune_get(const char *name, int *value);
 /* HIP-event timing of the kernel-level calls (same families as zxh_get_timing,

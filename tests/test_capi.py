@@ -46,7 +46,8 @@ def test_tuning_knobs_ranges():
              "tile.seg.c128": ([3, 4, 5, 6], [2, 7]), "tile.seg.c64": ([3, 4, 5, 6], [0, 12]),
              "tile.bufs.c128": ([2, 3], [1, 4]), "tile.bufs.c64": ([2, 3], [0, 5]),
              "tile.upper.c128": ([0, 18, 64], [-1, 65]), "tile.upper.c64": ([0, 17, 64], [-2, 100]),
-             "block.variant.c128": ([0, 3, 4, 7], [-1, 8]), "block.variant.c64": ([0, 5, 7], [8, 9])}
+             "block.variant.c128": ([0, 3, 4, 7], [-1, 8]), "block.variant.c64": ([0, 5, 7], [8, 9]),
+             "block.tiles.c128": ([0, 1, 4, 4096], [-1, 4097]), "block.tiles.c64": ([0, 8], [5000])}
     for knob, (good, bad) in cases.items():
         old = autotune.tune_get(knob)
         for v in good:

@@ -60,16 +60,20 @@ def test_block_matches_dense_groups(gpu_engine, n, targets, prec):
     assert abs(np.linalg.norm(got) - np.linalg.norm(psi.astype(np.complex128))) < (1e-12 if prec == "c128" else 1e-5)
 
 
+@pytest.mark.parametrize("tiles", [0, 1, 3])
 @pytest.mark.parametrize("variant", range(8))
 @pytest.mark.parametrize("prec", ["c128", "c64"])
-def test_every_kernel_variant_matches_dense_groups(gpu_engine, prec, variant):
+def test_every_kernel_variant_matches_dense_groups(gpu_engine, prec, variant, tiles):
     """Each variant of the block kernel (prefetch, matrix from LDS or
-    registers, cached or nontemporal access), on a state with several wave
-    tiles per wave so the prefetch path runs, and low targets."""
+    registers, cached or nontemporal access) and grid (persistent, one or
+    three wave tiles per wave), on a state with several wave tiles per wave
+    so the prefetch path runs, and low targets."""
     from zxh_adae_amd import autotune
     from zxh_adae_amd.kernels import DeviceState
     knob = f"block.variant.{prec}"
+    tknob = f"block.tiles.{prec}"
     old = autotune.tune_get(knob)
+    old_tiles = autotune.tune_get(tknob)
     rng = np.random.default_rng(variant + (100 if prec == "c64" else 0))
     n, targets = 23, [0, 1, 7, 15, 22]
     u = haar(rng)
@@ -79,10 +83,12 @@ def test_every_kernel_variant_matches_dense_groups(gpu_engine, prec, variant):
     d.upload(psi)
     try:
         autotune.tune_set(knob, variant)
+        autotune.tune_set(tknob, tiles)
         d.block5(targets, u)
         d.block5(targets[::-1], u.conj().T)
     finally:
         autotune.tune_set(knob, old)
+        autotune.tune_set(tknob, old_tiles)
     got = d.download().astype(np.complex128)
     want = apply_block(apply_block(psi, n, targets, u), n, targets[::-1], u.conj().T)
     tol = 1e-12 if prec == "c128" else 4e-6

@@ -264,6 +264,8 @@ Knob g_knobs[] = {
     {"tile.upper.c64", 18, 0, 64},
     {"block.variant.c128", 4, 0, 7},
     {"block.variant.c64", 7, 0, 7},
+    {"block.tiles.c128", 0, 0, 4096},
+    {"block.tiles.c64", 0, 0, 4096},
 };
 std::mutex g_knob_mu;
 } // namespace

@@ -186,6 +186,8 @@ struct Ctx
 //                                  (64: never)
 //   block.variant.c128 / .c64      dense-block kernel variant (bit 0 prefetch,
 //                                  bit 1 matrix from LDS, bit 2 cached access)
+//   block.tiles.c128 / .c64        wave tiles per wave of a block sweep (0 =
+//                                  persistent grid)
 int tune_get(const char *name);
 int tune_set(const char *name, int value);
 

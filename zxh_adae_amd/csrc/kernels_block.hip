@@ -28,16 +28,20 @@
 // times (a c64 block on qubits 0, 1, 7, 15, 22 of 33: 84 ms nontemporal,
 // 33 ms cached).  Variants (knob block.variant.<prec>): prefetch of the next
 // wave tile, the matrix re-read from LDS per tile (fewer registers, more
-// waves) or held in registers, cached or nontemporal access; the defaults
-// were chosen on the 32/33-qubit random circuits: complex64 prefetch + LDS +
-// cached (29.6 ms per 33-qubit sweep), complex128 registers + cached (28.2 ms
-// per 32-qubit sweep; prefetching with FP64 matrix cores busy was slower).
+// waves) or held in registers, cached or nontemporal access; and (knob
+// block.tiles.<prec>) a persistent grid or a fixed number of wave tiles per
+// wave.  The defaults were chosen on the 32/33-qubit random circuits:
+// complex64 prefetch + LDS + cached (28.2 ms per 33-qubit sweep), complex128
+// registers + cached (27.1 ms per 32-qubit sweep; prefetching with FP64
+// matrix cores busy was slower), both on persistent grids (four wave tiles
+// per wave gave 28.4 and 27.3 ms over the whole circuits).
 // M is pre-arranged on the host in the A operand's lane order and read from
 // LDS, one word per lane per step (conflict-free).  Each wave owns disjoint
 // groups: it reads them all, multiplies, and writes them back.
 #include "dev.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -417,8 +421,16 @@ int block5_apply(Ctx &ctx, void *state, int prec, unsigned n, const unsigned *ta
     // wave tile, bit 1 re-reads the matrix from LDS at every tile, bit 2
     // uses ordinary (cached) loads and stores instead of nontemporal ones
     const int var = tune_get(c64 ? "block.variant.c64" : "block.variant.c128");
+    // wave tiles per wave (knob block.tiles.<prec>): 0 = persistent grid
+    // (occupancy x CUs workgroups), else that many tiles per wave
+    const int per_wave = tune_get(c64 ? "block.tiles.c64" : "block.tiles.c128");
     auto launch = [&](auto kernel, auto *st, const auto *frag) {
-        const int grid = grid_for_waves(ctx, kernel, A.nwtiles);
+        int grid = grid_for_waves(ctx, kernel, A.nwtiles);
+        if (per_wave > 0)
+        {
+            const uint64_t per_wg = uint64_t(per_wave) * kWaves;
+            grid = static_cast<int>(std::min<uint64_t>((A.nwtiles + per_wg - 1) / per_wg, uint64_t(INT_MAX)));
+        }
         A.gstep = deposit_bits(uint64_t(grid) * kWaves, A.sorted, A.nbits);
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, st, A, static_cast<const BTab *>(dT),
                            frag, static_cast<const uint64_t *>(dL));
