@@ -446,6 +446,74 @@ def test_upper_kernel_matches_the_tile_kernel(upper_knobs, prec, seg, n, targets
     assert np.max(np.abs(got[0].astype(np.complex128) - want)) < (TOL128 if prec == "c128" else TOL64)
 
 
+@pytest.mark.parametrize("prec,seg,n,targets,upper", [
+    ("c128", 6, 20, [14, 15, 16, 17, 18, 19], 1), ("c128", 6, 22, [21, 16, 12], 1),
+    ("c128", 4, 21, [10, 12, 14, 16, 18, 20], 1), ("c64", 6, 20, [0, 14, 15, 16, 17, 18, 19], 1),
+    ("c64", 3, 20, [13, 15, 17, 19], 1), ("c128", 6, 20, [2, 15, 19], 0)])
+def test_upper_kernel_with_per_pair_selectors(upper_knobs, prec, seg, n, targets, upper):
+    """Selectors with bits beyond the target (the side of a pair follows the
+    selector's parity over its address, beta = 1 swaps it), including the
+    complex64 in-unit bit: the upper kernel (knob 0) gives the tile kernel's
+    amplitudes exactly (knob 64) and matches the oracle."""
+    from zxh_adae_amd import kernels as K
+    A = upper_knobs
+    A.tune_set(f"tile.seg.{prec}", seg)
+    rng = np.random.default_rng(n * 17 + seg + len(targets))
+    psi = rand_state(rng, n)
+    others = [q for q in range(n) if q not in targets]
+    descs = []
+    for q in targets:
+        extra = sum(1 << int(b) for b in rng.choice(others, 3, replace=False))
+        descs.append((pack((1 << q) | extra, bool(rng.random() < 0.5)), 1 << q, rand_u(rng)))
+    want = oracle_u3(psi, n, descs)
+    got, count = {}, {}
+    for knob in (64, 0):
+        A.tune_set(f"tile.upper.{prec}", knob)
+        d = dev(upper_knobs, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+        K.set_timing(True)
+        K.reset_timing()
+        try:
+            d.u3_batch(descs)
+            count[knob] = K.timing("u3_upper")["launches"]
+            assert K.timing("u3_tile")["launches"] == 1
+        finally:
+            K.set_timing(False)
+        got[knob] = d.download()
+    assert count == {64: 0, 0: upper}
+    assert np.array_equal(got[0], got[64])
+    assert np.max(np.abs(got[0].astype(np.complex128) - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
+@pytest.mark.parametrize("n,cx,prec", [(20, 60, "c128"), (22, 200, "c128"), (21, 90, "c64")])
+def test_upper_kernel_on_affine_maps(upper_knobs, n, cx, prec):
+    """A full layer on a CX-transported map (wide pair offsets, per-pair
+    flips): every pass that qualifies runs on the upper kernel (knob 0) with
+    the tile kernel's amplitudes exactly (knob 64), matching the oracle."""
+    from zxh_adae_amd import kernels as K
+    A = upper_knobs
+    rng = np.random.default_rng(n + cx)
+    rows, deltas = random_affine(rng, n, cx)
+    psi = rand_state(rng, n)
+    descs = [(pack(rows[q], bool(rng.random() < 0.5)), deltas[q], rand_u(rng)) for q in range(n)]
+    want = oracle_u3(psi, n, descs)
+    got, count = {}, {}
+    for knob in (64, 0):
+        A.tune_set(f"tile.upper.{prec}", knob)
+        d = dev(upper_knobs, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+        K.set_timing(True)
+        K.reset_timing()
+        try:
+            d.u3_batch(descs)
+            count[knob] = K.timing("u3_upper")["launches"]
+        finally:
+            K.set_timing(False)
+        got[knob] = d.download()
+    print(n, cx, prec, count)
+    assert count[64] == 0
+    assert np.array_equal(got[0], got[64])
+    assert np.max(np.abs(got[0].astype(np.complex128) - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 def test_autotuners_follow_their_rules(upper_knobs, prec):
     """tune_diag keeps the fastest diagonal variant; tune_tile keeps the

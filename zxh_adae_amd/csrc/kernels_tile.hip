@@ -276,6 +276,52 @@ __device__ __forceinline__ void unit_pair2(typename UnitT<kC64>::T &a, typename 
     }
 }
 
+// Gate g on register bit i of the thread's units v (base ^ toff: the unit
+// address of register 0; g.rpar: the selector's parity over the register
+// offsets of the layout in use).
+template <bool kC64, bool kFlip>
+__device__ __forceinline__ void gate_on_reg_bit(typename UnitT<kC64>::T (&v)[kRegs], int i, ZXA_CONST TGate &g,
+                                                uint64_t base, uint64_t toff)
+{
+    Real<kC64> m[8];
+    load_matrix<kC64>(m, g);
+    const uint64_t amask = kFlip ? g.amask : 0;
+    const uint32_t pt = kFlip ? (par((base ^ toff) & amask) ^ g.beta) : 0u;
+    const bool a0 = kFlip && g.a0 != 0;
+    if (kFlip && g.rpar == 0u)
+    {
+        // the side does not change across the thread's registers (rpar is
+        // uniform): select the matrix once, for both unit halves (the
+        // operands are laundered: the compiler would otherwise turn the
+        // select into a run-time index into m)
+        Real<kC64> ml[8], mh[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++)
+        {
+            Real<kC64> keep = m[e], swap = m[e ^ 6];
+            asm volatile("" : "+v"(keep), "+v"(swap));
+            ml[e] = pt ? swap : keep;
+            mh[e] = (pt ^ uint32_t(a0)) ? swap : keep;
+        }
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+        {
+            if ((r >> i) & 1)
+                continue;
+            unit_pair2<kC64>(v[r], v[r | (1 << i)], ml, mh);
+        }
+        return;
+    }
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+    {
+        if ((r >> i) & 1)
+            continue;
+        const bool flip = kFlip && ((pt ^ (g.rpar >> r)) & 1u) != 0u;
+        unit_pair<kC64, kFlip>(v[r], v[r | (1 << i)], m, flip, a0);
+    }
+}
+
 // Gates on the register bits of layout L (and, in layout 0, the complex64
 // in-unit gate).  base ^ toff is the unit address of the thread's register 0.
 template <bool kC64, bool kFlip, int L, bool kFan = false>
@@ -312,43 +358,7 @@ __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs]
                 }
             }
         }
-        Real<kC64> m[8];
-        load_matrix<kC64>(m, g);
-        const uint64_t amask = kFlip ? g.amask : 0;
-        const uint32_t pt = kFlip ? (par((base ^ toff) & amask) ^ g.beta) : 0u;
-        const bool a0 = kFlip && g.a0 != 0;
-        if (kFlip && g.rpar == 0u)
-        {
-            // the side does not change across the thread's registers (rpar
-            // is uniform): select the matrix once, for both unit halves
-            // (the operands are laundered: the compiler would otherwise turn
-            // the select into a run-time index into m)
-            Real<kC64> ml[8], mh[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++)
-            {
-                Real<kC64> keep = m[e], swap = m[e ^ 6];
-                asm volatile("" : "+v"(keep), "+v"(swap));
-                ml[e] = pt ? swap : keep;
-                mh[e] = (pt ^ uint32_t(a0)) ? swap : keep;
-            }
-#pragma unroll
-            for (int r = 0; r < kRegs; r++)
-            {
-                if ((r >> i) & 1)
-                    continue;
-                unit_pair2<kC64>(v[r], v[r | (1 << i)], ml, mh);
-            }
-            continue;
-        }
-#pragma unroll
-        for (int r = 0; r < kRegs; r++)
-        {
-            if ((r >> i) & 1)
-                continue;
-            const bool flip = kFlip && ((pt ^ (g.rpar >> r)) & 1u) != 0u;
-            unit_pair<kC64, kFlip>(v[r], v[r | (1 << i)], m, flip, a0);
-        }
+        gate_on_reg_bit<kC64, kFlip>(v, i, g, base, toff);
     }
     if constexpr (kC64 && L == 0)
     {
@@ -744,29 +754,34 @@ __device__ __forceinline__ int upper_thread(int t)
 }
 
 // Passes whose gates all sit on unit bits 6..11 (complex64: and the in-unit
-// bit), without per-pair selectors or fans: one tile per workgroup and no
-// LDS, so the workgroups per CU are bound by registers instead of by a 64 KiB
-// tile buffer (more tiles in flight per CU).  Each thread holds the units of
-// layout-0 thread upper_thread(t), registers = unit bits 8..11 as in layout 0.
-// After the layout-0 gates the lane swaps make unit bits 6, 7 register bits 0,
-// 1 (and unit bits 8, 9 lane bits 4, 5) for their gates, and are undone
-// before the diagonal epilogue and the store.  The gates run in the tile
-// kernel's order (layout 0, then unit bits 6, 7) with the same arithmetic.
-template <bool kC64, bool kDiag>
+// bit), without fans: one tile per workgroup and no LDS, so the workgroups per
+// CU are bound by registers instead of by a 64 KiB tile buffer (more tiles in
+// flight per CU).  Each thread holds the units of layout-0 thread
+// upper_thread(t), registers = unit bits 8..11 as in layout 0.  After the
+// layout-0 gates the lane swaps make unit bits 6, 7 register bits 0, 1 (and
+// unit bits 8, 9 lane bits 4, 5) for their gates, and are undone before the
+// diagonal epilogue and the store.  The gates run in the tile kernel's order
+// (layout 0, then unit bits 6, 7) with the same arithmetic.  kFlip: per-pair
+// selectors; the rpar masks of slots 6 and 7 are then those of the swapped
+// registers (unit bits 6, 7, 10, 11), set by the host.
+template <bool kC64, bool kDiag, bool kFlip>
 __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ state, const TPass P,
                                                          const TTab *__restrict__ tabp,
                                                          const double *__restrict__ ttab,
                                                          const double *__restrict__ tphase)
 {
+    static_assert(!(kDiag && kFlip), "no diagonal epilogue with per-pair selectors");
     using T = typename UnitT<kC64>::T;
     const int tl = upper_thread(threadIdx.x);
     T *p = static_cast<T *>(state);
     const uint64_t tile = blockIdx.x;
-    const uint64_t at = deposit(tile, P.sorted) ^ thr_off<0>(tl, P);
+    const uint64_t base = deposit(tile, P.sorted);
+    const uint64_t toff = thr_off<0>(tl, P);
+    const uint64_t at = base ^ toff;
     ZXA_CONST TTab *G = (ZXA_CONST TTab *)tabp;
     T v[kRegs];
     load_tile<kC64>(v, p, at, G);
-    apply_layout<kC64, false, 0>(v, P, G, 0, 0);
+    apply_layout<kC64, kFlip, 0>(v, P, G, base, toff);
 #pragma unroll
     for (int r = 0; r < kRegs; r += 2)
         lane_swap<false>(v[r], v[r + 1]);
@@ -774,16 +789,14 @@ __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ stat
     for (int r = 0; r < kRegs; r++)
         if (!(r & 2))
             lane_swap<true>(v[r], v[r + 2]);
+    // unit address of register 0 in the swapped layout: lane bits 4, 5 now
+    // stand for unit bits 8, 9 instead of 6, 7
+    uint64_t toff_s = 0;
+    if constexpr (kFlip)
+        toff_s = toff ^ (((tl >> 6) & 1) ? P.vec[6] ^ P.vec[8] : 0) ^ (((tl >> 7) & 1) ? P.vec[7] ^ P.vec[9] : 0);
 #pragma unroll
     for (int i = 0; i < 2; i++)
-    {
-        Real<kC64> m[8];
-        load_matrix<kC64>(m, G->g[6 + i]);
-#pragma unroll
-        for (int r = 0; r < kRegs; r++)
-            if (!((r >> i) & 1))
-                unit_pair<kC64, false>(v[r], v[r | (1 << i)], m, false, false);
-    }
+        gate_on_reg_bit<kC64, kFlip>(v, i, G->g[6 + i], base, toff_s);
 #pragma unroll
     for (int r = 0; r < kRegs; r++)
         if (!(r & 2))
@@ -1295,23 +1308,35 @@ int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip,
 }
 
 // One upper_kernel workgroup per tile (ttab and tphase set: the diagonal
-// epilogue).
-int launch_u(Ctx &ctx, bool c64, void *state, const TPass &P, const TTab *tab, const double *ttab,
-             const double *tphase)
+// epilogue; flip: per-pair selectors).
+template <bool kC64>
+void launch_u_prec(Ctx &ctx, void *state, const TPass &P, const TTab *tab, const double *ttab,
+                   const double *tphase, bool flip)
 {
     const dim3 grid(unsigned(P.ntiles));
-    if (c64 && ttab)
-        hipLaunchKernelGGL((upper_kernel<true, true>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab, ttab,
-                           tphase);
-    else if (c64)
-        hipLaunchKernelGGL((upper_kernel<true, false>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab, nullptr,
-                           nullptr);
-    else if (ttab)
-        hipLaunchKernelGGL((upper_kernel<false, true>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab, ttab,
-                           tphase);
-    else
-        hipLaunchKernelGGL((upper_kernel<false, false>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab,
+    if (ttab)
+        hipLaunchKernelGGL((upper_kernel<kC64, true, false>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab,
+                           ttab, tphase);
+    else if (flip)
+        hipLaunchKernelGGL((upper_kernel<kC64, false, true>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab,
                            nullptr, nullptr);
+    else
+        hipLaunchKernelGGL((upper_kernel<kC64, false, false>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab,
+                           nullptr, nullptr);
+}
+
+int launch_u(Ctx &ctx, bool c64, void *state, const TPass &P, const TTab *tab, const double *ttab,
+             const double *tphase, bool flip)
+{
+    if (flip && ttab)
+    {
+        set_error("tile pass: diagonal epilogue with per-pair selectors");
+        return EINVAL_;
+    }
+    if (c64)
+        launch_u_prec<true>(ctx, state, P, tab, ttab, tphase, flip);
+    else
+        launch_u_prec<false>(ctx, state, P, tab, ttab, tphase, flip);
     return hip_check(hipGetLastError(), "upper_kernel launch");
 }
 
@@ -1852,15 +1877,12 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     }
     if (any_fan)
         ZXA_TRY(setup_fans(ctx, n, gs, fans, pl, P, G));
-    void *dtab = nullptr;
-    ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
-    const TTab *tab = static_cast<const TTab *>(dtab);
     // passes whose targets all sit on unit bits 6..11 run on upper_kernel (one
     // tile per workgroup, no LDS) when their highest target unit-address bit
     // is at least tile.upper.<prec>: spread over more than 2^k units, a tile's
     // runs are fetched faster with more tiles in flight per CU than the LDS
     // allows the tile kernel
-    bool upper = !flip && !any_fan && P.ntiles <= uint64_t(INT_MAX);
+    bool upper = !any_fan && P.ntiles <= uint64_t(INT_MAX);
     unsigned high = 0;
     for (size_t gi = 0; gi < gs.size() && upper; gi++)
     {
@@ -1871,10 +1893,34 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         high = std::max(high, P.ub[slot]);
     }
     upper = upper && high >= unsigned(tune_get(c64 ? "tile.upper.c64" : "tile.upper.c128"));
+    if (upper && flip)
+    {
+        // the upper kernel applies slots 6 and 7 with registers = unit bits
+        // 6, 7, 10, 11: their parity masks over those register offsets
+        const uint64_t sv[4] = {P.vec[6], P.vec[7], P.vec[10], P.vec[11]};
+        for (int own = 0; own < 2; own++)
+        {
+            TGate &tg = G.g[6 + own];
+            tg.rpar = 0;
+            for (int r = 0; r < kRegs; r++)
+            {
+                if ((r >> own) & 1)
+                    continue;
+                uint64_t off = 0;
+                for (int i = 0; i < 4; i++)
+                    if ((r >> i) & 1)
+                        off ^= sv[i];
+                tg.rpar |= uint32_t(__builtin_popcountll(off & tg.amask) & 1) << r;
+            }
+        }
+    }
+    void *dtab = nullptr;
+    ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
+    const TTab *tab = static_cast<const TTab *>(dtab);
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
     if (upper)
-        ZXA_TRY(launch_u(ctx, c64, state, P, tab, ttab, tphase));
+        ZXA_TRY(launch_u(ctx, c64, state, P, tab, ttab, tphase, flip));
     else if (c64)
         ZXA_TRY(launch_p<true>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
     else
