@@ -30,9 +30,13 @@ Reported beside it (same JSON line):
                 complex64) in one batched engine replayed from a hipGraph
   zxh_workloads the reference's own 30-45-qubit benchmark circuits (QASM),
                 execute() time in lazy ZXH mode and in the plain method
-  cpu_baseline  the reference's CPU path (oracle/_ref when built, else the
-                oracle port) on a bounded prefix of the same circuit, timed on
-                this host
+  c1            the 20-qubit plain random circuit (1180 gates) and
+                Sampling(1024) on the GPU, execute() and Sampling() timed apart
+  cpu_baseline  the oracle's restatement of the reference's OpenMP CPU path
+                (oracle/liboracle_base.so) in child processes on this host, at
+                full threads and at one thread: C1 in full, the 30-qubit U3 and
+                diagonal passes on bounded samples, the QFT-30 step estimated
+                from them (value), with nproc / lscpu
 
 roofline.traffic is the HBM bytes per launch of the dominant kernel as the
 two counter runs summarised in profiles/pmc_traffic_qft30_c128.txt measured
@@ -394,26 +398,124 @@ def c5_circuit(W, K, prec, n, layers=40, shots=1000000, seed=5):
             "mfma_peak_tflops": peak, "sampling": {"shots": shots, "seconds": round(t_samp, 3)}}
 
 
-def cpu_baseline(W, targets=4):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle_py as O
-    gates = []
-    for t in range(targets):
-        gates.append((W.H, 0, t, 0.0, 0.0, 0.0))
-        for j in range(t):
-            gates.append((W.CP, j, t, 3.141592653589793 / (2 ** (t - j)), 0.0, 0.0))
-    kind = "reference" if O.ref_available() else "port"
-    cls = O.RefSim if kind == "reference" else O.OracleSim
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    sim = cls(NQ, False, True)
+def accuracy(sim64, sim128):
+    """Infidelity 1 - |<psi64|psi128>| and max amplitude error of a complex64
+    simulator's physical state against a complex128 one that ran the same
+    operations (both plain method, same width)."""
+    from zxh_adae_amd.kernels import DeviceState
+    p64, m64 = sim64.device_state()
+    p128, m128 = sim128.device_state()
+    assert m64 == m128
+    c = DeviceState.wrap(p64, m64, "c64").compare(DeviceState.wrap(p128, m128, "c128"))
+    return {"infidelity": c["infidelity"], "max_amp_error": c["max_amp_error"],
+            "norm2_c64": c["norm2_self"]}
+
+
+def c5_accuracy(W, K, n, layers=40, seed=5):
+    """The large random circuit's layers (blocks, diagonal batch, MCX cascade)
+    at a width where both precisions fit together: the complex64 state
+    against the complex128 one after all layers."""
+    from zxh_adae_amd.kernels import DeviceState, apply_layer, set_basis_zero
+    spec = W.c5_layers(n, layers, seed)
+    a, b = DeviceState(n, "c64"), DeviceState(n, "c128")
+    for st in (a, b):
+        set_basis_zero(st)
+        for L in spec:
+            apply_layer(st, L)
+    c = a.compare(b)
+    a.free()
+    b.free()
+    return {"qubits": n, "layers": layers, "infidelity": c["infidelity"], "max_amp_error": c["max_amp_error"],
+            "norm2_c64": c["norm2_self"], "norm2_c128": c["norm2_other"]}
+
+
+def host_info():
+    """Core counts of this host: nproc, the affinity set, and lscpu's sockets x
+    cores per socket x threads per core."""
+    import subprocess
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        keys = {"Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+                "Thread(s) per core": "threads_per_core", "Model name": "model", "CPU(s)": "cpus"}
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in keys:
+                info[keys[k.strip()]] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        info["lscpu"] = "unavailable"
+    return info
+
+
+def _cpu_run(threads, extra, timeout_s):
+    import subprocess
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), *extra]
+    r = subprocess.run(cmd, env=env, stdout=subprocess.PIPE, text=True, timeout=timeout_s)
+    if r.returncode != 0:
+        raise RuntimeError(f"cpu baseline child failed ({r.returncode})")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(c1_gpu):
+    """The oracle's restatement of the reference's CPU path (oracle/
+    liboracle_base.so, OpenMP), in child processes on this host: once on the
+    cores this process may use and once on one thread.  value = the QFT-30
+    step as the reference schedules it, estimated from the measured per-pass
+    times at full threads (gates/s)."""
+    host = host_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or host["affinity"]
+    log(f"cpu baseline: {threads} threads")
+    full = _cpu_run(threads, ["--c1-repeats", "5", "--pass-repeats", "5"], 400)
+    log("cpu baseline: 1 thread")
+    one = _cpu_run(1, ["--c1-repeats", "5", "--pass-repeats", "3", "--slice-shift", "3", "--c1-budget-s", "25"], 400)
+    q = full["qft30"]
+    out = {"value": q["gates/s"], "unit": "gates/s", "cores": threads, "kind": "port",
+           "sample": (f"QFT-30 complex128 step as the reference runs it (reset + 30 U3 sweeps + 29 diagonal "
+                      f"windows), {q['estimated_ms'] / 1e3:.1f} s estimated from per-pass times measured on "
+                      f"bounded samples of the 2^30 state (2^28 amplitudes for U3, B x 2^k = 2^28 selector "
+                      f"evaluations per window; 1 warm-up + 5 runs each); C1 (20q, 1180 gates, 1024 shots) "
+                      f"timed in full"),
+           "build": "oracle/liboracle_base.so (C restatement of the reference's OpenMP CPU backend, "
+                    "gcc -O3 -fopenmp -ffp-contract=off, selector calls out of line)",
+           "host": host, "threads": {"full": full, "one": one}}
+    if c1_gpu:
+        g = c1_gpu
+        out["c1_gpu_over_cpu"] = {
+            "execute": round(full["c1"]["execute"]["mean_ms"] / g["execute"]["mean_ms"], 1),
+            "sampling": round(full["c1"]["sampling"]["mean_ms"] / g["sampling"]["mean_ms"], 1)}
+    return out
+
+
+def c1_gpu(Z, W, repeats=5):
+    """C1 on the GPU through the drop-in: 20 qubits, plain method, complex128,
+    1180 gates, then Sampling(1024) with set_seed(31415926); execute() and
+    Sampling() timed separately (1 warm-up, then repeats runs)."""
+    import statistics
+    gates = W.random_layers(20, 20, 1)
+    sim = Z.ZXH(20, False, True)
     sim.gates(gates)
-    t0 = time.perf_counter()
+    sim.set_seed(31415926)
     sim.execute()
-    dt = time.perf_counter() - t0
+    sim.Sampling(1024)
+    te, ts = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        sim.execute()
+        t1 = time.perf_counter()
+        sim.measure(1024)
+        sim.results_bits()
+        t2 = time.perf_counter()
+        te.append(t1 - t0)
+        ts.append(t2 - t1)
     del sim
-    return {"value": round(len(gates) / dt, 3), "unit": "gates/s", "cores": cores, "kind": kind,
-            "sample": f"first {targets} targets of the 30-qubit QFT ({len(gates)} gates, complex128, "
-                      f"one execute() incl. its 16 GiB state reset), {dt:.1f} s"}
+
+    def st(xs):
+        return {"mean_ms": round(1e3 * statistics.fmean(xs), 4), "sd_ms": round(1e3 * statistics.stdev(xs), 4),
+                "runs": len(xs)}
+    ex = st(te)
+    return {"qubits": 20, "gates": len(gates), "shots": 1024, "execute": ex, "sampling": st(ts),
+            "gates/s": round(len(gates) / (ex["mean_ms"] * 1e-3), 1)}
 
 
 def kernels_only(Z, K, W, repeats, prec):
@@ -550,14 +652,32 @@ def main():
     del simr
 
     if not args.no_extras:
+        # complex64: the same QFT step, then each c64 result compared with the
+        # complex128 state that went through the same operations (infidelity
+        # 1 - |<psi64|psi128>| and max |psi64_i - psi128_i|, one device pass)
+        log(f"QFT-{NQ} c64")
+        sim64 = Z.ZXH(NQ, False, True, precision="c64", policy="fused")
+        sim64.gates(W.hadamard_all(NQ))
+        sim64.execute()
+        tuned["c64"] = autotune.tune_diag(wrap_state(sim64.device_state(), "c64"))
+        tuned["tile_c64"] = autotune.tune_tile(wrap_state(sim64.device_state(), "c64"))
+        sim64, _, t64, stats64 = qft_step(Z, W, "c64", args.steps, 1, sim=sim64)
+        result["c64"] = {"gates/s": round(ngates / t64, 3), "ms_per_step": round(t64 * 1e3, 3),
+                         **accuracy(sim64, sim)}
         # uniform superposition input for the diagonal batches and the U3 pass
-        sim.clear_gates()
-        sim.gates(W.hadamard_all(NQ))
-        sim.execute()
-        log("diag batches c128")
-        result["diag_batches"] = {"c128": diag_batches(Z, W, K, sim.device_state(), "c128")}
-        log("u3 all c128")
-        result["u3_all"] = {"c128": u3_all(Z, W, K, sim.device_state(), "c128")}
+        for s_ in (sim, sim64):
+            s_.clear_gates()
+            s_.gates(W.hadamard_all(NQ))
+            s_.execute()
+        log("diag batches c128, c64")
+        result["diag_batches"] = {"c128": diag_batches(Z, W, K, sim.device_state(), "c128"),
+                                  "c64": diag_batches(Z, W, K, sim64.device_state(), "c64")}
+        result["diag_batches"]["c64"].update(accuracy(sim64, sim))
+        log("u3 all c128, c64")
+        result["u3_all"] = {"c128": u3_all(Z, W, K, sim.device_state(), "c128"),
+                            "c64": u3_all(Z, W, K, sim64.device_state(), "c64")}
+        result["u3_all"]["c64"].update(accuracy(sim64, sim))
+        del sim64
         log("sampling 1e6 shots")
         sim.set_seed(5)
         sim.set_timing(True)
@@ -567,20 +687,6 @@ def main():
                               "search_kernel_ms": round(sim.timing("sample")["ms"], 3)}
         sim.set_timing(False)
         del sim
-        log(f"QFT-{NQ} c64")
-        sim64 = Z.ZXH(NQ, False, True, precision="c64", policy="fused")
-        sim64.gates(W.hadamard_all(NQ))
-        sim64.execute()
-        tuned["c64"] = autotune.tune_diag(wrap_state(sim64.device_state(), "c64"))
-        tuned["tile_c64"] = autotune.tune_tile(wrap_state(sim64.device_state(), "c64"))
-        sim64, _, t64, stats64 = qft_step(Z, W, "c64", args.steps, 1, sim=sim64)
-        result["c64"] = {"gates/s": round(ngates / t64, 3), "ms_per_step": round(t64 * 1e3, 3)}
-        sim64.clear_gates()
-        sim64.gates(W.hadamard_all(NQ))
-        sim64.execute()
-        result["diag_batches"]["c64"] = diag_batches(Z, W, K, sim64.device_state(), "c64")
-        result["u3_all"]["c64"] = u3_all(Z, W, K, sim64.device_state(), "c64")
-        del sim64
 
     if not args.no_extras:
         log("adjoint 28q c64 (two-local, 8 layers, 200 terms)")
@@ -593,10 +699,15 @@ def main():
         result["c5"] = {"c64": c5_circuit(W, K, "c64", 33)}
         log("large random circuit 32q c128, 40 layers, 1e6 shots")
         result["c5"]["c128"] = c5_circuit(W, K, "c128", 32)
+        log("large random circuit at 31q in both precisions (c64 accuracy)")
+        acc = c5_accuracy(W, K, 31)
+        result["c5"]["c64"].update({"infidelity": acc["infidelity"], "max_amp_error": acc["max_amp_error"],
+                                    "accuracy_at_qubits": 31, "accuracy_31q": acc})
 
+    log("C1: 20q plain c128, 1180 gates, Sampling(1024)")
+    result["c1"] = c1_gpu(Z, W)
     if not args.no_cpu_baseline:
-        log("cpu baseline")
-        result["cpu_baseline"] = cpu_baseline(W)
+        result["cpu_baseline"] = cpu_baseline(result["c1"])
     print(json.dumps(result), flush=True)
 
 

@@ -121,7 +121,12 @@ int zxh_num_qubits(const zxh_sim *s, size_t *out); /* ZXH::num_qubits */
 
 /* ---- extensions over the reference's class (no reference counterpart) ---- */
 int zxh_set_policy(zxh_sim *s, int policy); /* ZXH_POLICY_*; default ZXH_POLICY_FUSED */
-/* sampling segment bits, the ZXHSIM_SEG_MIN_I setting (sv_common.cpp:62-100) */
+/* sampling segment bits I (segments of 2^min(M, I) amplitudes,
+ * sv_common.cpp:87-100).  zxh_create reads the default from the environment
+ * variable ZXHSIM_SEG_MIN_I with the reference's parse rules
+ * (parse_env_size_t, sv_common.cpp:49-66: decimal, the whole string; unset,
+ * empty, trailing text or 0 give 18); this call overrides it for one
+ * simulator (0 restores 18). */
 int zxh_set_seg_bits(zxh_sim *s, size_t bits);
 /* raw physical state: 2^m amplitudes of the simulator's precision (host copy) */
 int zxh_get_phys_state(const zxh_sim *s, void *out, size_t out_bytes, size_t *m_out);
@@ -215,6 +220,12 @@ int zxh_k_expect_z(const void *state, int precision, size_t local_bits, const in
  * step.  No reference counterpart. */
 int zxh_k_apply_z(const void *src, void *dst, int precision, size_t local_bits, const int32_t *qi,
                   const int32_t *qj, const double *coef, size_t nterms, double *out);
+/* Two states of 2^local_bits amplitudes (each complex128 or complex64),
+ * read once each: out5 = { Re <a|b>, Im <a|b>, |a|^2, |b|^2 (FP64 with
+ * compensated, fixed-order sums), max_i |a_i - b_i| }.  Gives a complex64
+ * result's infidelity 1 - |<a|b>| and maximum amplitude error against the
+ * complex128 run of the same circuit.  No reference counterpart. */
+int zxh_k_compare(const void *a, int precision_a, const void *b, int precision_b, size_t local_bits, double *out5);
 /* |0...0> in place: worker_set_zero + worker_mem_set (mem.h:9-19) */
 int zxh_k_reset(void *state, int precision, size_t local_bits);
 /* Dense 5-qubit unitary on physical bits targets[0..5) (bit b of the

@@ -1,0 +1,200 @@
+"""CPU baseline: the oracle's restatement of the reference's CPU path, timed on
+this host's cores (test infrastructure; run by bench.py as a child process).
+
+TEST INFRASTRUCTURE ONLY.  It times oracle/liboracle_base.so, the oracle's C
+restatement of the reference's OpenMP CPU backend built the way the reference
+builds it (g++/gcc -O3 -fopenmp -ffp-contract=off, selector evaluation out of
+line as selector.cpp:26-38 with no interprocedural optimisation), never the
+product and never the reference's own sources.
+
+Rows (one JSON line on stdout):
+  c1      the 20-qubit plain-method random circuit (20 layers of U3 on every
+          qubit, a CP ladder, P on every qubit: 1180 gates), complex128, then
+          Sampling(1024) with set_seed(31415926): execute() and Sampling()
+          timed separately, as the reference's benchmark runner does
+          (run_backend_zxh_cuda.py:85-106); 1 warm-up, then `repeats` runs.
+  passes  30-qubit complex128 passes on a uniform-superposition state: one U3
+          sweep (kernel_omp.cpp:190-222) and diagonal windows of B random P/CP
+          gates, B in {1, 4, 16, 64, 256} (kernel_omp.cpp:72-115).  A pass
+          does the same work on every amplitude, so each is timed over the
+          first 2^k amplitudes (a bounded sample: 2^(28 - shift) amplitudes for
+          the U3 sweep, B x 2^k = 2^(28 - shift) selector evaluations for a
+          window) and scaled by 2^(30 - k); `amplitudes_timed` gives 2^k.
+  reset   execute() of an empty 28-qubit plain circuit (the reference's reset,
+          a memset of the state, sv_common.cpp:286-295), scaled by 4 to the
+          30-qubit state.
+  qft30   the QFT-30 step as the reference runs it (one reset, 30 one-gate U3
+          sweeps, 29 diagonal windows of t = 1..29 CP gates), estimated from
+          the per-pass times: window cost interpolated linearly in B between
+          the measured B.
+
+    OMP_NUM_THREADS=16 OMP_PROC_BIND=close OMP_PLACES=cores \\
+        python oracle/cpu_baseline.py --c1-repeats 5 --pass-repeats 5
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import statistics
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+sys.path.insert(0, os.path.dirname(HERE))
+
+import numpy as np  # noqa: E402
+
+import oracle_py as O  # noqa: E402
+
+O.ORACLE_SO = os.path.join(HERE, "liboracle_base.so")
+
+from zxh_adae_amd import workloads as W  # noqa: E402  (pure-Python circuit generators)
+
+NQ = 30
+C1_SEED, C1_SHOTS, C1_SAMPLE_SEED = 1, 1024, 31415926
+
+
+def log(msg: str) -> None:
+    print(f"[cpu_baseline] {msg}", file=sys.stderr, flush=True)
+
+
+def _stats(xs):
+    return {"mean_ms": round(1e3 * statistics.fmean(xs), 3),
+            "sd_ms": round(1e3 * (statistics.stdev(xs) if len(xs) > 1 else 0.0), 3), "runs": len(xs)}
+
+
+def c1(repeats: int, budget_s: float):
+    """C1; the repeat count is lowered (not below 2) when the warm-up run says
+    that `repeats` runs would take longer than budget_s."""
+    gates = W.random_layers(20, 20, C1_SEED)
+    s = O.OracleSim(20, False, True)
+    s.gates(gates)
+    s.set_seed(C1_SAMPLE_SEED)
+    lib = s.lib
+    packed = np.zeros(C1_SHOTS, dtype=np.uint64)
+
+    def sampling():
+        if lib.zo_measure(s.h, C1_SHOTS) or lib.zo_get_results(s.h, O._u64ptr(packed), C1_SHOTS):
+            raise RuntimeError("oracle sampling failed")
+
+    t0 = time.perf_counter()
+    s.execute()
+    sampling()
+    warm = time.perf_counter() - t0
+    repeats = max(2, min(repeats, int(budget_s / max(warm, 1e-9))))
+    log(f"c1 warm-up {warm:.2f} s; {repeats} timed runs")
+    te, ts = [], []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        s.execute()
+        t1 = time.perf_counter()
+        sampling()
+        t2 = time.perf_counter()
+        te.append(t1 - t0)
+        ts.append(t2 - t1)
+    ex, sa = _stats(te), _stats(ts)
+    return {"qubits": 20, "gates": len(gates), "shots": C1_SHOTS, "execute": ex, "sampling": sa,
+            "gates/s": round(len(gates) / (ex["mean_ms"] * 1e-3), 2),
+            "shots/s": round(C1_SHOTS / (sa["mean_ms"] * 1e-3), 1)}
+
+
+def _time_pass(fn, repeats):
+    fn()  # warm-up
+    xs = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        fn()
+        xs.append(time.perf_counter() - t0)
+    return xs
+
+
+def passes(repeats: int, shift: int):
+    """Per-pass times at 30 qubits complex128, each timed over a bounded
+    sample of 2^k amplitudes and scaled to 2^30."""
+    lib = O.oracle_lib()
+    top = 28 - shift
+    psi = np.full(1 << top, 2.0 ** (-NQ / 2), dtype=np.complex128)
+    buf = psi.view(np.float64)
+    out = {}
+    u = W.u3_matrix(0.3, 0.2, 0.1)
+    d = O.U3Desc()
+    d.packed_sel = 1 << (top - 1)
+    d.delta = d.packed_sel
+    for i, v in enumerate((u[0, 0], u[0, 1], u[1, 0], u[1, 1])):
+        d.u[2 * i], d.u[2 * i + 1] = v.real, v.imag
+    xs = _time_pass(lambda: lib.zo_k_u3_batch(O._dptr(buf), top, C.byref(d), 1), repeats)
+    scale = float(1 << (NQ - top))
+    out["u3"] = {"amplitudes_timed": 1 << top, **_stats([x * scale for x in xs])}
+    for B in (1, 4, 16, 64, 256):
+        kb = top - int(math.log2(B))
+        gs = W.diag_batch(NQ, B, 100 + B)
+        s0 = np.array([1 << g[2] for g in gs], dtype=np.uint64)
+        s1 = np.array([0xFFFFFFFFFFFFFFFF if g[0] == W.P else (1 << g[1]) for g in gs], dtype=np.uint64)
+        th = np.array([g[3] for g in gs])
+
+        def run(kb=kb, s0=s0, s1=s1, th=th):
+            rc = lib.zo_k_diag(O._dptr(buf), kb, O._u64ptr(s0), O._u64ptr(s1), O._dptr(th), th.size)
+            if rc:
+                raise RuntimeError("oracle diag rejected the window")
+
+        xs = _time_pass(run, repeats)
+        scale = float(1 << (NQ - kb))
+        out[f"diag_B{B}"] = {"amplitudes_timed": 1 << kb, **_stats([x * scale for x in xs])}
+    del psi, buf
+    return out
+
+
+def reset(repeats: int):
+    s = O.OracleSim(NQ - 2, False, True)
+    xs = _time_pass(s.execute, repeats)
+    del s
+    return {"amplitudes_timed": 1 << (NQ - 2), **_stats([4.0 * x for x in xs])}
+
+
+def qft_estimate(p, reset_ms):
+    Bs = [1, 4, 16, 64, 256]
+    ms = [p[f"diag_B{b}"]["mean_ms"] for b in Bs]
+
+    def window(b):
+        for i in range(len(Bs) - 1):
+            if Bs[i] <= b <= Bs[i + 1]:
+                f = (b - Bs[i]) / (Bs[i + 1] - Bs[i])
+                return ms[i] + f * (ms[i + 1] - ms[i])
+        return ms[-1]
+
+    total = reset_ms + NQ * p["u3"]["mean_ms"] + sum(window(t) for t in range(1, NQ))
+    gates = NQ + NQ * (NQ - 1) // 2
+    return {"gates": gates, "estimated_ms": round(total, 1), "gates/s": round(gates / (total * 1e-3), 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--c1-repeats", type=int, default=5)
+    ap.add_argument("--pass-repeats", type=int, default=5)
+    ap.add_argument("--slice-shift", type=int, default=0,
+                    help="time each pass over 2^shift times fewer amplitudes (scaled to 2^30)")
+    ap.add_argument("--c1-budget-s", type=float, default=20.0)
+    ap.add_argument("--no-passes", action="store_true")
+    args = ap.parse_args()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    res = {"threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "omp_places": os.environ.get("OMP_PLACES"), "build": "oracle/liboracle_base.so"}
+    t0 = time.perf_counter()
+    log(f"{threads} threads: C1")
+    res["c1"] = c1(args.c1_repeats, args.c1_budget_s)
+    if not args.no_passes:
+        log(f"{threads} threads: 30-qubit passes (sample shift {args.slice_shift})")
+        res["passes"] = passes(args.pass_repeats, args.slice_shift)
+        log(f"{threads} threads: reset")
+        res["reset"] = reset(args.pass_repeats)
+        res["qft30"] = qft_estimate(res["passes"], res["reset"]["mean_ms"])
+    res["wall_s"] = round(time.perf_counter() - t0, 1)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -72,6 +72,21 @@ static inline int sel_eval(uint64_t packed, uint64_t x)
     return parity != sel_beta(packed);
 }
 
+/* Selector evaluation where the reference calls selector_t::eval/eval_u64
+ * (diagonal window, X, CX and CP kernels; kernel_omp.cpp:84-109, 155, 177,
+ * 387).  Those are defined in their own translation unit (selector.cpp:26-38)
+ * and the reference builds without interprocedural optimisation
+ * (CMakeLists.txt:483), so every evaluation is a call.  The CPU-baseline build
+ * (-DZO_REFERENCE_COST, oracle/liboracle_base.so) keeps that call
+ * (zo_selector.c) so that its timing matches the reference's; the checker
+ * build inlines it.  Both give identical results. */
+#ifdef ZO_REFERENCE_COST
+int zo_sel_eval_call(uint64_t packed, uint64_t x);
+#define SEL_EVAL_CALL(p, x) zo_sel_eval_call((p), (x))
+#else
+#define SEL_EVAL_CALL(p, x) sel_eval((p), (x))
+#endif
+
 static inline uint64_t low_mask(size_t nbits)
 {
     if (nbits == 0)
@@ -119,7 +134,7 @@ void zo_k_x(double *psi, size_t local_bits, uint64_t delta, uint64_t packed_sel)
     const size_t n = (size_t)1 << local_bits;
     cplx *p = (cplx *)psi;
     for (size_t off = 0; off < n; off++) {
-        if (sel_eval(packed_sel, off))
+        if (SEL_EVAL_CALL(packed_sel, off))
             continue;
         const size_t peer = off ^ delta;
         if (peer >= n)
@@ -136,7 +151,7 @@ void zo_k_cx(double *psi, size_t local_bits, uint64_t delta, uint64_t packed_cq,
     const size_t n = (size_t)1 << local_bits;
     cplx *p = (cplx *)psi;
     for (size_t off = 0; off < n; off++) {
-        if (sel_eval(packed_q, off) || !sel_eval(packed_cq, off))
+        if (SEL_EVAL_CALL(packed_q, off) || !SEL_EVAL_CALL(packed_cq, off))
             continue;
         const size_t peer = off ^ delta;
         if (peer >= n)
@@ -154,7 +169,7 @@ void zo_k_cp(double *psi, size_t local_bits, uint64_t packed_cq, uint64_t packed
     cplx *p = (cplx *)psi;
     const cplx ph = c_phase(theta);
     for (size_t off = 0; off < n; off++) {
-        if (sel_eval(packed_cq, off) && sel_eval(packed_q, off))
+        if (SEL_EVAL_CALL(packed_cq, off) && SEL_EVAL_CALL(packed_q, off))
             p[off] = c_mul(p[off], ph);
     }
 }
@@ -249,22 +264,22 @@ static void dw_apply(const diag_window *w, double *psi, size_t n)
         const uint64_t slice = xr & ~(uint64_t)0xffu;
         double th = 0.0;
         for (size_t k = 0; k < w->n[B_PL]; k++)
-            if (sel_eval(w->e[B_PL][k].s0, lane))
+            if (SEL_EVAL_CALL(w->e[B_PL][k].s0, lane))
                 th += w->e[B_PL][k].theta;
         for (size_t k = 0; k < w->n[B_PH]; k++)
-            if (sel_eval(w->e[B_PH][k].s0, slice))
+            if (SEL_EVAL_CALL(w->e[B_PH][k].s0, slice))
                 th += w->e[B_PH][k].theta;
         for (size_t k = 0; k < w->n[B_PX]; k++)
-            if (sel_eval(w->e[B_PX][k].s0, xr))
+            if (SEL_EVAL_CALL(w->e[B_PX][k].s0, xr))
                 th += w->e[B_PX][k].theta;
         for (size_t k = 0; k < w->n[B_CPLL]; k++)
-            if (sel_eval(w->e[B_CPLL][k].s0, lane) && sel_eval(w->e[B_CPLL][k].s1, lane))
+            if (SEL_EVAL_CALL(w->e[B_CPLL][k].s0, lane) && SEL_EVAL_CALL(w->e[B_CPLL][k].s1, lane))
                 th += w->e[B_CPLL][k].theta;
         for (size_t k = 0; k < w->n[B_CPHH]; k++)
-            if (sel_eval(w->e[B_CPHH][k].s0, slice) && sel_eval(w->e[B_CPHH][k].s1, slice))
+            if (SEL_EVAL_CALL(w->e[B_CPHH][k].s0, slice) && SEL_EVAL_CALL(w->e[B_CPHH][k].s1, slice))
                 th += w->e[B_CPHH][k].theta;
         for (size_t k = 0; k < w->n[B_CPHL]; k++)
-            if (sel_eval(w->e[B_CPHL][k].s0, lane) && sel_eval(w->e[B_CPHL][k].s1, slice))
+            if (SEL_EVAL_CALL(w->e[B_CPHL][k].s0, lane) && SEL_EVAL_CALL(w->e[B_CPHL][k].s1, slice))
                 th += w->e[B_CPHL][k].theta;
         if (th != 0.0)
             p[ii] = c_mul(p[ii], c_phase(th));

@@ -577,3 +577,31 @@ def test_gather_reads_the_addressed_amplitudes(gpu_engine, prec):
     assert d.gather(np.zeros(0, dtype=np.uint64)).size == 0
     with pytest.raises(ZXHError):
         d.gather(np.array([4096], dtype=np.uint64))
+
+
+# ZXHSIM_SEG_MIN_I read by the drop-in at construction, with the reference's
+# parse rules (parse_env_size_t, sv_common.cpp:49-66): decimal strtoull of the
+# whole string (leading blanks and a sign accepted); 0, empty or trailing text
+# give the default 18.  The indices must equal the reference's for that
+# segment size on the same state and seed.
+@pytest.mark.parametrize("env,seg", [("1", 1), ("2", 2), ("4", 4), ("18", 18), (" 2", 2), ("+4", 4),
+                                     ("0", 18), ("4x", 18), ("", 18), (None, 18)])
+def test_seg_min_i_environment_through_zxh(gpu_engine, monkeypatch, env, seg):
+    if env is None:
+        monkeypatch.delenv("ZXHSIM_SEG_MIN_I", raising=False)
+    else:
+        monkeypatch.setenv("ZXHSIM_SEG_MIN_I", env)
+    gold = np.load(os.path.join(GOLDEN, f"samp_seg{seg}.npz"))
+    for key in gold.files:
+        name, seed = key.split("__")
+        psi = SAMPLER_STATES[name]
+        m = psi.size.bit_length() - 1
+        if m == 0:
+            continue  # the drop-in needs at least one qubit
+        sim = gpu_engine.ZXH(m, False, True)
+        sim.execute()
+        sim.set_phys_state(psi)
+        sim.set_seed(int(seed))
+        got = sim.sample_real(gold[key].size)
+        assert np.array_equal(got, gold[key]), (env, name, seed)
+        del sim

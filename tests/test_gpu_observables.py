@@ -129,3 +129,51 @@ def test_observable_rejects_bad_qubits(gpu_engine):
         d.expect_z([(1.0, 8)])
     with pytest.raises(ZXHError):
         d.expect_z([(1.0, 0, 9)])
+
+
+# ---------------- state comparison (complex64 result vs complex128) ----------------
+
+@pytest.mark.parametrize("n", [1, 5, 12, 17])
+@pytest.mark.parametrize("pa,pb", [("c64", "c128"), ("c128", "c128"), ("c64", "c64"), ("c128", "c64")])
+def test_compare_matches_exact_sums(gpu_engine, n, pa, pb):
+    """<a|b>, |a|^2, |b|^2 against math.fsum over the same (widened) values;
+    the max amplitude error is exact (an order-free max)."""
+    from zxh_adae_amd.kernels import DeviceState
+    rng = np.random.default_rng(7 * n + len(pa) + 3 * len(pb))
+    a = rand_state(rng, n, np.complex64 if pa == "c64" else np.complex128)
+    b = (a.astype(np.complex128) + 1e-4 * rand_state(rng, n)).astype(np.complex64 if pb == "c64" else np.complex128)
+    da, db = DeviceState(n, pa), DeviceState(n, pb)
+    da.upload(a)
+    db.upload(b)
+    got = da.compare(db)
+    a2, b2 = a.astype(np.complex128), b.astype(np.complex128)
+    ov = np.conj(a2) * b2
+    want_re, want_im = math.fsum(ov.real), math.fsum(ov.imag)
+    # compensated per-thread sums, then fixed-order trees: a few ulps of the
+    # unit-norm totals
+    assert abs(got["overlap"].real - want_re) <= 1e-14
+    assert abs(got["overlap"].imag - want_im) <= 1e-14
+    assert abs(got["norm2_self"] - math.fsum(np.abs(a2) ** 2)) <= 1e-14
+    assert abs(got["norm2_other"] - math.fsum(np.abs(b2) ** 2)) <= 1e-14
+    assert got["max_amp_error"] == pytest.approx(float(np.max(np.abs(a2 - b2))), rel=1e-14, abs=0)
+    assert got["infidelity"] == pytest.approx(1.0 - abs(complex(want_re, want_im)), abs=1e-14)
+
+
+def test_compare_identical_and_phase_rotated(gpu_engine):
+    """A state against itself: zero error, overlap = norm; against i*state:
+    overlap i*|a|^2 (so the infidelity ignores a global phase) and
+    max error sqrt(2) max|a_i|."""
+    from zxh_adae_amd.kernels import DeviceState
+    rng = np.random.default_rng(11)
+    n = 18
+    a = rand_state(rng, n)
+    da, db = DeviceState(n, "c128"), DeviceState(n, "c128")
+    da.upload(a)
+    db.upload(a)
+    same = da.compare(db)
+    assert same["max_amp_error"] == 0.0
+    assert abs(same["infidelity"]) < 1e-14
+    db.upload(1j * a)
+    rot = da.compare(db)
+    assert abs(rot["overlap"] - 1j * same["norm2_self"]) < 1e-14
+    assert rot["max_amp_error"] == pytest.approx(math.sqrt(2) * float(np.max(np.abs(a))), rel=1e-14)

@@ -86,6 +86,7 @@ _SIGS = [
     ("zxh_k_expect_z", C.c_int, [_vp, C.c_int, _sz, _i32p, _i32p, _dp, _sz, _dp]),
     ("zxh_k_block5", C.c_int, [_vp, C.c_int, _sz, C.POINTER(C.c_uint32), _dp]),
     ("zxh_k_reset", C.c_int, [_vp, C.c_int, _sz]),
+    ("zxh_k_compare", C.c_int, [_vp, C.c_int, _vp, C.c_int, _sz, _dp]),
     ("zxh_ansatz_create", C.c_int, [_sz, C.c_int, _vp, _sz, _sz, C.POINTER(_vp)]),
     ("zxh_ansatz_destroy", C.c_int, [_vp]),
     ("zxh_ansatz_set_observable", C.c_int, [_vp, _i32p, _i32p, _dp, _sz]),

@@ -553,6 +553,22 @@ int zxh_k_apply_z(const void *src, void *dst, int precision, size_t local_bits, 
     return observable_call(src, dst, precision, local_bits, qi, qj, coef, nterms, out);
 }
 
+int zxh_k_compare(const void *a, int precision_a, const void *b, int precision_b, size_t local_bits, double *out5)
+{
+    int rc = check_kernel_args(a, precision_a, local_bits);
+    if (rc)
+        return rc;
+    rc = check_kernel_args(b, precision_b, local_bits);
+    if (rc)
+        return rc;
+    if (!out5)
+        return fail(ZXH_EINVAL, "null output");
+    if (local_bits < 1)
+        return fail(ZXH_EINVAL, "compare: state width must be at least 1 bit");
+    WITH_CTX(c);
+    return finish(c, zxa::compare_states(*c, a, precision_a, b, precision_b, unsigned(local_bits), out5));
+}
+
 int zxh_k_reset(void *state, int precision, size_t local_bits)
 {
     int rc = check_kernel_args(state, precision, local_bits);

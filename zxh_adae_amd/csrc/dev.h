@@ -389,6 +389,11 @@ int prob_total(Ctx &ctx, const void *state, int prec, unsigned m, double *out);
 int observable_apply(Ctx &ctx, const void *src, void *dst, int prec, unsigned nbits, const int32_t *qi,
                      const int32_t *qj, const double *coef, size_t nterms, double *out2);
 
+// Compare two states of 2^n amplitudes (either precision each): out5 (host)
+// receives Re <a|b>, Im <a|b>, |a|^2, |b|^2 (FP64, compensated, fixed order)
+// and max_i |a_i - b_i|.
+int compare_states(Ctx &ctx, const void *a, int prec_a, const void *b, int prec_b, unsigned n, double *out5);
+
 // Host threshold stream (measure_common.cpp:14-71) from an already mixed seed.
 void threshold_stream(uint64_t stream_seed, size_t cnt, double begin, double end, double *out);
 uint64_t mix_seed(uint64_t base, uint64_t salt);

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 
@@ -130,8 +131,25 @@ void runtime_set_active(bool on)
     g_active.store(on);
 }
 
+// ZXHSIM_SEG_MIN_I as parse_env_size_t reads it (sv_common.cpp:49-66):
+// decimal strtoull of the whole string; unset, empty, trailing text or 0
+// give the default of 18.  The reference reads it once per process; each
+// Sim reads it at construction, which gives the same value to any process
+// that sets it before its first simulator.
+static size_t env_seg_min_i()
+{
+    const char *env = std::getenv("ZXHSIM_SEG_MIN_I");
+    if (env == nullptr || *env == '\0')
+        return 18;
+    char *end = nullptr;
+    const unsigned long long v = std::strtoull(env, &end, 10);
+    if (end == env || (end != nullptr && *end != '\0') || v == 0)
+        return 18;
+    return static_cast<size_t>(v);
+}
+
 Sim::Sim(size_t n, bool disable_x, bool eager, int prec)
-    : N_(n), disable_x_(disable_x), eager_(eager), prec_(prec), rows_(n, 0), b_(n, 0)
+    : N_(n), disable_x_(disable_x), eager_(eager), prec_(prec), rows_(n, 0), b_(n, 0), seg_min_i_(env_seg_min_i())
 {
 }
 

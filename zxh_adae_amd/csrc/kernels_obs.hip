@@ -399,6 +399,117 @@ void observable_poly(unsigned n, const int32_t *qi, const int32_t *qj, const dou
     }
 }
 
+// Two states of 2^n amplitudes compared in one read of each (a: kA64 ? c64 :
+// c128, b likewise): per thread, Kahan sums in FP64 of Re and Im <a|b> =
+// sum conj(a_i) b_i, |a|^2 and |b|^2, and the running max of |a_i - b_i|^2
+// (order-free); then the fixed-order workgroup tree of wg_reduce2 on pairs.
+// Each thread takes one pair of amplitudes per step (a 16-byte load of a c64
+// state, two of a c128 state).  partial: 6 doubles per workgroup.
+template <bool kA64, bool kB64>
+__global__ __launch_bounds__(kThreads) void compare_kernel(const void *__restrict__ a, const void *__restrict__ b,
+                                                           uint64_t npairs, double *__restrict__ partial)
+{
+    __shared__ double sh[2 * kThreads];
+    Kahan re, im, na, nb;
+    double mx = 0.0;
+    const uint64_t stride = uint64_t(gridDim.x) * kThreads;
+    for (uint64_t i = uint64_t(blockIdx.x) * kThreads + threadIdx.x; i < npairs; i += stride)
+    {
+        double av[4], bv[4];
+        if (kA64)
+        {
+            const fvec4 v = __builtin_nontemporal_load(static_cast<const fvec4 *>(a) + i);
+            av[0] = v.x, av[1] = v.y, av[2] = v.z, av[3] = v.w;
+        }
+        else
+        {
+            const dvec2 v0 = __builtin_nontemporal_load(static_cast<const dvec2 *>(a) + 2 * i);
+            const dvec2 v1 = __builtin_nontemporal_load(static_cast<const dvec2 *>(a) + 2 * i + 1);
+            av[0] = v0.x, av[1] = v0.y, av[2] = v1.x, av[3] = v1.y;
+        }
+        if (kB64)
+        {
+            const fvec4 v = __builtin_nontemporal_load(static_cast<const fvec4 *>(b) + i);
+            bv[0] = v.x, bv[1] = v.y, bv[2] = v.z, bv[3] = v.w;
+        }
+        else
+        {
+            const dvec2 v0 = __builtin_nontemporal_load(static_cast<const dvec2 *>(b) + 2 * i);
+            const dvec2 v1 = __builtin_nontemporal_load(static_cast<const dvec2 *>(b) + 2 * i + 1);
+            bv[0] = v0.x, bv[1] = v0.y, bv[2] = v1.x, bv[3] = v1.y;
+        }
+#pragma unroll
+        for (int e = 0; e < 2; e++)
+        {
+            const double ar = av[2 * e], ai = av[2 * e + 1], br = bv[2 * e], bi = bv[2 * e + 1];
+            re.add(ar * br + ai * bi);
+            im.add(ar * bi - ai * br);
+            na.add(ar * ar + ai * ai);
+            nb.add(br * br + bi * bi);
+            const double dr = ar - br, di = ai - bi;
+            mx = fmax(mx, dr * dr + di * di);
+        }
+    }
+    double *out = partial + 6 * size_t(blockIdx.x);
+    double two[2];
+    wg_reduce2(re.s, im.s, sh, two);
+    if (threadIdx.x == 0)
+        out[0] = two[0], out[1] = two[1];
+    __syncthreads();
+    wg_reduce2(na.s, nb.s, sh, two);
+    if (threadIdx.x == 0)
+        out[2] = two[0], out[3] = two[1];
+    __syncthreads();
+    // max: tree of fmax (exact in any order)
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1)
+    {
+        if (int(threadIdx.x) < w)
+            sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[4] = sh[0], out[5] = 0.0;
+}
+
+// Fixed-order final pass over compare_kernel's partials (one workgroup).
+__global__ __launch_bounds__(kThreads) void compare_final_kernel(const double *__restrict__ partial, int count,
+                                                                 double *__restrict__ out)
+{
+    __shared__ double sh[2 * kThreads];
+    Kahan re, im, na, nb;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < count; i += kThreads)
+    {
+        const double *p = partial + 6 * size_t(i);
+        re.add(p[0]);
+        im.add(p[1]);
+        na.add(p[2]);
+        nb.add(p[3]);
+        mx = fmax(mx, p[4]);
+    }
+    double two[2];
+    wg_reduce2(re.s, im.s, sh, two);
+    if (threadIdx.x == 0)
+        out[0] = two[0], out[1] = two[1];
+    __syncthreads();
+    wg_reduce2(na.s, nb.s, sh, two);
+    if (threadIdx.x == 0)
+        out[2] = two[0], out[3] = two[1];
+    __syncthreads();
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1)
+    {
+        if (int(threadIdx.x) < w)
+            sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[4] = sqrt(sh[0]);
+}
+
 template <typename K>
 int grid_of(Ctx &ctx, K kernel, uint64_t items)
 {
@@ -535,6 +646,33 @@ int observable_apply(Ctx &ctx, const void *src, void *dst, int prec, unsigned n,
         ZXA_HIP(hipStreamSynchronize(ctx.stream));
     }
     return ctx.ring.done(ctx.stream);
+}
+
+int compare_states(Ctx &ctx, const void *a, int prec_a, const void *b, int prec_b, unsigned n, double *out5)
+{
+    if (n < 1 || n > 40)
+    {
+        set_error("compare: state width must be 1..40 bits");
+        return EINVAL_;
+    }
+    const uint64_t npairs = uint64_t(1) << (n - 1);
+    const bool a64 = prec_a == C64, b64 = prec_b == C64;
+    auto k = a64 ? (b64 ? compare_kernel<true, true> : compare_kernel<true, false>)
+                 : (b64 ? compare_kernel<false, true> : compare_kernel<false, false>);
+    const int grid = grid_of(ctx, k, (npairs + kThreads - 1) / kThreads);
+    ZXA_TRY(ctx.scratch_e.ensure(size_t(grid) * 6 * sizeof(double) + 8 * sizeof(double)));
+    double *part = static_cast<double *>(ctx.scratch_e.ptr);
+    double *res = part + size_t(grid) * 6;
+    const uint64_t bytes = (uint64_t(1) << n) * (elem_bytes(prec_a) + elem_bytes(prec_b));
+    ZXA_TRY(ctx.tic());
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, a, b, npairs, part);
+    ZXA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(compare_final_kernel, dim3(1), dim3(kThreads), 0, ctx.stream, part, grid, res);
+    ZXA_HIP(hipGetLastError());
+    ZXA_TRY(ctx.toc(F_OBS, bytes));
+    ZXA_HIP(hipMemcpyAsync(out5, res, 5 * sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
+    ZXA_HIP(hipStreamSynchronize(ctx.stream));
+    return OK;
 }
 
 ObsBatch::~ObsBatch()

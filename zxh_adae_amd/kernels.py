@@ -192,6 +192,18 @@ class DeviceState:
         check(self.lib.zxh_k_prob_total(self.ptr, self.prec, self.nbits, C.byref(out)))
         return out.value
 
+    def compare(self, other: "DeviceState") -> dict:
+        """This state against another of the same width (either precision),
+        one read of each: overlap <self|other>, both squared norms, the
+        infidelity 1 - |<self|other>| and max_i |self_i - other_i|."""
+        assert other.nbits == self.nbits
+        out = np.zeros(5)
+        check(self.lib.zxh_k_compare(self.ptr, self.prec, other.ptr, other.prec, self.nbits,
+                                     out.ctypes.data_as(_lib._dp)))
+        ov = complex(out[0], out[1])
+        return {"overlap": ov, "norm2_self": float(out[2]), "norm2_other": float(out[3]),
+                "infidelity": 1.0 - abs(ov), "max_amp_error": float(out[4])}
+
 
 def sync():
     check(_lib.load().zxh_dev_sync())
