@@ -388,14 +388,80 @@ def c5_circuit(W, K, prec, n, layers=40, shots=1000000, seed=5):
     t1 = time.perf_counter()
     st.sample(18, mix_seed(seed, 0), shots)
     t_samp = time.perf_counter() - t1
+    tree = K.timing("tree")
     K.set_timing(False)
     st.free()
+    tree_ms = tree["ms"] / max(1, tree["launches"])
+    tree_gbps = tree["bytes"] / max(1, tree["launches"]) / (tree_ms * 1e-3) / 1e9 if tree["launches"] else 0.0
     nblocks = sum(len(L["blocks"]) for L in spec)
     ngates = nblocks + sum(len(L["diag"]) + len(L["mcx"]) for L in spec)
     return {"qubits": n, "state": "complex64" if prec == "c64" else "complex128", "layers": layers,
             "gates": ngates, "blocks": nblocks, "circuit_s": round(t_circ, 3), "gates/s": round(ngates / t_circ, 1),
             "kernels": fams, "block_tflops": round(tflops, 2), "block_mfma_frac": round(tflops / peak, 4),
-            "mfma_peak_tflops": peak, "sampling": {"shots": shots, "seconds": round(t_samp, 3)}}
+            "mfma_peak_tflops": peak,
+            "sampling": {"shots": shots, "seconds": round(t_samp, 3), "tree_ms": round(tree_ms, 3),
+                         "tree_GB/s": round(tree_gbps, 1), "tree_frac": round(tree_gbps / HBM_PEAK_GBPS, 4)}}
+
+
+def sampling(sim, shots=1000000, repeats=3):
+    """measure(shots) on the simulator's state: wall time of the whole call
+    (tree build, host thresholds overlapped with it, search, results back),
+    the tree pass's and the search's kernel times, and the tree pass against
+    the HBM roofline (bytes: one read of the state plus the stored levels)."""
+    sim.set_seed(5)
+    sim.measure(shots)  # warm: pinned buffers, scratch
+    sim.reset_timing()
+    sim.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        sim.measure(shots)
+    dt = (time.perf_counter() - t0) / repeats
+    sim.set_timing(False)
+    tree, search = sim.timing("tree"), sim.timing("sample")
+    tree_ms = tree["ms"] / tree["launches"]
+    gbps = tree["bytes"] / tree["launches"] / (tree_ms * 1e-3) / 1e9
+    return {"shots": shots, "qubits": sim.device_state()[1], "measure_ms": round(dt * 1e3, 3),
+            "shots/s": round(shots / dt, 1), "tree_ms": round(tree_ms, 4),
+            "search_ms": round(search["ms"] / search["launches"], 4),
+            "roofline": {"bound": "hbm", "achieved": round(gbps, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                         "frac": round(gbps / HBM_PEAK_GBPS, 4), "kernel": "tree",
+                         "bytes_per_launch": int(tree["bytes"] / tree["launches"])}}
+
+
+def sampling_section(Z, W, K):
+    """measure(1e6) on the 30-qubit uniform superposition (c128, c64), and the
+    kernel-level sampler on a 33-qubit complex64 state (64 GiB)."""
+    from zxh_adae_amd.kernels import DeviceState, mix_seed, set_basis_zero
+    out = {}
+    for prec in ("c128", "c64"):
+        sim = Z.ZXH(NQ, False, True, precision=prec)
+        sim.gates(W.hadamard_all(NQ))
+        sim.execute()
+        out[prec] = sampling(sim)
+        del sim
+    n = 33
+    st = DeviceState(n, "c64")
+    set_basis_zero(st)
+    st.u3_batch([((1 << q), (1 << q), W.u3_matrix(0.4 + 0.05 * q, 0.3, 0.2)) for q in range(n)])
+    st.sample(18, mix_seed(5, 0), 1000000)
+    K.reset_timing()
+    K.set_timing(True)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        st.sample(18, mix_seed(5, 0), 1000000)
+    dt = (time.perf_counter() - t0) / 3
+    tree, search = K.timing("tree"), K.timing("sample")
+    K.set_timing(False)
+    st.scale(1.0, 0.0)  # counter calibration: one read and one write of 64 GiB
+    K.sync()
+    st.free()
+    tree_ms = tree["ms"] / tree["launches"]
+    gbps = tree["bytes"] / tree["launches"] / (tree_ms * 1e-3) / 1e9
+    out["c64_33q_kernel"] = {"shots": 1000000, "seconds": round(dt, 4), "tree_ms": round(tree_ms, 3),
+                             "search_ms": round(search["ms"] / search["launches"], 3),
+                             "tree_GB/s": round(gbps, 1), "tree_frac": round(gbps / HBM_PEAK_GBPS, 4),
+                             "read_only_GB/s": round((8 << n) / (tree_ms * 1e-3) / 1e9, 1)}
+    return out
 
 
 def accuracy(sim64, sim128):
@@ -574,7 +640,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--kernels", choices=["c128", "c64"], default=None,
                     help="only launch each hot kernel --steps times on a 30-qubit state (profiling)")
-    ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128", "zxh_workloads"],
+    ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128", "zxh_workloads", "sampling"],
                     default=None,
                     help="run one extra alone, after the tile-geometry autotuning (profiling)")
     ap.add_argument("--workloads", default=None,
@@ -600,7 +666,7 @@ def main():
         return
     from zxh_adae_amd import autotune
     if args.section:
-        prec = "c128" if args.section in ("c5_c128", "zxh_workloads") else "c64"
+        prec = "c128" if args.section in ("c5_c128", "zxh_workloads", "sampling") else "c64"
         st = K.DeviceState(NQ, prec)
         st.scale(0.0)
         tuned = autotune.tune_tile(st)
@@ -612,6 +678,8 @@ def main():
             out = batch_c4(W)
         elif args.section == "c5_c64":
             out = c5_circuit(W, K, "c64", 33)
+        elif args.section == "sampling":
+            out = sampling_section(Z, W, K)
         elif args.section == "zxh_workloads":
             out = zxh_workloads(Z, breakdown=True, names=args.workloads.split(",") if args.workloads else None)
         else:
@@ -677,15 +745,11 @@ def main():
         result["u3_all"] = {"c128": u3_all(Z, W, K, sim.device_state(), "c128"),
                             "c64": u3_all(Z, W, K, sim64.device_state(), "c64")}
         result["u3_all"]["c64"].update(accuracy(sim64, sim))
+        log("sampling 1e6 shots c64")
+        result["sampling_c64"] = sampling(sim64)
         del sim64
         log("sampling 1e6 shots")
-        sim.set_seed(5)
-        sim.set_timing(True)
-        t0 = time.perf_counter()
-        sim.measure(1000000)
-        result["sampling"] = {"shots": 1000000, "qubits": NQ, "seconds": round(time.perf_counter() - t0, 4),
-                              "search_kernel_ms": round(sim.timing("sample")["ms"], 3)}
-        sim.set_timing(False)
+        result["sampling"] = sampling(sim)
         del sim
 
     if not args.no_extras:

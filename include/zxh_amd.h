@@ -143,7 +143,9 @@ int zxh_set_timing(zxh_sim *s, int on);
  * window), 10 observable, 11 adjoint, 12 dense block.  Fills launches, total
  * milliseconds and algorithmic bytes.  Family 13 only counts the launches of
  * families 1 and 9 that ran on the LDS-free upper kernel (time and bytes 0;
- * they are in those families). */
+ * they are in those families).  Family 14 is the sampler's probability tree
+ * (all passes of one tree build as one entry; bytes = one read of the state
+ * plus the stored tree levels); family 5 is the sampler's search. */
 int zxh_get_timing(zxh_sim *s, int family, uint64_t *launches, double *total_ms, uint64_t *bytes);
 int zxh_reset_timing(zxh_sim *s);
 

@@ -29,7 +29,8 @@ _PREC = {"c128": _lib.ZXH_C128, "complex128": _lib.ZXH_C128, "c64": _lib.ZXH_C64
 
 # kernel family ids of zxh_get_timing
 TIMING_FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6, "copy": 7,
-                   "scale": 8, "fused": 9, "obs": 10, "adjoint": 11, "block": 12, "u3_upper": 13}
+                   "scale": 8, "fused": 9, "obs": 10, "adjoint": 11, "block": 12, "u3_upper": 13,
+                   "tree": 14}
 _POLICY = {"fused": _lib.ZXH_POLICY_FUSED, "reference": _lib.ZXH_POLICY_REFERENCE}
 
 

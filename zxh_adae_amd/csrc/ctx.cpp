@@ -8,6 +8,7 @@
 #include <cstring>
 #include <mutex>
 #include <random>
+#include <thread>
 
 // The threshold recurrence must round every product and sum separately, as
 // the reference's x86-64 build does.
@@ -74,6 +75,32 @@ int ParamRing::init()
     for (int i = 0; i < kSlots; i++)
         ZXA_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
     return OK;
+}
+
+int PinBuf::ensure(size_t need)
+{
+    if (need <= bytes && ptr)
+        return OK;
+    release();
+    const size_t cap = std::max<size_t>(need, 4096);
+    hipError_t e = hipHostMalloc(&ptr, cap, hipHostMallocDefault);
+    if (e != hipSuccess)
+    {
+        ptr = nullptr;
+        bytes = 0;
+        set_error(std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+        return ENOMEM_;
+    }
+    bytes = cap;
+    return OK;
+}
+
+void PinBuf::release()
+{
+    if (ptr)
+        (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
 }
 
 void ParamRing::release()
@@ -312,23 +339,53 @@ uint64_t mix_seed(uint64_t base, uint64_t salt)
 // uniform_real_distribution<double>(0, 1), clamped below by DBL_MIN
 // (threshold_stream_t, measure_common.cpp:14-71).  The engine and the
 // distribution are the standard library's, as in the reference.
-void threshold_stream(uint64_t stream_seed, size_t cnt, double begin, double end, double *out)
+void threshold_draws(uint64_t stream_seed, size_t cnt, double *u)
 {
     std::mt19937_64 rng(stream_seed);
     std::uniform_real_distribution<double> dist(0.0, 1.0);
-    size_t rest = cnt;
+    for (size_t k = 0; k < cnt; k++)
+        u[k] = std::max(dist(rng), std::numeric_limits<double>::min());
+}
+
+void threshold_factors(double *uf, size_t cnt)
+{
+    const auto run = [uf, cnt](size_t b, size_t e) {
+        for (size_t k = b; k < e; k++)
+            uf[k] = 1.0 - std::pow(uf[k], 1.0 / static_cast<double>(cnt - k));
+    };
+    const unsigned hw = std::thread::hardware_concurrency();
+    const size_t nth = std::min<size_t>({hw ? hw : 1u, 16u, cnt / 16384 + 1});
+    if (nth <= 1)
+    {
+        run(0, cnt);
+        return;
+    }
+    std::vector<std::thread> pool;
+    const size_t per = (cnt + nth - 1) / nth;
+    for (size_t t = 1; t < nth; t++)
+        pool.emplace_back(run, std::min(cnt, t * per), std::min(cnt, (t + 1) * per));
+    run(0, std::min(cnt, per));
+    for (auto &th : pool)
+        th.join();
+}
+
+void threshold_apply(double *fh, size_t cnt, double begin, double end)
+{
     double lo = begin;
     for (size_t k = 0; k < cnt; k++)
     {
-        double u = dist(rng);
-        u = std::max(u, std::numeric_limits<double>::min());
         const double span = end - lo;
-        const double factor = 1.0 - std::pow(u, 1.0 / static_cast<double>(rest));
-        const double head = lo + span * factor;
-        out[k] = head;
+        const double head = lo + span * fh[k];
+        fh[k] = head;
         lo = head;
-        rest--;
     }
+}
+
+void threshold_stream(uint64_t stream_seed, size_t cnt, double begin, double end, double *out)
+{
+    threshold_draws(stream_seed, cnt, out);
+    threshold_factors(out, cnt);
+    threshold_apply(out, cnt, begin, end);
 }
 
 } // namespace zxa

@@ -73,6 +73,22 @@ struct DevBuf
     }
 };
 
+// Grow-only pinned host buffer (hipHostMalloc).
+struct PinBuf
+{
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    void release();
+    PinBuf() = default;
+    PinBuf(const PinBuf &) = delete;
+    PinBuf &operator=(const PinBuf &) = delete;
+    ~PinBuf()
+    {
+        release();
+    }
+};
+
 // Kernel families for event timing.
 enum Family
 {
@@ -90,7 +106,8 @@ enum Family
     F_ADJ = 11,
     F_BLOCK = 12,
     F_UPPER = 13, // count only: the F_U3TILE / F_FUSED launches that ran upper_kernel
-    F_COUNT = 14
+    F_TREE = 14,  // the sampler's probability tree (all its passes as one entry)
+    F_COUNT = 15
 };
 
 struct FamilyTiming
@@ -142,6 +159,7 @@ struct Ctx
     alignas(8) unsigned char sample_levels[64 * 8] = {};
     unsigned sample_m = 0, sample_I = 0;
     bool sample_ready = false;
+    PinBuf sample_total_pin; // the tree root, copied back asynchronously
     template <typename T>
     T &sample_levels_as()
     {
@@ -364,6 +382,10 @@ int scale_apply(Ctx &ctx, void *state, int prec, unsigned nbits, double re, doub
 // amplitudes and returns the total (the reference's local_prob); search
 // resolves sorted thresholds (host array) against it.
 int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, double *total);
+// sample_prepare in two halves, so that host work can overlap the tree build:
+// launch enqueues it (and the copy of the root to pinned memory); total waits.
+int sample_launch(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits);
+int sample_total(Ctx &ctx, double *total);
 // Optional packing of the samples into virtual result words on the device:
 // bit q = parity(real & rows[q]) ^ b[q]; out (host) gets ceil(nq / 64) words
 // per shot.
@@ -396,6 +418,14 @@ int compare_states(Ctx &ctx, const void *a, int prec_a, const void *b, int prec_
 
 // Host threshold stream (measure_common.cpp:14-71) from an already mixed seed.
 void threshold_stream(uint64_t stream_seed, size_t cnt, double begin, double end, double *out);
+// The same stream in three steps.  draws: the cnt uniforms u_k (mt19937_64,
+// uniform_real_distribution, clamped below by DBL_MIN), in order.  factors:
+// in place, f_k = 1 - u_k^(1 / (cnt - k)), which does not depend on the
+// interval, on host threads.  apply: in place, the recurrence
+// head_k = lo + (end - lo) * f_k, lo = head_k, from lo = begin.
+void threshold_draws(uint64_t stream_seed, size_t cnt, double *u);
+void threshold_factors(double *uf, size_t cnt);
+void threshold_apply(double *fh, size_t cnt, double begin, double end);
 uint64_t mix_seed(uint64_t base, uint64_t salt);
 
 } // namespace zxa

@@ -918,40 +918,49 @@ int Sim::measure(size_t cnt)
     }
     const unsigned m = active_bits();
     const unsigned I = unsigned(std::min<size_t>(M_ <= seg_min_i_ ? M_ : seg_min_i_, m));
+    result_count_ = 0;
+    // the probability tree builds on the GPU while the host draws the
+    // uniforms and their factors 1 - u^(1/rest), which do not depend on the
+    // total; the recurrence runs once the total is back
+    ZXA_TRY(sample_launch(ctx_, dstate_, prec_, m, I));
+    const uint64_t base = use_seed_ ? seed_ : uint64_t(std::random_device{}());
+    const uint64_t stream_seed = mix_seed(base, 0);
+    ZXA_TRY(thr_pin_.ensure(cnt * sizeof(double)));
+    double *thr = static_cast<double *>(thr_pin_.ptr);
+    threshold_draws(stream_seed, cnt, thr);
+    threshold_factors(thr, cnt);
     double total = 0.0;
-    ZXA_TRY(sample_prepare(ctx_, dstate_, prec_, m, I, &total));
+    ZXA_TRY(sample_total(ctx_, &total));
     if (!(total > 0.0))
     {
         set_error("state has zero total probability");
         return EZEROPROB;
     }
-    const uint64_t base = use_seed_ ? seed_ : uint64_t(std::random_device{}());
-    const uint64_t stream_seed = mix_seed(base, 0);
-    std::vector<double> thr(cnt);
-    threshold_stream(stream_seed, cnt, 0.0, total, thr.data());
-    real_results_.resize(cnt);
+    threshold_apply(thr, cnt, 0.0, total);
+    ZXA_TRY(real_pin_.ensure(cnt * sizeof(uint64_t)));
+    uint64_t *real = static_cast<uint64_t *>(real_pin_.ptr);
     // virtual result words (pack_virtual_results_host,
-    // measure_result_pack.h:24-44): the identity map (plain method) copies
-    // the real addresses; otherwise the device packs them after the search
+    // measure_result_pack.h:24-44): under the identity map (plain method)
+    // they are the real addresses; otherwise the device packs them after the
+    // search
     const size_t wc = (N_ + 63) / 64;
-    packed_.assign(cnt * wc, 0);
     bool identity = (b_.end() == std::find(b_.begin(), b_.end(), uint8_t(1))) && N_ <= 64;
     for (size_t q = 0; q < N_ && identity; q++)
         identity = rows_[q] == (uint64_t(1) << q);
+    packed_is_real_ = identity;
     if (identity)
     {
-        ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr.data(), cnt, real_results_.data()));
-        for (size_t i = 0; i < cnt; i++)
-            packed_[i] = real_results_[i];
+        ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr, cnt, real));
     }
     else
     {
+        ZXA_TRY(packed_pin_.ensure(cnt * wc * sizeof(uint64_t)));
         PackSpec ps;
         ps.nq = unsigned(N_);
         ps.rows = rows_.data();
         ps.b = b_.data();
-        ps.out = packed_.data();
-        ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr.data(), cnt, real_results_.data(), &ps));
+        ps.out = static_cast<uint64_t *>(packed_pin_.ptr);
+        ZXA_TRY(sample_search(ctx_, dstate_, prec_, thr, cnt, real, &ps));
     }
     result_count_ = cnt;
     return OK;
@@ -965,7 +974,9 @@ int Sim::get_results(uint64_t *packed, size_t cnt) const
         return EINVAL_;
     }
     const size_t wc = (N_ + 63) / 64;
-    std::memcpy(packed, packed_.data(), cnt * wc * sizeof(uint64_t));
+    if (cnt == 0)
+        return OK;
+    std::memcpy(packed, packed_is_real_ ? real_pin_.ptr : packed_pin_.ptr, cnt * wc * sizeof(uint64_t));
     return OK;
 }
 
@@ -976,7 +987,8 @@ int Sim::get_real_results(uint64_t *real, size_t cnt) const
         set_error("get_results count exceeds cached result count");
         return EINVAL_;
     }
-    std::memcpy(real, real_results_.data(), cnt * sizeof(uint64_t));
+    if (cnt)
+        std::memcpy(real, real_pin_.ptr, cnt * sizeof(uint64_t));
     return OK;
 }
 

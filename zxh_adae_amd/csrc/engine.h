@@ -153,8 +153,11 @@ class Sim
     bool executed_ = false;
     size_t seg_min_i_ = 18;
     // measure cache
-    std::vector<uint64_t> real_results_;
-    std::vector<uint64_t> packed_;
+    // pinned host buffers of the last measure: thresholds, real addresses and
+    // (unless the map is the identity, when they equal the real addresses)
+    // packed virtual words
+    PinBuf thr_pin_, real_pin_, packed_pin_;
+    bool packed_is_real_ = true;
     size_t result_count_ = 0;
     // pending fused pass: single-bit U3 gates (each after its fan, if any),
     // then a diagonal window

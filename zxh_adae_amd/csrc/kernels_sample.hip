@@ -12,8 +12,9 @@
 //     excl_j   = (((0 + N_{l1}[.]) + N_{l2}[.]) + ...)   set bits of j, high to low
 //     prefix_s = same over segment indices, with levels >= I
 //     cdf_j    = (prefix_s + excl_j) + p_j,   tail_s = prefix_s + N_I[s]
-// So the GPU stores the tree levels >= LS (LS = min(4, I); lower levels are
-// recomputed from the 16 amplitudes of a block), and each shot performs
+// So the GPU stores the tree levels >= LS (LS = min(4, I) for complex128,
+// min(5, I) for complex64; lower levels are recomputed from the 256 bytes of
+// amplitudes of a block: 16 complex128 or 32 complex64), and each shot performs
 // libstdc++'s exact upper_bound probe sequence, evaluating cdf at each probe.
 // A threshold belongs to the first segment whose running-max tail exceeds
 // it, which is where the reference's sequential segment walk places it.
@@ -95,6 +96,128 @@ __global__ __launch_bounds__(kThreads) void tree_kernel(const void *__restrict__
     }
 }
 
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+// tree_amp_kernel geometry: a wave tile is 16 loads of 16 bytes per lane,
+// 1024 amplitudes (c128) or 2048 (c64); a workgroup holds four tiles.
+template <bool kC64>
+struct TreeGeo
+{
+    static constexpr int kTileBits = kC64 ? 11 : 10;
+    static constexpr int kRounds = kC64 ? 2 : 1;     // 512 level-1 nodes per round
+    static constexpr int kGroupBits = kTileBits + 2; // amplitudes per workgroup
+};
+constexpr int kRowPad = 9; // LDS doubles per row of 8 level-1 nodes (one pad)
+
+// Orders this wave's LDS accesses (a wave executes in lockstep; the fences
+// keep the compiler from moving accesses across the exchange).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The value of the neighbouring lane (lane ^ 1), by DPP quad permutation.
+__device__ __forceinline__ double lane_xor1(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp(int(b), 0xB1, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(int(b >> 32), 0xB1, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, (static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+
+// First tree pass when the stored levels start at the precision's low level
+// (4 for complex128, 5 for complex64): one read of the state, that level
+// written; the upper levels come from it in tree_kernel passes.
+// Each wave reduces a tile.  All of the tile's loads (coalesced, 16 bytes per
+// lane, nontemporal) are issued before any arithmetic.  Level 1 is formed in
+// registers: inside the lane for complex64 (a 16-byte load holds two
+// amplitudes), with the neighbouring lane by DPP for complex128.  Level-1
+// nodes go to LDS in rows of 8 (16 amplitudes; padded to 9 doubles so that the
+// row reads are conflict-free); each lane sums one row as the aligned-pair
+// tree (levels 2..4) in registers; complex128 stores the level-4 node (a
+// contiguous 512 bytes per wave), complex64 adds the neighbouring lane's (DPP)
+// and stores level 5 from the even lanes (256 bytes per wave and round).  The
+// stores are nontemporal: the state streams through without the tree's writes
+// breaking up the reads.  Every node is
+// left + right of the same two children as in the reference's balanced tree
+// (IEEE addition is commutative, so the order of the two operands does not
+// matter).
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void tree_amp_kernel(const void *__restrict__ state, double *__restrict__ tl)
+{
+    using G = TreeGeo<kC64>;
+    __shared__ double rows[kThreads / 64][64 * kRowPad];
+    const int w = threadIdx.x >> 6, L = threadIdx.x & 63;
+    const uint64_t tile = uint64_t(blockIdx.x) * (kThreads / 64) + w;
+    const uint64_t base = tile << G::kTileBits;
+    double *rb = rows[w];
+    if (kC64)
+    {
+        const fvec4 *src = static_cast<const fvec4 *>(state) + (base >> 1);
+        fvec4 v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            v[k] = __builtin_nontemporal_load(src + 64 * k + L);
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+        {
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+            {
+                const fvec4 a = v[8 * r + k];
+                const double x0 = a.x, y0 = a.y, x1 = a.z, y1 = a.w;
+                const double n1 = __dadd_rn(__dadd_rn(__dmul_rn(x0, x0), __dmul_rn(y0, y0)),
+                                            __dadd_rn(__dmul_rn(x1, x1), __dmul_rn(y1, y1)));
+                const int j = 64 * k + L; // level-1 node within the round
+                rb[(j >> 3) * kRowPad + (j & 7)] = n1;
+            }
+            wave_sync();
+            double q[8];
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+                q[c] = rb[L * kRowPad + c];
+            wave_sync();
+#pragma unroll
+            for (int c = 0; c < 4; c++)
+                q[c] = __dadd_rn(q[2 * c], q[2 * c + 1]);
+            // level 4 of row L, then level 5 with the neighbouring lane
+            const double n4 = __dadd_rn(__dadd_rn(q[0], q[1]), __dadd_rn(q[2], q[3]));
+            const double n5 = __dadd_rn(n4, lane_xor1(n4));
+            if ((L & 1) == 0)
+                __builtin_nontemporal_store(n5, tl + (base >> 5) + 32 * r + (L >> 1));
+        }
+    }
+    else
+    {
+        const dvec2 *src = static_cast<const dvec2 *>(state) + base;
+        dvec2 v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            v[k] = __builtin_nontemporal_load(src + 64 * k + L);
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+        {
+            const double p = __dadd_rn(__dmul_rn(v[k].x, v[k].x), __dmul_rn(v[k].y, v[k].y));
+            const double n1 = __dadd_rn(p, lane_xor1(p));
+            const int j = 32 * k + (L >> 1); // level-1 node within the tile
+            if ((L & 1) == 0)
+                rb[(j >> 3) * kRowPad + (j & 7)] = n1;
+        }
+        wave_sync();
+        double q[8];
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+            q[c] = rb[L * kRowPad + c];
+#pragma unroll
+        for (int c = 0; c < 4; c++)
+            q[c] = __dadd_rn(q[2 * c], q[2 * c + 1]);
+        __builtin_nontemporal_store(__dadd_rn(__dadd_rn(q[0], q[1]), __dadd_rn(q[2], q[3])), tl + (base >> 4) + L);
+    }
+}
+
 // prefix_s (Fenwick over levels >= I) and tail_s = prefix_s + N_I[s].
 __global__ __launch_bounds__(kThreads) void tails_kernel(const double *__restrict__ tree, const Levels lv, int I, int m,
                                                          uint64_t nseg, double *__restrict__ prefix,
@@ -152,8 +275,52 @@ __device__ __forceinline__ double pick(const double (&a)[N], int idx)
     return r;
 }
 
+// Levels below LB (<= kLBM) of the 2^LB-amplitude block holding g, as the
+// block's balanced pair sums: the Fenwick part of the exclusive prefix of g's
+// offset in the block (levels LB-1 .. 0, high to low) and p_g.
+template <int kLBM>
+__device__ __forceinline__ void block_prefix(const void *state, bool c64, uint64_t g, int LB, double &acc,
+                                             double &pg)
+{
+    constexpr int NB = 1 << kLBM;
+    const uint64_t gb = g & ~((uint64_t(1) << LB) - 1);
+    const int nb = 1 << LB;
+    double lv0[NB], lv1[NB / 2], lv2[NB / 4], lv3[NB / 8], lv4[NB / 16 > 0 ? NB / 16 : 1];
+#pragma unroll
+    for (int i = 0; i < NB; i++)
+        lv0[i] = i < nb ? p_of(state, c64, gb + i) : 0.0;
+#pragma unroll
+    for (int i = 0; i < NB / 2; i++)
+        lv1[i] = __dadd_rn(lv0[2 * i], lv0[2 * i + 1]);
+#pragma unroll
+    for (int i = 0; i < NB / 4; i++)
+        lv2[i] = __dadd_rn(lv1[2 * i], lv1[2 * i + 1]);
+#pragma unroll
+    for (int i = 0; i < NB / 8; i++)
+        lv3[i] = __dadd_rn(lv2[2 * i], lv2[2 * i + 1]);
+    if (kLBM == 5)
+    {
+#pragma unroll
+        for (int i = 0; i < NB / 16; i++)
+            lv4[i] = __dadd_rn(lv3[2 * i], lv3[2 * i + 1]);
+    }
+    const int off = static_cast<int>(g - gb);
+    if (kLBM == 5 && (off & 16))
+        acc = __dadd_rn(acc, lv4[0]);
+    if (off & 8)
+        acc = __dadd_rn(acc, pick(lv3, (off >> 3) - 1));
+    if (off & 4)
+        acc = __dadd_rn(acc, pick(lv2, (off >> 2) - 1));
+    if (off & 2)
+        acc = __dadd_rn(acc, pick(lv1, (off >> 1) - 1));
+    if (off & 1)
+        acc = __dadd_rn(acc, pick(lv0, off - 1));
+    pg = pick(lv0, off);
+}
+
 // One shot per thread.
-__global__ __launch_bounds__(kThreads) void search_kernel(const void *__restrict__ state, int c64,
+template <bool kC64>
+__global__ __launch_bounds__(kThreads) void search_kernel(const void *__restrict__ state,
                                                           const double *__restrict__ tree, const Levels lv, int I,
                                                           int LS, const double *__restrict__ cmax,
                                                           const double *__restrict__ prefix, uint64_t nseg,
@@ -201,32 +368,8 @@ __global__ __launch_bounds__(kThreads) void search_kernel(const void *__restrict
         for (int l = I - 1; l >= LB; l--)
             if ((mid >> l) & 1u)
                 acc = __dadd_rn(acc, tree[lv.off[l] + ((g >> l) - 1)]);
-        // levels below LB: balanced sums inside the 2^LB-amplitude block of g
-        const uint64_t gb = g & ~((uint64_t(1) << LB) - 1);
-        const int nb = 1 << LB;
-        double lv0[16], lv1[8], lv2[4], lv3[2];
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-            lv0[i] = i < nb ? p_of(state, c64 != 0, gb + i) : 0.0;
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-            lv1[i] = __dadd_rn(lv0[2 * i], lv0[2 * i + 1]);
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-            lv2[i] = __dadd_rn(lv1[2 * i], lv1[2 * i + 1]);
-#pragma unroll
-        for (int i = 0; i < 2; i++)
-            lv3[i] = __dadd_rn(lv2[2 * i], lv2[2 * i + 1]);
-        const int off = static_cast<int>(g - gb);
-        if (off & 8)
-            acc = __dadd_rn(acc, lv3[0]);
-        if (off & 4)
-            acc = __dadd_rn(acc, pick(lv2, (off >> 2) - 1));
-        if (off & 2)
-            acc = __dadd_rn(acc, pick(lv1, (off >> 1) - 1));
-        if (off & 1)
-            acc = __dadd_rn(acc, pick(lv0, off - 1));
-        const double pmid = pick(lv0, off);
+        double pmid;
+        block_prefix<kC64 ? 5 : 4>(state, kC64, g, LB, acc, pmid);
         const double cdf = __dadd_rn(__dadd_rn(head, acc), pmid);
         if (tau < cdf)
             len = half;
@@ -237,6 +380,13 @@ __global__ __launch_bounds__(kThreads) void search_kernel(const void *__restrict
         }
     }
     out[k] = sbase + (first == valid ? valid - 1 : first);
+}
+
+// Lowest stored tree level: below it the search recomputes from amplitudes
+// (256 bytes of state per block in both precisions).
+inline unsigned low_level(int prec, unsigned I)
+{
+    return std::min(prec == C64 ? 5u : 4u, I);
 }
 
 int build_tree(Ctx &ctx, const void *state, int prec, unsigned m, unsigned LS, Levels &lv, double **tree_out)
@@ -254,6 +404,22 @@ int build_tree(Ctx &ctx, const void *state, int prec, unsigned m, unsigned LS, L
     // level 0 -> up to kSpanBits, then from level kSpanBits, ...
     int lin = 0;
     bool first = true;
+    ZXA_TRY(ctx.tic());
+    const int gbits = prec == C64 ? TreeGeo<true>::kGroupBits : TreeGeo<false>::kGroupBits;
+    const unsigned amp_level = prec == C64 ? 5u : 4u;
+    if (LS == amp_level && int(m) >= gbits)
+    {
+        // amplitudes -> level 4 (c128) or 5 (c64) in one read of the state
+        const unsigned groups = static_cast<unsigned>(uint64_t(1) << (int(m) - gbits));
+        double *tl = tree + lv.off[amp_level];
+        if (prec == C64)
+            hipLaunchKernelGGL(tree_amp_kernel<true>, dim3(groups), dim3(kThreads), 0, ctx.stream, state, tl);
+        else
+            hipLaunchKernelGGL(tree_amp_kernel<false>, dim3(groups), dim3(kThreads), 0, ctx.stream, state, tl);
+        ZXA_HIP(hipGetLastError());
+        first = false;
+        lin = int(amp_level);
+    }
     while (first || lin < int(m))
     {
         const int span_bits = std::min<int>(kSpanBits, int(m) - lin);
@@ -274,6 +440,9 @@ int build_tree(Ctx &ctx, const void *state, int prec, unsigned m, unsigned LS, L
         if (span_bits == 0)
             break;
     }
+    // algorithmic bytes: one read of the state and one write of the stored
+    // levels (level 4 is read back once by the pass above it; not counted)
+    ZXA_TRY(ctx.toc(F_TREE, (uint64_t(1) << m) * elem_bytes(prec) + total * sizeof(double)));
     *tree_out = tree;
     return OK;
 }
@@ -284,24 +453,26 @@ int prob_total(Ctx &ctx, const void *state, int prec, unsigned m, double *out)
 {
     Levels lv;
     double *tree = nullptr;
-    const unsigned LS = std::min(4u, m);
+    const unsigned LS = low_level(prec, m);
     ZXA_TRY(build_tree(ctx, state, prec, m, LS, lv, &tree));
     ZXA_HIP(hipMemcpyAsync(out, tree + lv.off[m], sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
     ZXA_HIP(hipStreamSynchronize(ctx.stream));
     return OK;
 }
 
-int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, double *total)
+int sample_launch(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits)
 {
     if (m > 40)
     {
         set_error("sampler: state wider than 40 bits");
         return EINVAL_;
     }
+    ctx.sample_ready = false;
     const unsigned I = std::min(m, seg_bits);
-    const unsigned LS = std::min(4u, I);
+    const unsigned LS = low_level(prec, I);
     Levels &lv = ctx.sample_levels_as<Levels>();
     double *tree = nullptr;
+    ZXA_TRY(ctx.sample_total_pin.ensure(sizeof(double)));
     ZXA_TRY(build_tree(ctx, state, prec, m, LS, lv, &tree));
     const uint64_t nseg = uint64_t(1) << (m - I);
     ZXA_TRY(ctx.scratch_d.ensure(2 * nseg * sizeof(double)));
@@ -312,12 +483,30 @@ int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned s
     ZXA_HIP(hipGetLastError());
     hipLaunchKernelGGL(cummax_kernel, dim3(1), dim3(kThreads), 0, ctx.stream, cmax, nseg);
     ZXA_HIP(hipGetLastError());
-    ZXA_HIP(hipMemcpyAsync(total, tree + lv.off[m], sizeof(double), hipMemcpyDeviceToHost, ctx.stream));
-    ZXA_HIP(hipStreamSynchronize(ctx.stream));
+    ZXA_HIP(hipMemcpyAsync(ctx.sample_total_pin.ptr, tree + lv.off[m], sizeof(double), hipMemcpyDeviceToHost,
+                           ctx.stream));
     ctx.sample_m = m;
     ctx.sample_I = I;
     ctx.sample_ready = true;
     return OK;
+}
+
+int sample_total(Ctx &ctx, double *total)
+{
+    if (!ctx.sample_ready)
+    {
+        set_error("sampler: total before launch");
+        return ESTATE;
+    }
+    ZXA_HIP(hipStreamSynchronize(ctx.stream));
+    *total = *static_cast<const double *>(ctx.sample_total_pin.ptr);
+    return OK;
+}
+
+int sample_prepare(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, double *total)
+{
+    ZXA_TRY(sample_launch(ctx, state, prec, m, seg_bits));
+    return sample_total(ctx, total);
 }
 
 // Real address -> virtual result words (pack_virtual_results_host,
@@ -357,7 +546,7 @@ int sample_search(Ctx &ctx, const void *state, int prec, const double *threshold
         return ESTATE;
     }
     const unsigned m = ctx.sample_m, I = ctx.sample_I;
-    const unsigned LS = std::min(4u, I);
+    const unsigned LS = low_level(prec, I);
     const Levels &lv = ctx.sample_levels_as<Levels>();
     const double *tree = static_cast<const double *>(ctx.scratch_b.ptr);
     const uint64_t nseg = uint64_t(1) << (m - I);
@@ -378,8 +567,12 @@ int sample_search(Ctx &ctx, const void *state, int prec, const double *threshold
     ZXA_HIP(hipMemsetAsync(derr, 0, sizeof(int), ctx.stream));
     const unsigned grid = static_cast<unsigned>((shots + kThreads - 1) / kThreads);
     ZXA_TRY(ctx.tic());
-    hipLaunchKernelGGL(search_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, state, prec == C64 ? 1 : 0, tree, lv,
-                       int(I), int(LS), cmax, prefix, nseg, dthr, uint64_t(shots), dout, derr);
+    if (prec == C64)
+        hipLaunchKernelGGL(search_kernel<true>, dim3(grid), dim3(kThreads), 0, ctx.stream, state, tree, lv, int(I),
+                           int(LS), cmax, prefix, nseg, dthr, uint64_t(shots), dout, derr);
+    else
+        hipLaunchKernelGGL(search_kernel<false>, dim3(grid), dim3(kThreads), 0, ctx.stream, state, tree, lv, int(I),
+                           int(LS), cmax, prefix, nseg, dthr, uint64_t(shots), dout, derr);
     ZXA_HIP(hipGetLastError());
     ZXA_TRY(ctx.toc(F_SAMPLE, 0));
     if (pack)

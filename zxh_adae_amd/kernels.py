@@ -210,7 +210,7 @@ def sync():
 
 
 FAMILIES = {"diag": 0, "u3_tile": 1, "u3_pair": 2, "perm": 3, "cp": 4, "sample": 5, "reset": 6, "copy": 7,
-            "scale": 8, "fused": 9, "obs": 10, "adjoint": 11, "block": 12, "u3_upper": 13}
+            "scale": 8, "fused": 9, "obs": 10, "adjoint": 11, "block": 12, "u3_upper": 13, "tree": 14}
 
 
 def set_timing(on: bool):
