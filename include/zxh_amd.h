@@ -178,6 +178,22 @@ int zxh_k_fused_pass(void *state, int precision, size_t local_bits, const zxh_u3
  * ngates entries, sel1[i] == ~0 marks a P gate (Z/Rz already lowered to P). */
 int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
                const double *theta, size_t ngates);
+/* zxh_k_diag that allocates no device memory.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * per-chunk coefficients of the sweep go to a 32 MiB workspace that the
+ * library holds per GPU until the process ends (a device array, looked up by
+ * zxh_init on the current device); zxh_init must have been called.  A degree-2
+ * polynomial window (every plain-method P/CP/Rz/Z window) is applied to states
+ * of up to 30 qubits (complex128) or 31 (complex64): where the autotuned flat
+ * sweep's coefficients (128 bytes per 2^9 / 2^10 amplitudes) do not fit, the
+ * polynomial sweep (128 bytes per 2^12 / 2^13) runs instead, whose results
+ * agree within the same rounding bound; wider states return ZXH_EINVAL.
+ * Other windows run the per-gate sweep, which needs no workspace.  Every
+ * call on a GPU shares that GPU's workspace (rewritten in full before it is
+ * read, so nothing needs clearing): calls that overlap on one GPU race. */
+int zxh_k_diag_noalloc(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
+                       const double *theta, size_t ngates);
 /* x_block_kernel / cx_block_kernel / cp_kernel (kernels.h:18-19, 47) */
 int zxh_k_x(void *state, int precision, size_t local_bits, uint64_t delta, uint64_t packed_sel);
 int zxh_k_cx(void *state, int precision, size_t local_bits, uint64_t delta, uint64_t packed_sel_cq,

@@ -72,6 +72,7 @@ _SIGS = [
     ("zxh_dev_sync", C.c_int, []),
     ("zxh_k_u3_batch", C.c_int, [_vp, C.c_int, _sz, _vp, _sz]),
     ("zxh_k_diag", C.c_int, [_vp, C.c_int, _sz, _u64p, _u64p, _dp, _sz]),
+    ("zxh_k_diag_noalloc", C.c_int, [_vp, C.c_int, _sz, _u64p, _u64p, _dp, _sz]),
     ("zxh_k_fused_pass", C.c_int, [_vp, C.c_int, _sz, _vp, _sz, _u64p, _u64p, _dp, _sz]),
     ("zxh_k_x", C.c_int, [_vp, C.c_int, _sz, _u64, _u64]),
     ("zxh_k_cx", C.c_int, [_vp, C.c_int, _sz, _u64, _u64, _u64]),

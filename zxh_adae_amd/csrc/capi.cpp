@@ -54,7 +54,14 @@ int zxh_init(void)
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
         return fail(ZXH_ENODEV, "no HIP device available");
     zxa::runtime_set_active(true);
-    return ZXH_OK;
+    // the kernel-level context and, on the current device, the workspace of
+    // the no-allocation entry point
+    int rc = 0;
+    if (!zxa::default_ctx(&rc))
+        return rc;
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    return zxa::noalloc_workspace(&ws, &ws_bytes);
 }
 
 int zxh_finalize(void)
@@ -415,6 +422,29 @@ int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *se
     for (size_t i = 0; i < ngates; i++)
         g[i] = {sel0[i], sel1[i], theta[i]};
     return finish(c, zxa::diag_apply(*c, state, precision, unsigned(local_bits), g.data(), ngates));
+}
+
+int zxh_k_diag_noalloc(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
+                       const double *theta, size_t ngates)
+{
+    int rc = check_kernel_args(state, precision, local_bits);
+    if (rc)
+        return rc;
+    if (ngates && (!sel0 || !sel1 || !theta))
+        return fail(ZXH_EINVAL, "null gate arrays");
+    zxa::Ctx *c = zxa::default_ctx_if_ready();
+    if (!c)
+        return fail(ZXH_ESTATE, "zxh_k_diag_noalloc: call zxh_init() first (it creates the device context)");
+    void *ws = nullptr;
+    size_t ws_bytes = 0;
+    rc = zxa::noalloc_workspace(&ws, &ws_bytes);
+    if (rc)
+        return rc;
+    std::vector<zxa::DiagGate> g(ngates);
+    for (size_t i = 0; i < ngates; i++)
+        g[i] = {sel0[i], sel1[i], theta[i]};
+    return finish(c, zxa::diag_apply_ws(*c, state, precision, unsigned(local_bits), g.data(), ngates,
+                                        static_cast<double *>(ws), ws_bytes));
 }
 
 int zxh_k_x(void *state, int precision, size_t local_bits, uint64_t delta, uint64_t packed_sel)

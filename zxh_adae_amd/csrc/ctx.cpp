@@ -249,11 +249,22 @@ int Ctx::collect()
     return OK;
 }
 
+namespace
+{
+std::mutex g_ctx_mu;
+Ctx *g_ctx = nullptr;
+} // namespace
+
+Ctx *default_ctx_if_ready()
+{
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    return g_ctx;
+}
+
 Ctx *default_ctx(int *rc)
 {
-    static std::mutex mu;
-    static Ctx *ctx = nullptr;
-    std::lock_guard<std::mutex> lk(mu);
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    Ctx *&ctx = g_ctx;
     if (!ctx)
     {
         Ctx *c = new Ctx();

@@ -224,6 +224,15 @@ constexpr uint64_t kNoSel = ~uint64_t(0);
 
 // One diagonal sweep applying every gate of the window.
 int diag_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const DiagGate *gates, size_t ngates);
+// diag_apply with a caller-given coefficient workspace (nullable; see
+// kernels_diag.hip): no device allocation when ws is given.
+int diag_apply_ws(Ctx &ctx, void *state, int prec, unsigned nbits, const DiagGate *gates, size_t ngates, double *ws,
+                  size_t ws_bytes);
+// Device address and size of the no-allocation entry point's workspace on
+// the current device (looked up once per device, allocates nothing).
+int noalloc_workspace(void **ptr, size_t *bytes);
+// The process-wide context if it exists (nullptr otherwise; creates nothing).
+Ctx *default_ctx_if_ready();
 
 struct U3Gate
 {

@@ -28,6 +28,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace zxa
@@ -448,7 +450,8 @@ int launch_flat(Ctx &ctx, void *state, const double *tabs, const double *coef, u
 }
 
 // Flat-grid diagonal sweep with EPT sixteen-byte units per thread.
-int diag_flat(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates, int ept, bool nt)
+int diag_flat(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates, int ept, bool nt,
+              double *ws)
 {
     const bool c64 = prec == C64;
     const int LB = c64 ? 7 : 6;
@@ -515,8 +518,12 @@ int diag_flat(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates
     for (int i = 0; i < nh; i++)
         outer[i] = NCO + i;
     const uint64_t nchunks = uint64_t(1) << nh;
-    ZXA_TRY(ctx.scratch_a.ensure(nchunks * kCoefStride * sizeof(double)));
-    double *coef = static_cast<double *>(ctx.scratch_a.ptr);
+    double *coef = ws;
+    if (!coef)
+    {
+        ZXA_TRY(ctx.scratch_a.ensure(nchunks * kCoefStride * sizeof(double)));
+        coef = static_cast<double *>(ctx.scratch_a.ptr);
+    }
     ZXA_TRY(launch_block_coef(ctx, P, inner.data(), NCO, outer.data(), nh, coef));
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     const double *tabs = static_cast<const double *>(dblk);
@@ -535,7 +542,7 @@ int diag_flat(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates
     return ctx.ring.done(ctx.stream);
 }
 
-int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
+int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates, double *ws)
 {
     const bool c64 = prec == C64;
     const int LB = c64 ? Geo<true>::LB : Geo<false>::LB;
@@ -596,8 +603,12 @@ int diag_fast(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates
     for (int i = 0; i < nh; i++)
         outer[i] = NCO + i;
     const uint64_t nchunks = uint64_t(1) << nh;
-    ZXA_TRY(ctx.scratch_a.ensure(nchunks * kCoefStride * sizeof(double)));
-    double *coef = static_cast<double *>(ctx.scratch_a.ptr);
+    double *coef = ws;
+    if (!coef)
+    {
+        ZXA_TRY(ctx.scratch_a.ensure(nchunks * kCoefStride * sizeof(double)));
+        coef = static_cast<double *>(ctx.scratch_a.ptr);
+    }
     ZXA_TRY(launch_block_coef(ctx, P, inner.data(), NCO, outer.data(), nh, coef));
 
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
@@ -751,7 +762,15 @@ int launch_block_coef(Ctx &ctx, const Poly &P, const int *inner, int nco, const 
     return hip_check(hipGetLastError(), "block coefficient kernel");
 }
 
-int diag_apply(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
+// The window's dispatch: the autotuned flat sweep, else the chunked
+// polynomial sweep, else (windows that are not a degree-2 polynomial, or
+// states narrower than a chunk) the per-gate general sweep.  ws (nullable)
+// is a caller-given coefficient workspace of ws_bytes: a polynomial sweep
+// whose per-chunk coefficients (128 bytes per chunk) do not fit it takes the
+// polynomial sweep's larger chunks, and fails if those do not fit either.
+// With ws null the context's scratch buffer grows as needed.
+int diag_apply_ws(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates, double *ws,
+                  size_t ws_bytes)
 {
     if (ngates == 0)
         return OK;
@@ -761,21 +780,76 @@ int diag_apply(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gate
         return EINVAL_;
     }
     const bool simple = window_is_polynomial(n, gates, ngates);
+    const auto fits = [&](unsigned chunk_bits) {
+        return !ws || ((uint64_t(1) << (n - chunk_bits)) * kCoefStride * sizeof(double) <= ws_bytes);
+    };
     const int ept = tune_get(prec == C64 ? "diag.ept.c64" : "diag.ept.c128");
     if (simple && ept > 0)
     {
         const unsigned need = (prec == C64 ? 7u : 6u) + 2u + (ept == 1 ? 0u : ept == 2 ? 1u : 2u);
-        if (n >= need && (n - need) < 32)
+        if (n >= need && (n - need) < 32 && fits(need))
             return diag_flat(ctx, state, prec, n, gates, ngates, ept,
-                             tune_get(prec == C64 ? "diag.nt.c64" : "diag.nt.c128") != 0);
+                             tune_get(prec == C64 ? "diag.nt.c64" : "diag.nt.c128") != 0, ws);
     }
     const unsigned need = (prec == C64 ? Geo<true>::CB : Geo<false>::CB);
     if (simple && n >= need)
-        return diag_fast(ctx, state, prec, n, gates, ngates);
+    {
+        if (!fits(need))
+        {
+            set_error("diagonal sweep: the state's per-chunk coefficients (" +
+                      std::to_string((uint64_t(1) << (n - need)) * kCoefStride * sizeof(double)) +
+                      " bytes) exceed the workspace (" + std::to_string(ws_bytes) + " bytes)");
+            return EINVAL_;
+        }
+        return diag_fast(ctx, state, prec, n, gates, ngates, ws);
+    }
     // ring slot holds at most 256 KiB of gates; split very large windows
     const size_t per = ParamRing::kSlotBytes / sizeof(DiagGate);
     for (size_t off = 0; off < ngates; off += per)
         ZXA_TRY(diag_general(ctx, state, prec, n, gates + off, std::min(per, ngates - off)));
+    return OK;
+}
+
+int diag_apply(Ctx &ctx, void *state, int prec, unsigned n, const DiagGate *gates, size_t ngates)
+{
+    return diag_apply_ws(ctx, state, prec, n, gates, ngates, nullptr, 0);
+}
+
+// ---- the workspace of the no-allocation entry point ----
+//
+// A file-scope device array: every GPU the library runs on holds one copy,
+// at its own address, until the process ends.  Host code must not use the
+// array's name as a device address; noalloc_workspace() looks the address up
+// once per device (hipGetSymbolAddress, with that device current) and keeps it.
+constexpr size_t kNoAllocBytes = size_t(32) << 20;
+static __device__ __align__(256) double scratch[kNoAllocBytes / sizeof(double)];
+
+int noalloc_workspace(void **ptr, size_t *bytes)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    int dev = 0;
+    ZXA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices)
+    {
+        set_error("no-allocation workspace: device index out of range");
+        return EINVAL_;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev])
+    {
+        void *p = nullptr;
+        ZXA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p)
+        {
+            set_error("no-allocation workspace: symbol address is null");
+            return EHIP;
+        }
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    *bytes = kNoAllocBytes;
     return OK;
 }
 
