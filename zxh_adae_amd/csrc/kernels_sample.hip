@@ -218,6 +218,60 @@ __global__ __launch_bounds__(kThreads) void tree_amp_kernel(const void *__restri
     }
 }
 
+// Tree pass over stored nodes: level lin -> levels lin+1 .. lin+5, all
+// stored.  A wave tile is 2048 level-lin nodes (16 loads of two nodes per
+// lane); level lin+1 is the in-lane pair (stored from registers, 512
+// contiguous bytes per load), then each lane sums a row of 16 level-(lin+1)
+// nodes from LDS (rows padded to 17 doubles) as the aligned-pair tree and
+// stores its 8 + 4 + 2 + 1 nodes of levels lin+2 .. lin+5, which are
+// contiguous across the wave's lanes.  Needs 2^13 level-lin nodes per
+// workgroup.
+constexpr int kUpRowPad = 17;
+__global__ __launch_bounds__(kThreads) void tree_up_kernel(const double *__restrict__ in, double *__restrict__ o1,
+                                                           double *__restrict__ o2, double *__restrict__ o3,
+                                                           double *__restrict__ o4, double *__restrict__ o5)
+{
+    __shared__ double rows[kThreads / 64][64 * kUpRowPad];
+    const int w = threadIdx.x >> 6, L = threadIdx.x & 63;
+    const uint64_t tile = uint64_t(blockIdx.x) * (kThreads / 64) + w; // 1024 level-(lin+1) nodes each
+    const dvec2 *src = reinterpret_cast<const dvec2 *>(in) + (tile << 10);
+    double *rb = rows[w];
+    dvec2 v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+        v[k] = __builtin_nontemporal_load(src + 64 * k + L);
+#pragma unroll
+    for (int k = 0; k < 16; k++)
+    {
+        const double n = __dadd_rn(v[k].x, v[k].y);
+        const int j = 64 * k + L;
+        o1[(tile << 10) + j] = n;
+        rb[(j >> 4) * kUpRowPad + (j & 15)] = n;
+    }
+    wave_sync();
+    double q[16];
+#pragma unroll
+    for (int c = 0; c < 16; c++)
+        q[c] = rb[L * kUpRowPad + c];
+    const uint64_t r = (tile << 6) + L; // this lane's row: node index at level lin+5
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+        q[c] = __dadd_rn(q[2 * c], q[2 * c + 1]);
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        reinterpret_cast<dvec2 *>(o2)[4 * r + c] = dvec2{q[2 * c], q[2 * c + 1]};
+#pragma unroll
+    for (int c = 0; c < 4; c++)
+        q[c] = __dadd_rn(q[2 * c], q[2 * c + 1]);
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+        reinterpret_cast<dvec2 *>(o3)[2 * r + c] = dvec2{q[2 * c], q[2 * c + 1]};
+    q[0] = __dadd_rn(q[0], q[1]);
+    q[1] = __dadd_rn(q[2], q[3]);
+    reinterpret_cast<dvec2 *>(o4)[r] = dvec2{q[0], q[1]};
+    o5[r] = __dadd_rn(q[0], q[1]);
+}
+
 // prefix_s (Fenwick over levels >= I) and tail_s = prefix_s + N_I[s].
 __global__ __launch_bounds__(kThreads) void tails_kernel(const double *__restrict__ tree, const Levels lv, int I, int m,
                                                          uint64_t nseg, double *__restrict__ prefix,
@@ -420,6 +474,16 @@ int build_tree(Ctx &ctx, const void *state, int prec, unsigned m, unsigned LS, L
         first = false;
         lin = int(amp_level);
     }
+    // stored levels -> five more levels per pass while a pass fills its grid
+    while (!first && int(m) - lin >= 13 + 5)
+    {
+        const unsigned groups = static_cast<unsigned>(uint64_t(1) << (int(m) - lin - 13));
+        hipLaunchKernelGGL(tree_up_kernel, dim3(groups), dim3(kThreads), 0, ctx.stream, tree + lv.off[lin],
+                           tree + lv.off[lin + 1], tree + lv.off[lin + 2], tree + lv.off[lin + 3],
+                           tree + lv.off[lin + 4], tree + lv.off[lin + 5]);
+        ZXA_HIP(hipGetLastError());
+        lin += 5;
+    }
     while (first || lin < int(m))
     {
         const int span_bits = std::min<int>(kSpanBits, int(m) - lin);
@@ -441,7 +505,7 @@ int build_tree(Ctx &ctx, const void *state, int prec, unsigned m, unsigned LS, L
             break;
     }
     // algorithmic bytes: one read of the state and one write of the stored
-    // levels (level 4 is read back once by the pass above it; not counted)
+    // levels (the passes above the first read stored levels back; not counted)
     ZXA_TRY(ctx.toc(F_TREE, (uint64_t(1) << m) * elem_bytes(prec) + total * sizeof(double)));
     *tree_out = tree;
     return OK;
