@@ -69,6 +69,21 @@ __device__ __forceinline__ cf tof(cd a)
 {
     return {static_cast<float>(a.x), static_cast<float>(a.y)};
 }
+// exp(i a) for a complex64 state: the angle is summed and reduced modulo
+// 2 pi in double precision, the sine and cosine are taken in single precision
+// (a few ulps of float, the precision of the amplitudes they multiply; the
+// double-precision sincos made the complex64 sweep VALU-bound)
+__device__ __forceinline__ cf cis_f(double a)
+{
+    const double r = a - 6.283185307179586 * rint(a * 0.15915494309189535);
+    float s, c;
+    sincosf(static_cast<float>(r), &s, &c);
+    return {c, s};
+}
+__device__ __forceinline__ float readlane_f(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
 __device__ __forceinline__ double readlane_d(double v, int lane)
 {
     const unsigned long long b = __double_as_longlong(v);
@@ -340,11 +355,37 @@ __global__ __launch_bounds__(kThreads) void diag_flat_kernel(void *__restrict__ 
     for (int k = 0; k < RB; k++)
         if ((xw_own >> k) & 1)
             rang += cc[LB + k];
-    const cd R = cis(rang);
-#pragma unroll
-    for (int e = 0; e < EPL; e++)
+    if constexpr (kC64)
     {
-        const int xa = lane * EPL + e;
+        const cf R = cis_f(rang);
+#pragma unroll
+        for (int e = 0; e < EPL; e++)
+        {
+            const int xa = lane * EPL + e;
+            double ang = av[v * (1 << LB) + xa];
+#pragma unroll
+            for (int j = 0; j < LB; j++)
+                if ((xa >> j) & 1)
+                    ang += cc[j];
+            const cf E = cis_f(ang);
+#pragma unroll
+            for (int t = 0; t < kEPT; t++)
+            {
+                const double *g = gt + 2 * (xa * kEPT + t);
+                const cf rt = {readlane_f(R.x, t), readlane_f(R.y, t)};
+                const cf ph = cmulf(cmulf(E, {static_cast<float>(g[0]), static_cast<float>(g[1])}), rt);
+                float &re = e == 0 ? d4[t].x : d4[t].z;
+                float &im = e == 0 ? d4[t].y : d4[t].w;
+                const float a = re, b = im;
+                re = a * ph.x - b * ph.y;
+                im = a * ph.y + b * ph.x;
+            }
+        }
+    }
+    else
+    {
+        const cd R = cis(rang);
+        const int xa = lane;
         double ang = av[v * (1 << LB) + xa];
 #pragma unroll
         for (int j = 0; j < LB; j++)
@@ -357,21 +398,9 @@ __global__ __launch_bounds__(kThreads) void diag_flat_kernel(void *__restrict__ 
             const double *g = gt + 2 * (xa * kEPT + t);
             const cd rt = {readlane_d(R.x, t), readlane_d(R.y, t)};
             const cd ph = cmul(cmul(E, {g[0], g[1]}), rt);
-            if constexpr (kC64)
-            {
-                const float px = static_cast<float>(ph.x), py = static_cast<float>(ph.y);
-                float &re = e == 0 ? d4[t].x : d4[t].z;
-                float &im = e == 0 ? d4[t].y : d4[t].w;
-                const float a = re, b = im;
-                re = a * px - b * py;
-                im = a * py + b * px;
-            }
-            else
-            {
-                const double a = d2[t].x, b = d2[t].y;
-                d2[t].x = a * ph.x - b * ph.y;
-                d2[t].y = a * ph.y + b * ph.x;
-            }
+            const double a = d2[t].x, b = d2[t].y;
+            d2[t].x = a * ph.x - b * ph.y;
+            d2[t].y = a * ph.y + b * ph.x;
         }
     }
 #pragma unroll

@@ -223,6 +223,63 @@ __device__ __forceinline__ Pair<R> pair2(R ax, R ay, R bx, R by, const R (&m)[8]
 template <bool kC64>
 using Real = typename std::conditional<kC64, float, double>::type;
 
+// Complex64 gates on the matrix cores.  A 2x2 complex gate on the pair
+// (a, b) is the real 4x4 product (a', b') = M4 (ax, ay, bx, by) with rows
+//   [m00x -m00y m01x -m01y], [m00y m00x m01y m01x],
+//   [m10x -m10y m11x -m11y], [m10y m10x m11y m11x].
+// v_mfma_f32_4x4x1f32 (16 blocks of four lanes) gives lane 4b+q, register i,
+// the sum over k of A_k(lane 4b+i) * B_k(lane 4b+q): when lane q of every
+// block supplies row q of M4 as A (the same gate in all lanes) and the
+// components of its own pair as B, four steps (k = 0..3) leave each lane
+// with its own pair transformed.  The matrix cores take the gate arithmetic
+// off the VALU, which the complex64 passes otherwise saturate; the whole
+// wave must be active (every caller is uniform over the workgroup).
+typedef float v4f __attribute__((ext_vector_type(4)));
+struct MRow
+{
+    float a0, a1, a2, a3;
+};
+
+__device__ __forceinline__ MRow mfma_row(const float (&m)[8])
+{
+    const int q = threadIdx.x & 3;
+    const bool lo = q < 2, im = (q & 1) != 0;
+    const float c0 = lo ? m[0] : m[4], s0 = lo ? m[1] : m[5];
+    const float c1 = lo ? m[2] : m[6], s1 = lo ? m[3] : m[7];
+    MRow r;
+    r.a0 = im ? s0 : c0;
+    r.a1 = im ? c0 : -s0;
+    r.a2 = im ? s1 : c1;
+    r.a3 = im ? c1 : -s1;
+    return r;
+}
+
+__device__ __forceinline__ v4f mfma_pair(const MRow &r, float ax, float ay, float bx, float by)
+{
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(r.a0, ax, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(r.a1, ay, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(r.a2, bx, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_4x4x1f32(r.a3, by, acc, 0, 0, 0);
+    return acc;
+}
+
+// Units a, b of a complex64 register pair: both amplitude halves.
+__device__ __forceinline__ void mfma_units(const MRow &r, fvec4 &a, fvec4 &b)
+{
+    const v4f lo = mfma_pair(r, a.x, a.y, b.x, b.y);
+    const v4f hi = mfma_pair(r, a.z, a.w, b.z, b.w);
+    a = fvec4{lo.x, lo.y, hi.x, hi.y};
+    b = fvec4{lo.z, lo.w, hi.z, hi.w};
+}
+
+// The in-unit gate of a complex64 unit (its two amplitudes are the pair).
+__device__ __forceinline__ void mfma_intra(const MRow &r, fvec4 &v)
+{
+    const v4f o = mfma_pair(r, v.x, v.y, v.z, v.w);
+    v = fvec4{o.x, o.y, o.z, o.w};
+}
+
 template <bool kC64>
 __device__ __forceinline__ void load_matrix(Real<kC64> (&m)[8], ZXA_CONST TGate &g)
 {
@@ -279,7 +336,7 @@ __device__ __forceinline__ void unit_pair2(typename UnitT<kC64>::T &a, typename 
 // Gate g on register bit i of the thread's units v (base ^ toff: the unit
 // address of register 0; g.rpar: the selector's parity over the register
 // offsets of the layout in use).
-template <bool kC64, bool kFlip>
+template <bool kC64, bool kFlip, bool kMfma = true>
 __device__ __forceinline__ void gate_on_reg_bit(typename UnitT<kC64>::T (&v)[kRegs], int i, ZXA_CONST TGate &g,
                                                 uint64_t base, uint64_t toff)
 {
@@ -312,6 +369,18 @@ __device__ __forceinline__ void gate_on_reg_bit(typename UnitT<kC64>::T (&v)[kRe
         }
         return;
     }
+    if constexpr (kC64 && !kFlip && kMfma)
+    {
+        const MRow mr = mfma_row(m);
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+        {
+            if ((r >> i) & 1)
+                continue;
+            mfma_units(mr, v[r], v[r | (1 << i)]);
+        }
+        return;
+    }
 #pragma unroll
     for (int r = 0; r < kRegs; r++)
     {
@@ -324,7 +393,7 @@ __device__ __forceinline__ void gate_on_reg_bit(typename UnitT<kC64>::T (&v)[kRe
 
 // Gates on the register bits of layout L (and, in layout 0, the complex64
 // in-unit gate).  base ^ toff is the unit address of the thread's register 0.
-template <bool kC64, bool kFlip, int L, bool kFan = false>
+template <bool kC64, bool kFlip, int L, bool kFan = false, bool kMfma = true>
 __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs], const TPass &P, ZXA_CONST TTab *G,
                                              uint64_t base, uint64_t toff, int t = 0, ZXA_CONST double *fp = nullptr,
                                              const dvec2 *fl = nullptr)
@@ -358,13 +427,21 @@ __device__ __forceinline__ void apply_layout(typename UnitT<kC64>::T (&v)[kRegs]
                 }
             }
         }
-        gate_on_reg_bit<kC64, kFlip>(v, i, g, base, toff);
+        gate_on_reg_bit<kC64, kFlip, kMfma>(v, i, g, base, toff);
     }
     if constexpr (kC64 && L == 0)
     {
         ZXA_CONST TGate &g = G->g[kIntra];
         float m[8];
         load_matrix<true>(m, g);
+        if constexpr (!kFlip && kMfma)
+        {
+            const MRow mr = mfma_row(m);
+#pragma unroll
+            for (int r = 0; r < kRegs; r++)
+                mfma_intra(mr, v[r]);
+            return;
+        }
         const uint64_t amask = kFlip ? g.amask : 0;
         const uint32_t pt = kFlip ? (par((base ^ toff) & amask) ^ g.beta) : 0u;
 #pragma unroll
@@ -761,7 +838,9 @@ __device__ __forceinline__ int upper_thread(int t)
 // layout-0 gates the lane swaps make unit bits 6, 7 register bits 0, 1 (and
 // unit bits 8, 9 lane bits 4, 5) for their gates, and are undone before the
 // diagonal epilogue and the store.  The gates run in the tile kernel's order
-// (layout 0, then unit bits 6, 7) with the same arithmetic.  kFlip: per-pair
+// (layout 0, then unit bits 6, 7) with the same arithmetic in complex128;
+// complex64 gates run on the VALU here and on the matrix cores in the tile
+// kernel, which agree to float rounding.  kFlip: per-pair
 // selectors; the rpar masks of slots 6 and 7 are then those of the swapped
 // registers (unit bits 6, 7, 10, 11), set by the host.
 template <bool kC64, bool kDiag, bool kFlip>
@@ -781,7 +860,9 @@ __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ stat
     ZXA_CONST TTab *G = (ZXA_CONST TTab *)tabp;
     T v[kRegs];
     load_tile<kC64>(v, p, at, G);
-    apply_layout<kC64, kFlip, 0>(v, P, G, base, toff);
+    // VALU arithmetic here: with the matrix cores the compiler moves the
+    // results through AGPRs and the occupancy this kernel relies on halves
+    apply_layout<kC64, kFlip, 0, false, false>(v, P, G, base, toff);
 #pragma unroll
     for (int r = 0; r < kRegs; r += 2)
         lane_swap<false>(v[r], v[r + 1]);
@@ -796,7 +877,7 @@ __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ stat
         toff_s = toff ^ (((tl >> 6) & 1) ? P.vec[6] ^ P.vec[8] : 0) ^ (((tl >> 7) & 1) ? P.vec[7] ^ P.vec[9] : 0);
 #pragma unroll
     for (int i = 0; i < 2; i++)
-        gate_on_reg_bit<kC64, kFlip>(v, i, G->g[6 + i], base, toff_s);
+        gate_on_reg_bit<kC64, kFlip, false>(v, i, G->g[6 + i], base, toff_s);
 #pragma unroll
     for (int r = 0; r < kRegs; r++)
         if (!(r & 2))
@@ -976,13 +1057,28 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
         s[i * kSums + 2] = sf;
         R m[8];
         load_matrix<kC64>(m, G->g[k]);
-#pragma unroll
-        for (int r = 0; r < kRegs; r++)
+        if constexpr (kC64)
         {
-            if ((r >> i) & 1)
-                continue;
-            unit_pair<kC64, false>(v[r], v[r | (1 << i)], m, false, false);
-            unit_pair<kC64, false>(w[r], w[r | (1 << i)], m, false, false);
+            const MRow mr = mfma_row(m);
+#pragma unroll
+            for (int r = 0; r < kRegs; r++)
+            {
+                if ((r >> i) & 1)
+                    continue;
+                mfma_units(mr, v[r], v[r | (1 << i)]);
+                mfma_units(mr, w[r], w[r | (1 << i)]);
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int r = 0; r < kRegs; r++)
+            {
+                if ((r >> i) & 1)
+                    continue;
+                unit_pair<kC64, false>(v[r], v[r | (1 << i)], m, false, false);
+                unit_pair<kC64, false>(w[r], w[r | (1 << i)], m, false, false);
+            }
         }
         // one slot at a time: interleaving slots only lengthens live ranges
         __builtin_amdgcn_sched_barrier(0);
@@ -1000,13 +1096,12 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
         s[4 * kSums + 2] = sf;
         float m[8];
         load_matrix<true>(m, G->g[kIntra]);
+        const MRow mr = mfma_row(m);
 #pragma unroll
         for (int r = 0; r < kRegs; r++)
         {
-            const Pair<float> pv = pair2<float>(v[r].x, v[r].y, v[r].z, v[r].w, m, false);
-            v[r] = fvec4{pv.ax, pv.ay, pv.bx, pv.by};
-            const Pair<float> pw = pair2<float>(w[r].x, w[r].y, w[r].z, w[r].w, m, false);
-            w[r] = fvec4{pw.ax, pw.ay, pw.bx, pw.by};
+            mfma_intra(mr, v[r]);
+            mfma_intra(mr, w[r]);
         }
     }
     R h[half_up(kN)];
