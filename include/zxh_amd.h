@@ -163,6 +163,14 @@ int zxh_dev_alloc(size_t bytes, void **dptr);
 int zxh_dev_free(void *dptr);
 int zxh_dev_upload(void *dptr, const void *host, size_t bytes);
 int zxh_dev_download(void *host, const void *dptr, size_t bytes);
+/* worker_realloc (mem.h:9-19, mem_host.cpp:152-169): a zeroed allocation of
+ * new_bytes holding the first min(old, new) bytes of old_ptr, which is freed;
+ * new_bytes == 0 frees old_ptr and returns NULL */
+int zxh_dev_realloc(void *old_ptr, size_t old_bytes, size_t new_bytes, void **new_ptr);
+/* worker_set_zero: amplitudes [begin, end) of a state of the given precision */
+int zxh_dev_set_zero(void *dptr, int precision, size_t begin, size_t end);
+/* worker_mem_set: amplitude 0 = re + i im (the reset's |0...0>) */
+int zxh_dev_mem_set(void *dptr, int precision, double re, double im);
 int zxh_dev_sync(void);
 
 /* u3_block_batch_kernel (kernels.h:26): applies the descriptors (distinct
@@ -211,6 +219,15 @@ int zxh_k_mcx(void *state, int precision, size_t local_bits, uint64_t ctrl_mask,
 int zxh_k_sample(const void *state, int precision, size_t m, size_t seg_bits, uint64_t stream_seed, size_t shots,
                  uint64_t *out_real);
 /* sampler on a caller-provided sorted threshold array (host, length shots) */
+/* measure(sv, A, b, cache, cnt, use_seed, seed) + get_results (measure.h:
+ * 12-51, measure_common.cpp:73-92) on a state of 2^m amplitudes: the
+ * reference's threshold stream (mix_seed(use_seed ? seed : random_device,
+ * 0)), segments of 2^min(m, seg_bits), and the affine map of nq virtual qubits
+ * (rows[q] bit c = A[q][c], b[q]): packed gets ceil(nq / 64) words per shot
+ * (bit q = parity(real & rows[q]) ^ b[q], measure_result_pack.h:24-44),
+ * out_real (nullable) the real addresses, ascending. */
+int zxh_k_measure(const void *state, int precision, size_t m, size_t seg_bits, size_t nq, const uint64_t *rows,
+                  const uint8_t *b, int use_seed, uint64_t seed, size_t shots, uint64_t *packed, uint64_t *out_real);
 int zxh_k_sample_thresholds(const void *state, int precision, size_t m, size_t seg_bits, const double *thresholds,
                             size_t shots, uint64_t *out_real);
 /* sum of |amplitude|^2 in the reference's pairwise order (f64) */

@@ -618,3 +618,65 @@ def test_seg_min_i_environment_through_zxh(gpu_engine, monkeypatch, env, seg):
         got = sim.sample_real(gold[key].size)
         assert np.array_equal(got, gold[key]), (env, name, seed)
         del sim
+
+
+# Kernel-level measure (measure.h:12-51): the reference's sampler on a caller's
+# state and affine map, results packed into virtual words on the device.  The
+# oracle runs lazy-mode circuits (X/CX transport the map: A is not the
+# identity, b is not zero) and its packed results are the expected words.
+@pytest.mark.parametrize("n,count,seed", [(10, 120, 1), (14, 300, 2), (17, 400, 3)])
+def test_kernel_level_affine_measure_matches_oracle(gpu_engine, n, count, seed):
+    from zxh_adae_amd import workloads as W
+    from zxh_adae_amd.kernels import DeviceState
+    ora = O.OracleSim(n, False, False)
+    ora.gates(W.random_mixed(n, count, seed))
+    ora.execute()
+    psi = ora.phys_state()
+    rows, b, m = ora.affine()
+    assert psi.size == 1 << m
+    assert any(int(r) != (1 << q) for q, r in enumerate(rows)) or any(b)
+    shots = 3000
+    ora.set_seed(500 + seed)
+    ora.measure(shots)
+    wc = (n + 63) // 64
+    want = np.zeros(shots * wc, dtype=np.uint64)
+    assert ora.lib.zo_get_results(ora.h, O._u64ptr(want), shots) == 0
+    want_real = ora.real_results(shots)
+    d = DeviceState(m, "c128")
+    d.upload(psi)
+    packed, real = d.measure(18, rows, b, shots, seed=500 + seed)
+    assert np.array_equal(real, want_real)
+    assert np.array_equal(packed.reshape(-1), want)
+
+
+def test_dev_realloc_set_zero_and_mem_set(gpu_engine):
+    """worker_realloc keeps the common prefix and zero-fills the rest;
+    worker_set_zero clears a range; worker_mem_set writes amplitude 0."""
+    import ctypes as C
+    from zxh_adae_amd import _lib
+    lib = _lib.load()
+    n = 1000
+    host = (np.arange(n) + 1j * np.arange(n)).astype(np.complex128)
+    p = C.c_void_p()
+    assert lib.zxh_dev_alloc(host.nbytes, C.byref(p)) == 0
+    assert lib.zxh_dev_upload(p, host.ctypes.data, host.nbytes) == 0
+    q = C.c_void_p()
+    assert lib.zxh_dev_realloc(p, host.nbytes, 2 * host.nbytes, C.byref(q)) == 0
+    out = np.zeros(2 * n, dtype=np.complex128)
+    assert lib.zxh_dev_download(out.ctypes.data, q, out.nbytes) == 0
+    assert np.array_equal(out[:n], host) and not np.any(out[n:])
+    assert lib.zxh_dev_set_zero(q, _lib.ZXH_C128, 10, 20) == 0
+    assert lib.zxh_dev_mem_set(q, _lib.ZXH_C128, 0.5, -0.25) == 0
+    assert lib.zxh_dev_download(out.ctypes.data, q, out.nbytes) == 0
+    want = np.concatenate([host, np.zeros(n, dtype=np.complex128)])
+    want[10:20] = 0
+    want[0] = 0.5 - 0.25j
+    assert np.array_equal(out, want)
+    r = C.c_void_p()
+    assert lib.zxh_dev_realloc(q, 2 * host.nbytes, 100 * 16, C.byref(r)) == 0
+    small = np.zeros(100, dtype=np.complex128)
+    assert lib.zxh_dev_download(small.ctypes.data, r, small.nbytes) == 0
+    assert np.array_equal(small, want[:100])
+    z = C.c_void_p(1)
+    assert lib.zxh_dev_realloc(r, 100 * 16, 0, C.byref(z)) == 0
+    assert z.value is None

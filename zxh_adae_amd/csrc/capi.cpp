@@ -350,6 +350,66 @@ int zxh_dev_download(void *host, const void *dptr, size_t bytes)
     return zxa::hip_check(hipMemcpy(host, dptr, bytes, hipMemcpyDeviceToHost), "hipMemcpy D2H");
 }
 
+int zxh_dev_realloc(void *old_ptr, size_t old_bytes, size_t new_bytes, void **new_ptr)
+{
+    if (!new_ptr)
+        return fail(ZXH_EINVAL, "null output pointer");
+    *new_ptr = nullptr;
+    if (new_bytes == 0)
+        return old_ptr ? zxa::hip_check(hipFree(old_ptr), "hipFree") : ZXH_OK;
+    void *p = nullptr;
+    int rc = zxh_dev_alloc(new_bytes, &p);
+    if (rc)
+        return rc;
+    if (old_ptr)
+    {
+        const size_t keep = old_bytes < new_bytes ? old_bytes : new_bytes;
+        rc = zxa::hip_check(hipMemcpy(p, old_ptr, keep, hipMemcpyDeviceToDevice), "hipMemcpy D2D");
+        if (rc)
+        {
+            (void)hipFree(p);
+            return rc;
+        }
+        rc = zxa::hip_check(hipFree(old_ptr), "hipFree");
+        if (rc)
+        {
+            (void)hipFree(p);
+            return rc;
+        }
+    }
+    *new_ptr = p;
+    return ZXH_OK;
+}
+
+int zxh_dev_set_zero(void *dptr, int precision, size_t begin, size_t end)
+{
+    if (!dptr)
+        return fail(ZXH_EINVAL, "null pointer");
+    if (precision != ZXH_C128 && precision != ZXH_C64)
+        return fail(ZXH_EINVAL, "unknown precision");
+    if (end < begin)
+        return fail(ZXH_EINVAL, "set_zero: end before begin");
+    const size_t esz = precision == ZXH_C128 ? 16 : 8;
+    return zxa::hip_check(hipMemset(static_cast<char *>(dptr) + begin * esz, 0, (end - begin) * esz), "hipMemset");
+}
+
+int zxh_dev_mem_set(void *dptr, int precision, double re, double im)
+{
+    if (!dptr)
+        return fail(ZXH_EINVAL, "null pointer");
+    if (precision == ZXH_C128)
+    {
+        const double v[2] = {re, im};
+        return zxa::hip_check(hipMemcpy(dptr, v, sizeof v, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    if (precision == ZXH_C64)
+    {
+        const float v[2] = {static_cast<float>(re), static_cast<float>(im)};
+        return zxa::hip_check(hipMemcpy(dptr, v, sizeof v, hipMemcpyHostToDevice), "hipMemcpy H2D");
+    }
+    return fail(ZXH_EINVAL, "unknown precision");
+}
+
 int zxh_dev_sync(void)
 {
     int rc = 0;
@@ -503,6 +563,26 @@ int zxh_k_sample(const void *state, int precision, size_t m, size_t seg_bits, ui
     std::vector<double> thr(shots);
     zxa::threshold_stream(stream_seed, shots, 0.0, total, thr.data());
     return zxa::sample_search(*c, state, precision, thr.data(), shots, out_real);
+}
+
+int zxh_k_measure(const void *state, int precision, size_t m, size_t seg_bits, size_t nq, const uint64_t *rows,
+                  const uint8_t *b, int use_seed, uint64_t seed, size_t shots, uint64_t *packed, uint64_t *out_real)
+{
+    int rc = check_kernel_args(state, precision, m);
+    if (rc)
+        return rc;
+    if (nq == 0 || (nq && (!rows || !b)))
+        return fail(ZXH_EINVAL, "measure: null affine map");
+    if (shots && !packed)
+        return fail(ZXH_EINVAL, "null output");
+    for (size_t q = 0; q < nq; q++)
+        if (m < 64 && (rows[q] >> m) != 0)
+            return fail(ZXH_EINVAL, "measure: affine row reaches above the state's width");
+    if (shots == 0)
+        return ZXH_OK;
+    WITH_CTX(c);
+    return zxa::measure_affine(*c, state, precision, unsigned(m), unsigned(seg_bits), unsigned(nq), rows, b,
+                               use_seed != 0, seed, shots, packed, out_real);
 }
 
 int zxh_k_sample_thresholds(const void *state, int precision, size_t m, size_t seg_bits, const double *thresholds,

@@ -160,6 +160,7 @@ struct Ctx
     unsigned sample_m = 0, sample_I = 0;
     bool sample_ready = false;
     PinBuf sample_total_pin; // the tree root, copied back asynchronously
+    PinBuf sample_thr_pin;   // thresholds of the kernel-level measure
     template <typename T>
     T &sample_levels_as()
     {
@@ -407,6 +408,14 @@ struct PackSpec
 };
 int sample_search(Ctx &ctx, const void *state, int prec, const double *thresholds, size_t shots, uint64_t *out_real,
                   const PackSpec *pack = nullptr);
+// measure (measure_common.cpp:73-92) on a caller's state of 2^m amplitudes
+// and affine map (rows[q] bit c = A[q][c], b[q]; nq virtual qubits):
+// thresholds from mix_seed(use_seed ? seed : random_device, 0), segments of
+// 2^min(m, seg_bits); packed gets ceil(nq / 64) words per shot, out_real
+// (nullable) the real addresses, ascending.
+int measure_affine(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, unsigned nq,
+                   const uint64_t *rows, const uint8_t *b, bool use_seed, uint64_t seed, size_t shots,
+                   uint64_t *packed, uint64_t *out_real);
 // Both steps with caller-given thresholds.
 int sample_apply(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, const double *thresholds,
                  size_t shots, uint64_t *out_real);

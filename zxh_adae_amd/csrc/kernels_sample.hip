@@ -23,6 +23,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <random>
 #include <vector>
 
 #pragma clang fp contract(off)
@@ -659,6 +660,44 @@ int sample_search(Ctx &ctx, const void *state, int prec, const double *threshold
         return EUNRESOLVED;
     }
     return OK;
+}
+
+// measure (measure_common.cpp:73-92) on a caller's state and affine map:
+// the tree builds while the host draws the threshold stream's uniforms and
+// factors, the search resolves the thresholds, and the device packs the
+// virtual result words.
+int measure_affine(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, unsigned nq,
+                   const uint64_t *rows, const uint8_t *b, bool use_seed, uint64_t seed, size_t shots,
+                   uint64_t *packed, uint64_t *out_real)
+{
+    if (shots == 0)
+        return OK;
+    ZXA_TRY(sample_launch(ctx, state, prec, m, std::min(m, seg_bits)));
+    const uint64_t base = use_seed ? seed : uint64_t(std::random_device{}());
+    ZXA_TRY(ctx.sample_thr_pin.ensure(shots * sizeof(double)));
+    double *thr = static_cast<double *>(ctx.sample_thr_pin.ptr);
+    threshold_draws(mix_seed(base, 0), shots, thr);
+    threshold_factors(thr, shots);
+    double total = 0.0;
+    ZXA_TRY(sample_total(ctx, &total));
+    if (!(total > 0.0))
+    {
+        set_error("state has zero total probability");
+        return EZEROPROB;
+    }
+    threshold_apply(thr, shots, 0.0, total);
+    std::vector<uint64_t> realv;
+    if (!out_real)
+    {
+        realv.resize(shots);
+        out_real = realv.data();
+    }
+    PackSpec ps;
+    ps.nq = nq;
+    ps.rows = rows;
+    ps.b = b;
+    ps.out = packed;
+    return sample_search(ctx, state, prec, thr, shots, out_real, &ps);
 }
 
 int sample_apply(Ctx &ctx, const void *state, int prec, unsigned m, unsigned seg_bits, const double *thresholds,

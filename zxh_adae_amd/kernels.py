@@ -172,6 +172,23 @@ class DeviceState:
                                     out.ctypes.data_as(_lib._u64p)))
         return out[:shots]
 
+    def measure(self, seg_bits: int, rows, b, shots: int, seed=None):
+        """measure() + get_results() of the reference on this state with the
+        affine map (rows[q]: bit c = A[q][c]; b[q]): returns (packed words,
+        ceil(nq/64) per shot, and the real addresses).  seed None draws the
+        stream seed from random_device, as the reference does without one."""
+        r = np.ascontiguousarray(rows, dtype=np.uint64)
+        bb = np.ascontiguousarray(b, dtype=np.uint8)
+        nq = r.size
+        wc = (nq + 63) // 64
+        packed = np.zeros(max(shots * wc, 1), dtype=np.uint64)
+        real = np.zeros(max(shots, 1), dtype=np.uint64)
+        check(self.lib.zxh_k_measure(self.ptr, self.prec, self.nbits, seg_bits, nq, r.ctypes.data_as(_lib._u64p),
+                                     bb.ctypes.data_as(C.POINTER(C.c_uint8)), int(seed is not None),
+                                     0 if seed is None else int(seed), shots, packed.ctypes.data_as(_lib._u64p),
+                                     real.ctypes.data_as(_lib._u64p)))
+        return packed[: shots * wc].reshape(shots, wc), real[:shots]
+
     def sample_thresholds(self, seg_bits: int, thresholds: np.ndarray) -> np.ndarray:
         thr = np.ascontiguousarray(thresholds, dtype=np.float64)
         out = np.zeros(max(thr.size, 1), dtype=np.uint64)
