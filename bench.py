@@ -244,18 +244,38 @@ def batch_c4(W, repeats=3, sequential=64):
     the sequential estimate and the per-set gradient check."""
     import numpy as np
     from zxh_adae_amd.ansatz import Ansatz, AnsatzBatch, two_local
+    from zxh_adae_amd import autotune
     sets = W.c4_sets()
-    b = AnsatzBatch("c64")
-    nps = []
+    full = []
     for n, layers, terms in sets:
         ops, npar = two_local(n, layers)
+        full.append((n, ops, npar, terms))
+    nps = [s[2] for s in full]
+    offs = np.concatenate([[0], np.cumsum(nps)])
+    params = np.array(W.ansatz_params(int(offs[-1]), 4))
+    # the fewest-passes rule for every width, timed as a reference
+    b = AnsatzBatch("c64")
+    for n, ops, npar, terms in full:
         b.add(n, ops, npar, terms)
-        nps.append(npar)
+    b.finalize()
+    b.run(params, gradients=True)
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        b.run(params, gradients=True)
+    t_rule = (time.perf_counter() - t0) / repeats
+    b.close()
+    # geometry per width class from the timing autotuner
+    t0 = time.perf_counter()
+    tuned = autotune.tune_batch(full, "c64")
+    t_tune = time.perf_counter() - t0
+    b = AnsatzBatch("c64")
+    for n, seg in tuned["choice"].items():
+        b.set_geometry(int(n), int(seg))
+    for n, ops, npar, terms in full:
+        b.add(n, ops, npar, terms)
     t0 = time.perf_counter()
     b.finalize()
     t_setup = time.perf_counter() - t0
-    offs = np.concatenate([[0], np.cumsum(nps)])
-    params = np.array(W.ansatz_params(int(offs[-1]), 4))
     b.run(params, gradients=True)
     tf = tb = 0.0
     for _ in range(repeats):
@@ -305,6 +325,8 @@ def batch_c4(W, repeats=3, sequential=64):
             "forward_ms": round(tf * 1e3, 3), "forward_backward_ms": round(tb * 1e3, 3),
             "sets/s": round(len(sets) / tb, 1), "grads/s": round(int(offs[-1]) / tb, 1),
             "forward_GB/s": round(fwd_bytes / tf / 1e9, 1), "forward_backward_GB/s": round(bwd_bytes / tb / 1e9, 1),
+            "rule_forward_backward_ms": round(t_rule * 1e3, 3),
+            "autotune": {**tuned, "seconds": round(t_tune, 2)},
             "sequential_single_circuit_ms": round(seq_est * 1e3, 3), "sequential_sample": nseq,
             "speedup_vs_sequential": round(seq_est / tb, 2),
             "max_grad_diff_vs_single": gdiff, "max_energy_diff_vs_single": ediff}

@@ -323,6 +323,10 @@ int zxh_batch_destroy(zxh_batch *b);
 int zxh_batch_add(zxh_batch *b, size_t n, const zxh_ansatz_op *ops, size_t nops, size_t nparams,
                   const int32_t *qi, const int32_t *qj, const double *coef, size_t nterms);
 /* allocates the arena, prepares the launches, captures the graphs */
+/* Tile geometry (3..6 contiguous unit bits; 0 = the fewest-passes rule) of
+ * the sets of n qubits added after this call: the batch autotuner times each
+ * width class under every geometry and sets its choice here. */
+int zxh_batch_set_geometry(zxh_batch *b, size_t n, int seg);
 int zxh_batch_finalize(zxh_batch *b);
 /* params: all sets' parameters concatenated in set order; energies[2 * sets]
  * (<H>, <psi|psi>); grads (nullable: forward only) concatenated like params */

@@ -66,3 +66,38 @@ def test_batch_rejects_narrow_and_unfinalized(gpu_engine):
     b.finalize()
     with pytest.raises(ZXHError):
         b.set_info(1)
+
+
+@pytest.mark.parametrize("seg", [3, 4, 6])
+def test_batch_with_forced_geometry_matches_single_circuits(gpu_engine, seg):
+    """The batch autotuner's per-width geometry (zxh_batch_set_geometry): a
+    width class forced to geometry seg, another left to the rule, both equal
+    to the single-circuit path (complex128)."""
+    from zxh_adae_amd.ansatz import Ansatz, AnsatzBatch, two_local
+    rng = np.random.default_rng(seg)
+    widths = [17, 15, 17, 16]
+    b = AnsatzBatch("c128")
+    b.set_geometry(17, seg)
+    sets = []
+    for k, n in enumerate(widths):
+        ops, npar = two_local(n, 2 + k % 2)
+        terms = rand_terms(rng, n, 10)
+        b.add(n, ops, npar, terms)
+        sets.append((n, ops, npar, terms))
+    b.finalize()
+    params = [rng.uniform(-math.pi, math.pi, s[2]) for s in sets]
+    e, nrm, g = b.run(params)
+    for k, (n, ops, npar, terms) in enumerate(sets):
+        a = Ansatz(n, ops, npar, "c128")
+        a.set_observable(terms)
+        gk, ek = a.backward(params[k])
+        a.close()
+        scale = max(1.0, sum(abs(t[0]) for t in terms))
+        assert abs(e[k] - ek) < 1e-11 * scale
+        assert np.max(np.abs(g[k] - gk)) < 1e-11 * scale
+    from zxh_adae_amd import ZXHError
+    with pytest.raises(ZXHError):
+        b.set_geometry(16, 4)  # after finalize
+    b2 = AnsatzBatch("c64")
+    with pytest.raises(ZXHError):
+        b2.set_geometry(16, 7)

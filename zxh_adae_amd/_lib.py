@@ -88,6 +88,7 @@ _SIGS = [
     ("zxh_k_block5", C.c_int, [_vp, C.c_int, _sz, C.POINTER(C.c_uint32), _dp]),
     ("zxh_k_reset", C.c_int, [_vp, C.c_int, _sz]),
     ("zxh_k_compare", C.c_int, [_vp, C.c_int, _vp, C.c_int, _sz, _dp]),
+    ("zxh_batch_set_geometry", C.c_int, [_vp, _sz, C.c_int]),
     ("zxh_k_measure", C.c_int, [_vp, C.c_int, _sz, _sz, _sz, _u64p, C.POINTER(C.c_uint8), C.c_int, _u64, _sz,
                                 _u64p, _u64p]),
     ("zxh_dev_realloc", C.c_int, [_vp, _sz, _sz, C.POINTER(_vp)]),

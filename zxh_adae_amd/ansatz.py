@@ -167,6 +167,11 @@ class AnsatzBatch:
                                      cf.ctypes.data_as(_lib._dp), len(cf)))
         self.nparams.append(int(nparams))
 
+    def set_geometry(self, n: int, seg: int):
+        """Tile geometry (3..6; 0 = the fewest-passes rule) for the sets of n
+        qubits added after this call."""
+        check(self.lib.zxh_batch_set_geometry(self._h, int(n), int(seg)))
+
     def finalize(self):
         check(self.lib.zxh_batch_finalize(self._h))
 

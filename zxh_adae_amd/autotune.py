@@ -127,3 +127,49 @@ def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dic
     return {"knob": knob, "best": best, "previous": old,
             "layer_ms": {f"seg{k}": round(v, 4) for k, v in results.items()},
             "passes": {f"seg{k}": v for k, v in npasses.items()}}
+
+
+def tune_batch(sets, precision: str = "c64", sample: int = 8, repeats: int = 2, seed: int = 11) -> Dict[str, object]:
+    """Tile geometry per width class of a batched-ansatz workload, by timing.
+
+    sets: the workload's (n, ops, nparams, terms).  For every width n, up to
+    `sample` of its sets form a small batch, timed (forward and adjoint, one
+    graph replay each, after a warm-up) under the fewest-passes rule (0) and
+    under each geometry 3..6; the fastest is kept, the rule when it is within
+    TILE_TOLERANCE of the fastest.  Returns {n: seg} with the timings."""
+    import time
+
+    import numpy as np
+
+    from .ansatz import AnsatzBatch
+
+    by_width: Dict[int, list] = {}
+    for s in sets:
+        by_width.setdefault(int(s[0]), []).append(s)
+    choice: Dict[int, int] = {}
+    timings: Dict[str, Dict[str, float]] = {}
+    rng = np.random.default_rng(seed)
+    for n in sorted(by_width):
+        group = by_width[n][:sample]
+        total = sum(int(s[2]) for s in group)
+        params = rng.uniform(-np.pi, np.pi, total)
+        res: Dict[int, float] = {}
+        for seg in (0,) + TILE_SEGS:
+            b = AnsatzBatch(precision)
+            try:
+                b.set_geometry(n, seg)
+                for (w, ops, npar, terms) in group:
+                    b.add(w, ops, npar, terms)
+                b.finalize()
+                b.run(params)
+                t0 = time.perf_counter()
+                for _ in range(repeats):
+                    b.run(params)
+                res[seg] = (time.perf_counter() - t0) / repeats
+            finally:
+                b.close()
+        fastest = min(res.values())
+        best = 0 if res[0] <= fastest * (1.0 + TILE_TOLERANCE) else min(res, key=res.get)
+        choice[n] = best
+        timings[str(n)] = {("rule" if k == 0 else f"seg{k}"): round(v * 1e3, 3) for k, v in res.items()}
+    return {"choice": {str(k): v for k, v in choice.items()}, "ms": timings, "sample": sample}

@@ -105,11 +105,20 @@ int Ansatz::set_observable(const int32_t *qi, const int32_t *qj, const double *c
 }
 
 // The schedule depends only on the structure: the tile geometry with the
-// fewest passes, ties going to the longest contiguous runs.
+// fewest passes, ties going to the longest contiguous runs, unless a
+// geometry was set (the batch autotuner's timed choice).
 int Ansatz::build_schedule()
 {
     std::vector<Pass> best, p;
     unsigned bseg = 0;
+    if (force_seg_ >= 3 && force_seg_ <= 6)
+    {
+        schedule_for(force_seg_, best);
+        passes_.swap(best);
+        seg_ = force_seg_;
+        tile_passes_ = passes_.size();
+        return OK;
+    }
     for (unsigned seg = 6; seg >= 3; seg--)
     {
         schedule_for(seg, p);

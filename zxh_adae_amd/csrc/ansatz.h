@@ -58,6 +58,12 @@ class Ansatz
     // device = false: schedule only (no stream, no device states), for the
     // batched engine, which keeps every set's states in one arena.
     int init(const AnsatzOp *ops, size_t nops, bool device = true);
+    // Before init: build the schedule for tile geometry seg (3..6) instead of
+    // the one with the fewest passes (0 restores that rule).
+    void set_geometry(unsigned seg)
+    {
+        force_seg_ = seg;
+    }
     int set_observable(const int32_t *qi, const int32_t *qj, const double *coef, size_t nterms);
     // energy[0] = <psi|H|psi>, energy[1] = <psi|psi>
     int forward(const double *params, double *energy);
@@ -138,7 +144,8 @@ class Ansatz
     std::vector<AnsatzOp> ops_;
     std::vector<Pass> passes_; // tile passes, in forward order
     size_t tile_passes_ = 0;
-    unsigned seg_ = 0; // tile geometry the schedule was built for
+    unsigned seg_ = 0;       // tile geometry the schedule was built for
+    unsigned force_seg_ = 0; // geometry asked for by set_geometry (0: the rule)
     std::vector<int32_t> qi_, qj_;
     std::vector<double> coef_;
     void *psi_ = nullptr;

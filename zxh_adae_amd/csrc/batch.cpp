@@ -59,11 +59,29 @@ int AnsatzBatch::add(unsigned n, const AnsatzOp *ops, size_t nops, size_t nparam
     }
     Set s;
     s.a.reset(new Ansatz(n, prec_, nparams));
+    if (n < 65)
+        s.a->set_geometry(geometry_[n]);
     ZXA_TRY(s.a->init(ops, nops, false));
     ZXA_TRY(s.a->set_observable(qi, qj, coef, nterms));
     s.poff = total_params_;
     total_params_ += nparams;
     sets_.push_back(std::move(s));
+    return OK;
+}
+
+int AnsatzBatch::set_geometry(unsigned n, unsigned seg)
+{
+    if (n >= 65 || (seg != 0 && (seg < 3 || seg > 6)))
+    {
+        set_error("batch: geometry must be 0 or 3..6, width at most 64");
+        return EINVAL_;
+    }
+    if (final_)
+    {
+        set_error("batch: set_geometry after finalize");
+        return ESTATE;
+    }
+    geometry_[n] = seg;
     return OK;
 }
 

@@ -29,6 +29,9 @@ class AnsatzBatch
     int add(unsigned n, const AnsatzOp *ops, size_t nops, size_t nparams, const int32_t *qi, const int32_t *qj,
             const double *coef, size_t nterms);
     int finalize();
+    // Tile geometry (3..6; 0: the fewest-passes rule) for the sets of n
+    // qubits added after this call.
+    int set_geometry(unsigned n, unsigned seg);
     // params: every set's parameters, concatenated in set order.  energies
     // [2 * sets]: <H>, <psi|psi> per set.  grads (nullable: forward only):
     // concatenated like params.
@@ -59,6 +62,7 @@ class AnsatzBatch
     }
 
   private:
+    unsigned geometry_[65] = {}; // per width: forced tile geometry (0: the rule)
     struct Set
     {
         std::unique_ptr<Ansatz> a;

@@ -854,6 +854,14 @@ int zxh_batch_add(zxh_batch *b, size_t n, const zxh_ansatz_op *ops, size_t nops,
                      nparams, qi, qj, coef, nterms);
 }
 
+int zxh_batch_set_geometry(zxh_batch *b, size_t n, int seg)
+{
+    CHECK_BATCH(b);
+    if (seg < 0)
+        return fail(ZXH_EINVAL, "batch: negative geometry");
+    return b->b->set_geometry(unsigned(std::min<size_t>(n, 65)), unsigned(seg));
+}
+
 int zxh_batch_finalize(zxh_batch *b)
 {
     CHECK_BATCH(b);
