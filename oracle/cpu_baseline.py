@@ -30,6 +30,16 @@ Rows (one JSON line on stdout):
 
     OMP_NUM_THREADS=16 OMP_PROC_BIND=close OMP_PLACES=cores \\
         python oracle/cpu_baseline.py --c1-repeats 5 --pass-repeats 5
+
+Calibration (only where oracle/_ref/libzxhref.so, the reference's own omp
+sources compiled by `make -C oracle ref`, is built; never on the GPU box):
+
+    OMP_NUM_THREADS=8 python oracle/cpu_baseline.py --calibrate
+
+times C1's execute() and Sampling(1024) on both builds in the same process
+and prints the port / reference ratio of the execute times (the baseline is
+meant to stay within 20% of the reference) and checks that both sample the
+same bits.
 """
 from __future__ import annotations
 
@@ -171,6 +181,33 @@ def qft_estimate(p, reset_ms):
     return {"gates": gates, "estimated_ms": round(total, 1), "gates/s": round(gates / (total * 1e-3), 3)}
 
 
+def calibrate(repeats: int):
+    """C1 on the port (liboracle_base.so) and on the compiled reference."""
+    if not O.ref_available():
+        raise SystemExit("calibration needs oracle/_ref/libzxhref.so (make -C oracle ref)")
+    gates = W.random_layers(20, 20, C1_SEED)
+    out = {}
+    for name, cls in (("port", O.OracleSim), ("reference", O.RefSim)):
+        s = cls(20, False, True)
+        s.gates(gates)
+        s.set_seed(C1_SAMPLE_SEED)
+        s.execute()
+        s.sampling_bits(C1_SHOTS)
+        te, ts = [], []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            s.execute()
+            t1 = time.perf_counter()
+            bits = s.sampling_bits(C1_SHOTS)
+            ts.append(time.perf_counter() - t1)
+            te.append(t1 - t0)
+        out[name] = {"execute": _stats(te), "sampling": _stats(ts), "bits": bits}
+        del s
+    same = bool(np.array_equal(out["port"].pop("bits"), out["reference"].pop("bits")))
+    ratio = out["port"]["execute"]["mean_ms"] / out["reference"]["execute"]["mean_ms"]
+    return {**out, "same_sample_bits": same, "execute_port_over_reference": round(ratio, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--c1-repeats", type=int, default=5)
@@ -179,8 +216,13 @@ def main():
                     help="time each pass over 2^shift times fewer amplitudes (scaled to 2^30)")
     ap.add_argument("--c1-budget-s", type=float, default=20.0)
     ap.add_argument("--no-passes", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="C1 on the port and on oracle/_ref (this container only)")
     args = ap.parse_args()
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    if args.calibrate:
+        print(json.dumps({"threads": threads, **calibrate(args.c1_repeats)}), flush=True)
+        return
     res = {"threads": threads, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
            "omp_places": os.environ.get("OMP_PLACES"), "build": "oracle/liboracle_base.so"}
     t0 = time.perf_counter()
