@@ -2127,118 +2127,228 @@ struct BItem
     uint64_t psi;     // state (16-byte units)
     uint64_t lam;     // adjoint: second state
     uint64_t tab;     // TTab
-    uint64_t ttab;    // epilogue per-thread table, 0: no window
-    uint64_t tphase;  // epilogue per-tile factors
+    uint64_t ttab;    // epilogue per-thread table (the identity table: no window)
+    uint64_t tphase;  // epilogue per-tile factors (one identity row: no window)
+    uint64_t tstride; // doubles per tile in tphase; 0: no window
     uint64_t partial; // adjoint: per tile and wave, a row of 64 lane sums
     uint32_t ub[kUnitBits];
     uint32_t sorted[kUnitBits];
 };
 
-// kL1 / kL2: the launch's items use layout 1 / 2 (any of them); items that
-// do not get identity matrices there.
-template <bool kC64, bool kAdj, bool kL1, bool kL2>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_tile_kernel(
+// The item holding global tile g: binary search over the items' first-tile
+// indices (scalar loads).
+__device__ __forceinline__ int batch_find(ZXA_CONST BItem *IT, int nitems, uint64_t g)
+{
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi)
+    {
+        const int mid = (lo + hi + 1) >> 1;
+        if (IT[mid].first <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void batch_pass(ZXA_CONST BItem &I, TPass &P)
+{
+#pragma unroll
+    for (int k = 0; k < kUnitBits; k++)
+    {
+        P.ub[k] = I.ub[k];
+        P.sorted[k] = I.sorted[k];
+        P.vec[k] = uint64_t(1) << I.ub[k];
+    }
+}
+
+template <bool kC64>
+__device__ __forceinline__ void batch_load(typename UnitT<kC64>::T (&v)[kRegs], ZXA_CONST BItem &I, uint64_t g, int t)
+{
+    using T = typename UnitT<kC64>::T;
+    TPass P;
+    batch_pass(I, P);
+    const uint64_t base = deposit(g - I.first, P.sorted);
+    load_tile<kC64>(v, reinterpret_cast<const T *>(I.psi), base ^ thr_off<0>(t, P), (ZXA_CONST TTab *)I.tab);
+}
+
+template <bool kC64>
+__device__ __forceinline__ void batch_store(const typename UnitT<kC64>::T (&v)[kRegs], ZXA_CONST BItem &I, uint64_t g,
+                                            int t)
+{
+    using T = typename UnitT<kC64>::T;
+    TPass P;
+    batch_pass(I, P);
+    const uint64_t base = deposit(g - I.first, P.sorted);
+    store_tile<kC64>(v, reinterpret_cast<T *>(I.psi), base ^ thr_off<0>(t, P), (ZXA_CONST TTab *)I.tab);
+}
+
+// Gates and transposes of one forward tile, then the item's window (tphase
+// set) with the per-thread table e loaded before the tile's work.
+template <bool kC64, bool kL1, bool kL2>
+__device__ __forceinline__ void batch_work(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds,
+                                           ZXA_CONST BItem &I, uint64_t g, int t, const Epi<kC64> &e)
+{
+    TPass P;
+    batch_pass(I, P);
+    const uint64_t local = g - I.first;
+    const uint64_t base = deposit(local, P.sorted);
+    const uint64_t toff = thr_off<0>(t, P);
+    ZXA_CONST TTab *G = (ZXA_CONST TTab *)I.tab;
+    apply_layout<kC64, false, 0>(v, P, G, base, toff);
+    if constexpr (kL1)
+    {
+        transpose<kC64, 0, 1>(v, lds, t);
+        apply_layout<kC64, false, 1>(v, P, G, base, toff);
+        if constexpr (kL2)
+        {
+            transpose<kC64, 1, 2>(v, lds, t);
+            apply_layout<kC64, false, 2>(v, P, G, base, toff);
+            transpose<kC64, 2, 0>(v, lds, t);
+        }
+        else
+        {
+            transpose<kC64, 1, 0>(v, lds, t);
+        }
+    }
+    else if constexpr (kL2)
+    {
+        transpose<kC64, 0, 2>(v, lds, t);
+        apply_layout<kC64, false, 2>(v, P, G, base, toff);
+        transpose<kC64, 2, 0>(v, lds, t);
+    }
+    if (I.tstride != 0)
+        e.apply(v, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * Epi<kC64>::kStride), t);
+}
+
+// Forward passes of many states, pipelined as tile_kernel: two register
+// buffers, the loads of the workgroup's next tile in flight while the
+// current one is worked on, its store issued after those loads.  The next
+// tile may belong to another item.  Every item has a valid per-thread
+// epilogue table (the identity where it has no window), so that table's
+// loads are unconditional and issued before the prefetch: waiting for them
+// never waits for the prefetched tile.  Past the workgroup's last tile the
+// prefetch re-reads the current one.
+template <bool kC64, bool kL1, bool kL2>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_fwd_kernel(
+    const BItem *__restrict__ items, int nitems, uint64_t total)
+{
+    using T = typename UnitT<kC64>::T;
+    __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
+    const int t = threadIdx.x;
+    const uint64_t grid = gridDim.x;
+    const BItem *I0 = items;
+    asm volatile("" : "+s"(I0));
+    ZXA_CONST BItem *IT = (ZXA_CONST BItem *)I0;
+    T a[kRegs], b[kRegs];
+    Epi<kC64> e;
+    uint64_t g0 = blockIdx.x;
+    int i0 = batch_find(IT, nitems, g0);
+    batch_load<kC64>(a, IT[i0], g0, t);
+    e.init(reinterpret_cast<const double *>(IT[i0].ttab), t);
+    uint64_t g1 = g0 + grid;
+    bool m1 = g1 < total;
+    int i1 = m1 ? batch_find(IT, nitems, g1) : i0;
+    uint64_t h1 = m1 ? g1 : g0;
+    batch_load<kC64>(b, IT[i1], h1, t);
+    batch_work<kC64, kL1, kL2>(a, lds, IT[i0], g0, t, e);
+    for (;;)
+    {
+        batch_store<kC64>(a, IT[i0], g0, t);
+        if (!m1)
+            break;
+        const uint64_t g2 = g1 + grid;
+        const bool m2 = g2 < total;
+        const int i2 = m2 ? batch_find(IT, nitems, g2) : i1;
+        const uint64_t h2 = m2 ? g2 : g1;
+        e.init(reinterpret_cast<const double *>(IT[i1].ttab), t);
+        batch_load<kC64>(a, IT[i2], h2, t);
+        batch_work<kC64, kL1, kL2>(b, lds, IT[i1], g1, t, e);
+
+        batch_store<kC64>(b, IT[i1], g1, t);
+        if (!m2)
+            break;
+        const uint64_t g3 = g2 + grid;
+        const bool m3 = g3 < total;
+        const int i3 = m3 ? batch_find(IT, nitems, g3) : i2;
+        const uint64_t h3 = m3 ? g3 : g2;
+        e.init(reinterpret_cast<const double *>(IT[i2].ttab), t);
+        batch_load<kC64>(b, IT[i3], h3, t);
+        batch_work<kC64, kL1, kL2>(a, lds, IT[i2], g2, t, e);
+
+        g0 = g2;
+        i0 = i2;
+        g1 = g3;
+        i1 = i3;
+        m1 = m3;
+    }
+}
+
+// Adjoint passes of many states, one tile per step (two states per tile
+// leave no registers for a second buffer).  kL1 / kL2: the launch's items use
+// layout 1 / 2 (any of them); items that do not get identity matrices there.
+template <bool kC64, bool kL1, bool kL2>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void batch_adj_kernel(
     const BItem *__restrict__ items, int nitems, uint64_t total)
 {
     using T = typename UnitT<kC64>::T;
     __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
     const int t = threadIdx.x;
     const int lane = t & 63;
+    const BItem *I0 = items;
+    asm volatile("" : "+s"(I0));
+    ZXA_CONST BItem *IT = (ZXA_CONST BItem *)I0;
     for (uint64_t g = blockIdx.x; g < total; g += gridDim.x)
     {
-        const BItem *I0 = items;
-        asm volatile("" : "+s"(I0));
-        ZXA_CONST BItem *IT = (ZXA_CONST BItem *)I0;
-        int lo = 0, hi = nitems - 1;
-        while (lo < hi)
-        {
-            const int mid = (lo + hi + 1) >> 1;
-            if (IT[mid].first <= g)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        ZXA_CONST BItem &I = IT[lo];
+        ZXA_CONST BItem &I = IT[batch_find(IT, nitems, g)];
         TPass P;
-#pragma unroll
-        for (int k = 0; k < kUnitBits; k++)
-        {
-            P.ub[k] = I.ub[k];
-            P.sorted[k] = I.sorted[k];
-            P.vec[k] = uint64_t(1) << I.ub[k];
-        }
+        batch_pass(I, P);
         const uint64_t local = g - I.first;
         const uint64_t base = deposit(local, P.sorted);
         const uint64_t toff = thr_off<0>(t, P);
         ZXA_CONST TTab *G = (ZXA_CONST TTab *)I.tab;
         T *pp = reinterpret_cast<T *>(I.psi);
-        T v[kRegs];
+        T *pl = reinterpret_cast<T *>(I.lam);
+        T v[kRegs], w[kRegs];
         load_tile<kC64>(v, pp, base ^ toff, G);
-        if constexpr (kAdj)
-        {
-            T *pl = reinterpret_cast<T *>(I.lam);
-            T w[kRegs];
-            load_tile<kC64>(w, pl, base ^ toff, G);
-            AdjRest<kC64> R;
-            adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
-            if (I.ttab != 0)
-            {
-                Epi<kC64> epi;
-                epi.init(reinterpret_cast<const double *>(I.ttab), t);
-                epi.template apply_n<true>(
-                    v, w, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * Epi<kC64>::kStride),
-                    t);
-            }
-            store_tile<kC64>(v, pp, base ^ toff, G);
-            store_tile<kC64>(w, pl, base ^ toff, G);
-            reinterpret_cast<double *>(I.partial)[(local * kWavesPerTile + (t >> 6)) * 64 + lane] = R.total(lane);
-        }
-        else
-        {
-            apply_layout<kC64, false, 0>(v, P, G, base, toff);
-            if constexpr (kL1)
-            {
-                transpose<kC64, 0, 1>(v, lds, t);
-                apply_layout<kC64, false, 1>(v, P, G, base, toff);
-                if constexpr (kL2)
-                {
-                    transpose<kC64, 1, 2>(v, lds, t);
-                    apply_layout<kC64, false, 2>(v, P, G, base, toff);
-                    transpose<kC64, 2, 0>(v, lds, t);
-                }
-                else
-                {
-                    transpose<kC64, 1, 0>(v, lds, t);
-                }
-            }
-            else if constexpr (kL2)
-            {
-                transpose<kC64, 0, 2>(v, lds, t);
-                apply_layout<kC64, false, 2>(v, P, G, base, toff);
-                transpose<kC64, 2, 0>(v, lds, t);
-            }
-            if (I.ttab != 0)
-            {
-                Epi<kC64> epi;
-                epi.init(reinterpret_cast<const double *>(I.ttab), t);
-                epi.apply(v, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * Epi<kC64>::kStride),
-                          t);
-            }
-            store_tile<kC64>(v, pp, base ^ toff, G);
-        }
+        load_tile<kC64>(w, pl, base ^ toff, G);
+        AdjRest<kC64> R;
+        adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
+        // unconditional (identity tables where the item has no window): a
+        // branch around the epilogue made the compiler spill about 90 VGPRs
+        Epi<kC64> epi;
+        epi.init(reinterpret_cast<const double *>(I.ttab), t);
+        epi.template apply_n<true>(
+            v, w, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * I.tstride), t);
+        store_tile<kC64>(v, pp, base ^ toff, G);
+        store_tile<kC64>(w, pl, base ^ toff, G);
+        reinterpret_cast<double *>(I.partial)[(local * kWavesPerTile + (t >> 6)) * 64 + lane] = R.total(lane);
     }
 }
 
 template <bool kC64, bool kAdj>
 auto batch_kernel_for(bool l1, bool l2)
 {
-    if (l1 && l2)
-        return batch_tile_kernel<kC64, kAdj, true, true>;
-    if (l1)
-        return batch_tile_kernel<kC64, kAdj, true, false>;
-    if (l2)
-        return batch_tile_kernel<kC64, kAdj, false, true>;
-    return batch_tile_kernel<kC64, kAdj, false, false>;
+    if constexpr (kAdj)
+    {
+        if (l1 && l2)
+            return batch_adj_kernel<kC64, true, true>;
+        if (l1)
+            return batch_adj_kernel<kC64, true, false>;
+        if (l2)
+            return batch_adj_kernel<kC64, false, true>;
+        return batch_adj_kernel<kC64, false, false>;
+    }
+    else
+    {
+        if (l1 && l2)
+            return batch_fwd_kernel<kC64, true, true>;
+        if (l1)
+            return batch_fwd_kernel<kC64, true, false>;
+        if (l2)
+            return batch_fwd_kernel<kC64, false, true>;
+        return batch_fwd_kernel<kC64, false, false>;
+    }
 }
 
 using BatchKernel = void (*)(const BItem *, int, uint64_t);
@@ -2630,12 +2740,15 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
     for (size_t i = 0; i < specs.size() && adj; i++)
         clear_grads(&grads[i * kAccs]);
     b.nparams.assign(specs.size(), 0);
-    // epilogue tables: per item, per-thread table then per-tile factors
-    std::vector<size_t> epi_off(specs.size(), 0), epi_tt(specs.size(), 0);
-    size_t epi_bytes = 0;
+    // epilogue tables: per item, per-thread table then per-tile factors.  The
+    // items without a window share an identity per-thread table and one
+    // identity row of per-tile factors at the start (tstride 0), so that
+    // kernels can load and apply an epilogue without a branch.
     const int nb = c64 ? 5 : 4;
     const size_t ttab_doubles = size_t(kThreads) * 2 * (1 + nb);
     const size_t stride = 2 * (8 + (size_t(1) << nb));
+    std::vector<size_t> epi_off(specs.size(), 0), epi_tt(specs.size(), 0);
+    size_t epi_bytes = (ttab_doubles + stride) * sizeof(double);
     uint64_t first = 0, partial_doubles = 0;
     std::vector<Plan> plans(specs.size());
     for (size_t i = 0; i < specs.size(); i++)
@@ -2710,9 +2823,14 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
     // before the first set/upload is harmless
     ZXA_HIP(hipMemcpy(dtabs, b.h_tables, b.tab_bytes + b.adj_bytes, hipMemcpyHostToDevice));
     char *depi = nullptr;
-    if (epi_bytes)
-        ZXA_TRY(dmalloc(&depi, epi_bytes));
+    ZXA_TRY(dmalloc(&depi, epi_bytes));
     b.d_epi = depi;
+    {
+        std::vector<double> ident(ttab_doubles + stride, 0.0);
+        for (size_t k = 0; k < ident.size(); k += 2)
+            ident[k] = 1.0;
+        ZXA_HIP(hipMemcpy(depi, ident.data(), ident.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     double *dpart = nullptr;
     if (adj)
         ZXA_TRY(dmalloc(&dpart, partial_doubles * sizeof(double)));
@@ -2725,6 +2843,9 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         if (adj)
             it.partial = reinterpret_cast<uint64_t>(dpart + it.partial);
         const BatchPassSpec &sp = specs[i];
+        it.ttab = reinterpret_cast<uint64_t>(depi);
+        it.tphase = reinterpret_cast<uint64_t>(depi + ttab_doubles * sizeof(double));
+        it.tstride = 0;
         if (sp.window.empty())
             continue;
         Poly poly;
@@ -2747,6 +2868,7 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         ZXA_HIP(e);
         it.ttab = reinterpret_cast<uint64_t>(depi + epi_tt[i]);
         it.tphase = reinterpret_cast<uint64_t>(depi + epi_off[i]);
+        it.tstride = stride;
     }
     ZXA_TRY(dmalloc(&b.d_items, items.size() * sizeof(BItem)));
     ZXA_HIP(hipMemcpy(b.d_items, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice));
