@@ -178,8 +178,11 @@ int zxh_dev_sync(void);
 int zxh_k_u3_batch(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc);
 /* one fused tile pass: the U3 descriptors (single-bit pair offsets with
  * alpha == delta, at most 4 targets below bit 4 and 8 above, distinct), then
- * the diagonal window (selectors with at most one alpha bit), in one sweep
- * (no reference counterpart: the reference runs these as separate sweeps) */
+ * the diagonal window, in one sweep.  The window is a phase polynomial
+ * (selectors with at most one alpha bit, or one two-bit selector) or any
+ * selectors whose alpha parts span at most 10 dimensions over GF(2) (applied
+ * as a table over those forms); other windows are rejected (no reference
+ * counterpart: the reference runs these as separate sweeps) */
 int zxh_k_fused_pass(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc,
                      const uint64_t *sel0, const uint64_t *sel1, const double *theta, size_t ndiag);
 /* diag_pending_push_p / try_push_cp / flush (kernels.h:36-41) as one call:

@@ -381,3 +381,64 @@ def test_fans_let_dependent_gates_share_a_pass(gpu_engine, n, seed, seg):
     assert passes["fused"] * 2 < passes["reference"]
     c64 = make_sim(Z, n, False, True, gs, prec="c64")
     assert np.max(np.abs(c64.get_state_array() - want)) < 2e-5
+
+
+def transported_windows(n, layers, width, seed):
+    """A CX network (every row of the affine map several bits wide), then
+    layers of H on a few qubits followed by a window of P and CP gates on
+    `width` random qubits: in lazy mode each H cluster has wide pair offsets
+    or per-pair selectors, and each window's selectors are parities of many
+    address bits, spanning up to `width` dimensions."""
+    rng = np.random.default_rng(seed)
+    gs = [(W.H, 0, q, 0, 0, 0) for q in range(n)]
+    for _ in range(3):
+        for q in range(n - 1):
+            gs.append((W.CX, q, q + 1, 0, 0, 0))
+        for q in range(0, n - 2, 2):
+            gs.append((W.CX, q + 2, q, 0, 0, 0))
+    for _ in range(layers):
+        for q in rng.choice(n, 3, replace=False):
+            gs.append((W.H, 0, int(q), 0, 0, 0))
+        qs = [int(q) for q in rng.choice(n, width, replace=False)]
+        for k, q in enumerate(qs):
+            gs.append((W.P, 0, q, float(rng.uniform(-math.pi, math.pi)), 0, 0))
+            if k:
+                gs.append((W.CP, qs[k - 1], q, float(rng.uniform(-math.pi, math.pi)), 0, 0))
+    return gs
+
+
+@pytest.mark.parametrize("n,width,seed", [(16, 4, 81), (17, 9, 82), (18, 10, 83), (16, 12, 84)])
+def test_transported_windows_ride_the_tile_passes(gpu_engine, n, width, seed):
+    """Windows after CX transport (selectors over many address bits) as table
+    epilogues of the preceding cluster's last tile pass: the state matches
+    the oracle (complex128 1e-12, complex64 2e-5) and, while the windows span
+    at most ten dimensions, no diagonal sweep of their own runs but for the
+    first window, whose commuting part is flushed before any cluster.  The
+    open batch merges the windows of neighbouring layers, and a merged window
+    spanning more than ten dimensions falls back to a sweep: with wider
+    windows only some ride."""
+    Z = gpu_engine
+    gs = transported_windows(n, 12, width, seed)
+    o = O.OracleSim(n, False, False)
+    o.gates(gs)
+    o.execute()
+    want = o.get_state()
+    s = Z.ZXH(n, False, False)
+    s.gates(gs)
+    s.execute()
+    assert s.required_M() == o.required_M()
+    assert np.max(np.abs(s.get_state_array() - want)) < TOL128
+    s.reset_timing()
+    s.set_timing(True)
+    s.execute()
+    s.set_timing(False)
+    assert np.max(np.abs(s.get_state_array() - want)) < TOL128
+    sweeps = s.timing("diag")["launches"]
+    if width <= 4:
+        assert sweeps <= 1, sweeps
+    elif width <= 10:
+        assert sweeps < 12, sweeps
+    r = make_sim(Z, n, False, False, gs, policy="reference")
+    assert np.max(np.abs(r.get_state_array() - want)) < TOL128
+    c = make_sim(Z, n, False, False, gs, prec="c64")
+    assert np.max(np.abs(c.get_state_array() - want)) < 2e-5

@@ -568,6 +568,80 @@ def test_autotuners_follow_their_rules(upper_knobs, prec):
             A.tune_set(k, v)
 
 
+def spanned_window(rng, n, count, rank):
+    """count P/CP gates whose selectors are random XOR combinations (with
+    random beta) of `rank` random address forms: a window of that rank."""
+    basis = [int(rng.integers(1, 1 << n)) for _ in range(rank)]
+
+    def sel():
+        a = 0
+        for b in basis:
+            if rng.random() < 0.5:
+                a ^= b
+        return pack(a or basis[0], bool(rng.random() < 0.3))
+    s0, s1, th = [], [], []
+    for _ in range(count):
+        s0.append(sel())
+        s1.append(sel() if rng.random() < 0.5 else SEL_NONE)
+        th.append(float(rng.uniform(-math.pi, math.pi)))
+    return np.array(s0, dtype=np.uint64), np.array(s1, dtype=np.uint64), np.array(th)
+
+
+def brute_window(psi, n, s0, s1, th):
+    """The window by its definition: amplitude x times exp(i sum of theta_g
+    over the gates whose selectors hold at x), a selector holding where the
+    parity of alpha & x differs from beta.  (The oracle's kernel-level
+    diagonal rejects two-selector gates that mix the low eight and higher
+    address bits, as the reference's does; such gates are the point here.)"""
+    x = np.arange(1 << n, dtype=np.uint64)
+
+    def holds(sel):
+        a = np.uint64(int(sel) & ((1 << 63) - 1))
+        par = np.zeros(x.size, dtype=np.uint64)
+        v = x & a
+        while np.any(v):
+            par ^= v & np.uint64(1)
+            v >>= np.uint64(1)
+        return par.astype(bool) ^ bool(int(sel) >> 63)
+    ph = np.zeros(x.size)
+    for a, b, t in zip(s0, s1, th):
+        on = holds(a)
+        if int(b) != int(SEL_NONE):
+            on &= holds(b)
+        ph += np.where(on, t, 0.0)
+    return psi.astype(np.complex128) * np.exp(1j * ph)
+
+
+@pytest.mark.parametrize("prec,seg,n,targets,rank,upper", [
+    ("c128", 6, 14, [0, 2, 5, 9], 6, 64), ("c128", 6, 18, [12, 13, 14, 15, 16, 17], 10, 0),
+    ("c128", 4, 16, [1, 7, 15], 8, 64), ("c64", 3, 15, [0, 3, 8, 14], 9, 64),
+    ("c64", 6, 19, [0, 14, 15, 16, 17, 18], 10, 0), ("c128", 6, 15, [3, 14], 11, 64),
+    ("c64", 4, 16, [2, 11], 12, 64)])
+def test_fused_pass_with_a_tabulated_window(upper_knobs, prec, seg, n, targets, rank, upper):
+    """A window whose selectors are parities of many address bits (beyond the
+    phase-polynomial form) rides the tile pass as a table over its selector
+    forms (tile kernel and, forced on, the upper kernel): the oracle's U3
+    batch then window, complex128 1e-12, complex64 float rounding; windows
+    spanning more than ten dimensions are rejected."""
+    from zxh_adae_amd import ZXHError
+    A = upper_knobs
+    A.tune_set(f"tile.seg.{prec}", seg)
+    A.tune_set(f"tile.upper.{prec}", upper)
+    rng = np.random.default_rng(n * 7 + rank)
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in targets]
+    s0, s1, th = spanned_window(rng, n, 40, rank)
+    d = dev(upper_knobs, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    if rank > 10:
+        with pytest.raises(ZXHError):
+            d.fused_pass(descs, s0, s1, th)
+        return
+    want = brute_window(oracle_u3(psi, n, descs), n, s0, s1, th)
+    d.fused_pass(descs, s0, s1, th)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
 def test_fused_pass_rejects_multibit_and_overfull(gpu_engine):
     from zxh_adae_amd import ZXHError
     rng = np.random.default_rng(1)

@@ -254,8 +254,18 @@ unsigned tile_min_bits(int prec);
 // True when single-bit targets in this (circuit) order fit one staged tile
 // pass of a 2^n complex128 state: the order a pass with fans keeps.
 bool tile_staged_fits(int prec, unsigned n, const std::vector<unsigned> &targets, int geometry = -1);
-// Applies commuting gates on distinct targets (a fused U3 cluster).
-int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates);
+// A diagonal window whose selector forms span at most kWinRankMax
+// dimensions over GF(2) is a table over those forms: a tile pass can apply
+// it as its epilogue whatever its selectors (several bits, after CX
+// transport) and whether or not the pass has per-pair selectors.
+constexpr int kWinRankMax = 10;
+bool window_tabulable(unsigned nbits, const DiagGate *gates, size_t ngates);
+// Applies commuting gates on distinct targets (a fused U3 cluster), then the
+// diagonal window win (may be empty): as the last tile pass's epilogue when
+// the window is tabulable (or a phase polynomial on a pass without per-pair
+// selectors), else in a sweep of its own.
+int u3_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
+             const DiagGate *win = nullptr, size_t nwin = 0);
 // A phase applied just before a gate of a tile pass to the amplitudes whose
 // target bit is one: exp(i (c + sum_k theta[k] x_k)), theta over the state's
 // address bits (theta[target] is ignored).  A diagonal window that lies
@@ -269,8 +279,8 @@ struct GateFan
 };
 // One tile pass applying single-bit U3 gates (alpha == delta, any number of
 // targets below tile_low_bits and at most tile_upper_slots at or above),
-// then a diagonal window whose selectors have at most one alpha bit, in one
-// sweep.  Without fans the gates must commute; with fans (complex128, every
+// then a diagonal window (a phase polynomial, or tabulable and without
+// fans), in one sweep.  Without fans the gates must commute; with fans (complex128, every
 // target at or above tile_low_bits) they are applied in the given order,
 // each after its fan.
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,

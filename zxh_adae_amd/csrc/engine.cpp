@@ -331,8 +331,19 @@ int Sim::flush_diag(std::vector<DiagGate> &window)
     if (window.empty())
         return OK;
     ZXA_TRY(flush_group());
+    if (!wide_u3_.empty())
+        return flush_wide(&window);
     int rc = OK;
-    if (fusable() && window_is_polynomial(active_bits(), window.data(), window.size()))
+    bool ride = fusable() && window_is_polynomial(active_bits(), window.data(), window.size()) &&
+                window_is_polynomial(active_bits(), pass_diag_.data(), pass_diag_.size());
+    if (fusable() && !ride && !pass_has_fan_)
+    {
+        // selectors beyond the polynomial form: the pass's window as a table
+        std::vector<DiagGate> all = pass_diag_;
+        all.insert(all.end(), window.begin(), window.end());
+        ride = window_tabulable(active_bits(), all.data(), all.size());
+    }
+    if (ride)
     {
         // the window follows every U3 of the pending pass: it becomes (part
         // of) the pass's diagonal epilogue
@@ -370,8 +381,27 @@ int Sim::flush_group()
     return rc;
 }
 
+// The held-back wide cluster, with the window (which follows it in the
+// circuit, cleared here) as its last pass's epilogue when it is tabulable,
+// else in a sweep of its own.
+int Sim::flush_wide(std::vector<DiagGate> *window)
+{
+    std::vector<U3Gate> descs;
+    descs.swap(wide_u3_);
+    int rc = OK;
+    if (window && !window->empty())
+    {
+        rc = u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size(), window->data(), window->size());
+        window->clear();
+    }
+    else if (!descs.empty())
+        rc = u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size());
+    return rc;
+}
+
 int Sim::flush_pass()
 {
+    ZXA_TRY(flush_wide());
     if (pass_u3_.empty() && pass_diag_.empty())
         return OK;
     ZXA_TRY(flush_group());
@@ -395,6 +425,9 @@ int Sim::flush_pass()
 // False (nothing moved) otherwise.
 bool Sim::take_fan(unsigned t, GateFan &fan)
 {
+    // fans ride passes whose window is a phase polynomial
+    if (!window_is_polynomial(active_bits(), pass_diag_.data(), pass_diag_.size()))
+        return false;
     const uint64_t amask = (uint64_t(1) << 63) - 1;
     const uint64_t bit = uint64_t(1) << t;
     const unsigned n = active_bits();
@@ -476,8 +509,14 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
     {
         ZXA_TRY(flush_pass());
         ZXA_TRY(flush_group());
+        if (policy_ == POLICY_FUSED && used_bits_ >= tile_min_bits(prec_))
+        {
+            wide_u3_ = descs;
+            return OK;
+        }
         return u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size());
     }
+    ZXA_TRY(flush_wide());
     const unsigned low = tile_low_bits(prec_);
     const unsigned n = active_bits();
     // Complex128 (X and CX as affine updates): a gate whose target the
@@ -629,6 +668,7 @@ int Sim::execute()
     pass_high_ = 0;
     pass_has_fan_ = false;
     exp_group_.clear();
+    wide_u3_.clear();
 
     // Scheduling.  The reference policy batches each run of consecutive U3
     // gates and flushes the diagonal window when a run starts (zxh.cpp:440-470).
@@ -770,7 +810,7 @@ int Sim::execute()
                 // they commute with an expansion (both halves of each new pair
                 // get the same phase) and are applied after the group
                 const bool idle = pend_count == 0 && (stream || pre.empty()) && post.empty() && pass_u3_.empty() &&
-                                  pass_diag_.empty();
+                                  pass_diag_.empty() && wide_u3_.empty();
                 if (!idle)
                 {
                     ZXA_TRY(flush_all());
@@ -880,7 +920,11 @@ int Sim::execute()
             const uint64_t scq = local_selector(g.cq), sq = local_selector(g.q);
             if (stream && (scq == 0 || sq == 0))
                 break; // phase zero everywhere (see push_diag)
-            if (sel_class(scq) == 2 || sel_class(sq) == 2)
+            // the reference applies a CP whose selectors mix the low eight
+            // and higher address bits in a kernel of its own; the fused
+            // policy keeps it in the diagonal window (a table epilogue or a
+            // general sweep takes any selectors)
+            if (!stream && (sel_class(scq) == 2 || sel_class(sq) == 2))
             {
                 ZXA_TRY(flush_all());
                 ZXA_TRY(flush_pass());

@@ -169,6 +169,12 @@ class Sim
     int pass_high_ = 0;
     bool pass_has_fan_ = false;
     bool take_fan(unsigned t, GateFan &fan);
+    // a cluster of U3 gates with wide pair offsets or per-pair selectors
+    // (after CX transport), held back until the next flush so that a
+    // diagonal window flushed right after it can ride its last tile pass
+    // (u3_apply with the window as a table epilogue)
+    std::vector<U3Gate> wide_u3_;
+    int flush_wide(std::vector<DiagGate> *window = nullptr);
     // expansions in a row while nothing else is pending, applied at once
     // before the next physical operation
     std::vector<U3Gate> exp_group_;

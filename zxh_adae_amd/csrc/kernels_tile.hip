@@ -49,6 +49,7 @@ constexpr int kIntra = kUnitBits; // gate slot of the complex64 in-unit bit
 
 typedef double dvec2 __attribute__((ext_vector_type(2)));
 typedef float fvec4 __attribute__((ext_vector_type(4)));
+typedef float fvec2 __attribute__((ext_vector_type(2)));
 
 // Read-only tables are read through the constant address space: the loads
 // become scalar (s_load, counted by lgkmcnt) instead of uniform vector loads,
@@ -114,11 +115,27 @@ struct TPass
 // tile loop (12 matrices and 16 offsets would not fit).  Every slot of a
 // layout the pass uses holds a matrix (the identity where there is no gate):
 // the kernel applies all of them without branches.
+// A diagonal window as a table (any selectors whose linear forms span at
+// most kWinBits dimensions): with a_0 .. a_{r-1} a basis of the window's
+// selector forms over the amplitude address x, y(x) = sum_k (a_k . x) 2^k
+// and the window's phase is a function of y alone, exp(i theta) = T[y].  y
+// is linear over XOR, so an element's y is y(tile base) ^ y(thread offset)
+// ^ y(register offset) (^ yin for a complex64 unit's second amplitude).
+constexpr int kWinBits = kWinRankMax;
+struct TWin
+{
+    uint64_t a[kWinBits]; // basis forms over unit addresses (zero past the rank)
+    uint32_t yr[kRegs];   // y of each layout-0 register offset
+    uint32_t yin;         // complex64: y of the in-unit address bit
+    uint32_t rank;
+};
+
 struct TTab
 {
     uint64_t roff[kRegs]; // layout-0 register offsets (unit addresses)
     TGate g[kUnitBits + 1];
     TFan fan[kUnitBits + 1];
+    TWin win;             // table epilogue (tile passes launched with kTab)
 };
 
 // Register bits of the three layouts (unit bits); the thread bits are the
@@ -596,12 +613,76 @@ __device__ __forceinline__ void store_tile(const typename UnitT<kC64>::T (&v)[kR
         __builtin_nontemporal_store(v[r], p + (at ^ G->roff[r]));
 }
 
-// Gates, transposes and the diagonal epilogue of one tile held in v.
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false>
+// y of a unit address (see TWin).
+__device__ __forceinline__ uint32_t win_y(ZXA_CONST TWin &W, uint64_t u)
+{
+    uint32_t y = 0;
+#pragma unroll
+    for (int k = 0; k < kWinBits; k++)
+        y |= uint32_t(__popcll(u & W.a[k]) & 1) << k;
+    return y;
+}
+
+// Table entries as the kernels keep them in LDS: complex128 in double,
+// complex64 in float (the precision of the amplitudes they multiply).
+template <bool kC64>
+using WinT = std::conditional_t<kC64, fvec2, dvec2>;
+
+// Copies the window's table (2^rank entries of (cos, sin) in double) into
+// LDS; the caller synchronizes before the first use.
+template <bool kC64>
+__device__ __forceinline__ void win_load(WinT<kC64> *wt, const double *__restrict__ tab, ZXA_CONST TWin &W, int t)
+{
+    const int nent = 1 << W.rank;
+    for (int e = t; e < nent; e += kThreads)
+    {
+        if constexpr (kC64)
+            wt[e] = fvec2{float(tab[2 * e]), float(tab[2 * e + 1])};
+        else
+            wt[e] = dvec2{tab[2 * e], tab[2 * e + 1]};
+    }
+}
+
+// The window's phases on a tile in layout 0; y0 = y(base) ^ y(thread).
+template <bool kC64>
+__device__ __forceinline__ void win_apply(typename UnitT<kC64>::T (&v)[kRegs], const WinT<kC64> *wt,
+                                          ZXA_CONST TWin &W, uint32_t y0)
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+    {
+        const uint32_t y = y0 ^ W.yr[r];
+        if constexpr (kC64)
+        {
+            const fvec2 p0 = wt[y], p1 = wt[y ^ W.yin];
+            fvec4 &u = v[r];
+            const float ax = u.x * p0.x - u.y * p0.y, ay = u.x * p0.y + u.y * p0.x;
+            const float bx = u.z * p1.x - u.w * p1.y, by = u.z * p1.y + u.w * p1.x;
+            u.x = ax;
+            u.y = ay;
+            u.z = bx;
+            u.w = by;
+        }
+        else
+        {
+            const dvec2 p = wt[y];
+            dvec2 &u = v[r];
+            const double nx = u.x * p.x - u.y * p.y, ny = u.x * p.y + u.y * p.x;
+            u.x = nx;
+            u.y = ny;
+        }
+    }
+}
+
+// Gates, transposes and the diagonal epilogue of one tile held in v
+// (kDiag: the phase-polynomial epilogue; kTab: the window table wt, with
+// yt the thread offset's y).
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false, bool kTab = false>
 __device__ __forceinline__ void tile_work(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds,
                                           const TPass &P, ZXA_CONST TTab *G, const double *tphase,
                                           const Epi<kC64> &epi, int t, uint64_t toff, uint64_t tile, uint64_t base,
-                                          const dvec2 *fl = nullptr)
+                                          const dvec2 *fl = nullptr, const WinT<kC64> *wt = nullptr,
+                                          uint32_t yt = 0)
 {
     ZXA_CONST double *fp = nullptr;
     if constexpr (kFan)
@@ -630,6 +711,8 @@ __device__ __forceinline__ void tile_work(typename UnitT<kC64>::T (&v)[kRegs], t
     }
     if constexpr (kDiag)
         epi.apply(v, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), t);
+    if constexpr (kTab)
+        win_apply<kC64>(v, wt, G->win, win_y(G->win, base) ^ yt);
 }
 
 // The tables are re-read at every tile (see TTab): the pointer is laundered
@@ -641,7 +724,7 @@ __device__ __forceinline__ ZXA_CONST TTab *tables(const TTab *tabp)
 }
 
 // tabp: register offsets and gate matrices; ttab: per-thread ST and G
-// (kDiag only); tphase: per-tile E and D (kDiag only).
+// (kDiag), or the window table (kTab); tphase: per-tile E and D (kDiag only).
 //
 // Software pipeline over the workgroup's tiles t0, t1 = t0 + grid, ... with
 // two register buffers: while tile k is worked on, tile k + 1 is in flight,
@@ -649,15 +732,17 @@ __device__ __forceinline__ ZXA_CONST TTab *tables(const TTab *tabp)
 // that waiting for a tile's loads never waits for the previous stores.  The
 // prefetch is unconditional (past the last tile it re-reads the current one):
 // with no branch around it the compiler's wait counts stay exact.
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false>
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false, bool kTab = false>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void tile_kernel(
     void *__restrict__ state, const TPass P, const TTab *__restrict__ tabp, const double *__restrict__ ttab,
     const double *__restrict__ tphase)
 {
+    static_assert(!(kTab && (kDiag || kFan)), "one epilogue kind");
     using T = typename UnitT<kC64>::T;
     __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
     // fan passes: every slot's thread-nibble factors (TFan::nib)
     __shared__ dvec2 fl[kFan ? (kUnitBits + 1) * 32 : 1];
+    __shared__ WinT<kC64> wt[kTab ? (1 << kWinBits) : 1];
     const int t = threadIdx.x;
     T *p = static_cast<T *>(state);
     const uint64_t toff = thr_off<0>(t, P);
@@ -671,6 +756,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
             fl[e] = dvec2{tabp->fan[e >> 5].nib[e & 31][0], tabp->fan[e >> 5].nib[e & 31][1]};
         __syncthreads();
     }
+    uint32_t yt = 0;
+    if constexpr (kTab)
+    {
+        ZXA_CONST TTab *G0 = tables(tabp);
+        win_load<kC64>(wt, ttab, G0->win, t);
+        yt = win_y(G0->win, toff);
+        __syncthreads();
+    }
 
     T a[kRegs], b[kRegs];
     uint64_t t0 = blockIdx.x;
@@ -681,7 +774,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     bool m1 = t1 < P.ntiles;
     uint64_t b1 = next_base(b0, P);
     load_tile<kC64>(b, p, (m1 ? b1 : b0) ^ toff, G);
-    tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t0, b0, fl);
+    tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan, kTab>(a, lds, P, G, tphase, epi, t, toff, t0, b0, fl, wt, yt);
     for (;;)
     {
         G = tables(tabp);
@@ -692,7 +785,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const bool m2 = t2 < P.ntiles;
         const uint64_t b2 = next_base(b1, P);
         load_tile<kC64>(a, p, (m2 ? b2 : b1) ^ toff, G);
-        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(b, lds, P, G, tphase, epi, t, toff, t1, b1, fl);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan, kTab>(b, lds, P, G, tphase, epi, t, toff, t1, b1, fl, wt, yt);
 
         G = tables(tabp);
         store_tile<kC64>(b, p, b1 ^ toff, G);
@@ -702,7 +795,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         const bool m3 = t3 < P.ntiles;
         const uint64_t b3 = next_base(b2, P);
         load_tile<kC64>(b, p, (m3 ? b3 : b2) ^ toff, G);
-        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan>(a, lds, P, G, tphase, epi, t, toff, t2, b2, fl);
+        tile_work<kC64, kL1, kL2, kDiag, kFlip, kFan, kTab>(a, lds, P, G, tphase, epi, t, toff, t2, b2, fl, wt, yt);
 
         t0 = t2;
         b0 = b2;
@@ -843,14 +936,16 @@ __device__ __forceinline__ int upper_thread(int t)
 // kernel, which agree to float rounding.  kFlip: per-pair
 // selectors; the rpar masks of slots 6 and 7 are then those of the swapped
 // registers (unit bits 6, 7, 10, 11), set by the host.
-template <bool kC64, bool kDiag, bool kFlip>
+template <bool kC64, bool kDiag, bool kFlip, bool kTab = false>
 __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ state, const TPass P,
                                                          const TTab *__restrict__ tabp,
                                                          const double *__restrict__ ttab,
                                                          const double *__restrict__ tphase)
 {
-    static_assert(!(kDiag && kFlip), "no diagonal epilogue with per-pair selectors");
+    static_assert(!(kDiag && kFlip), "no phase-polynomial epilogue with per-pair selectors");
+    static_assert(!(kDiag && kTab), "one epilogue kind");
     using T = typename UnitT<kC64>::T;
+    __shared__ WinT<kC64> wt[kTab ? (1 << kWinBits) : 1];
     const int tl = upper_thread(threadIdx.x);
     T *p = static_cast<T *>(state);
     const uint64_t tile = blockIdx.x;
@@ -860,6 +955,8 @@ __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ stat
     ZXA_CONST TTab *G = (ZXA_CONST TTab *)tabp;
     T v[kRegs];
     load_tile<kC64>(v, p, at, G);
+    if constexpr (kTab)
+        win_load<kC64>(wt, ttab, G->win, threadIdx.x); // synchronized before its use below
     // VALU arithmetic here: with the matrix cores the compiler moves the
     // results through AGPRs and the occupancy this kernel relies on halves
     apply_layout<kC64, kFlip, 0, false, false>(v, P, G, base, toff);
@@ -890,6 +987,11 @@ __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ stat
         Epi<kC64> epi;
         epi.init(ttab, tl);
         epi.apply(v, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), tl);
+    }
+    if constexpr (kTab)
+    {
+        __syncthreads();
+        win_apply<kC64>(v, wt, G->win, win_y(G->win, base) ^ win_y(G->win, toff));
     }
     store_tile<kC64>(v, p, at, G);
 }
@@ -1344,13 +1446,14 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t tiles)
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
 }
 
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false>
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false, bool kTab = false>
 int launch_t(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const double *ttab, const double *tphase)
 {
-    // register buffers per workgroup (knob tile.bufs.<prec>); the flip and
-    // fan variants keep two (their tables take the registers of a third)
-    auto k = tile_kernel<kC64, kL1, kL2, kDiag, kFlip, kFan>;
-    if constexpr (!kFlip && !kFan)
+    // register buffers per workgroup (knob tile.bufs.<prec>); the flip, fan
+    // and window-table variants keep two (their tables take the registers of
+    // a third)
+    auto k = tile_kernel<kC64, kL1, kL2, kDiag, kFlip, kFan, kTab>;
+    if constexpr (!kFlip && !kFan && !kTab)
         if (tune_get(kC64 ? "tile.bufs.c64" : "tile.bufs.c128") == 3)
             k = tile_kernel3<kC64, kL1, kL2, kDiag, false>;
     const int grid = persistent_grid(ctx, k, P0.ntiles);
@@ -1360,24 +1463,36 @@ int launch_t(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const doub
     return hip_check(hipGetLastError(), "tile_kernel launch");
 }
 
-template <bool kC64, bool kDiag, bool kFlip, bool kFan = false>
+template <bool kC64, bool kDiag, bool kFlip, bool kFan = false, bool kTab = false>
 int launch_l(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, const TTab *tab, const double *ttab,
              const double *tphase)
 {
     if (l1 && l2)
-        return launch_t<kC64, true, true, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
+        return launch_t<kC64, true, true, kDiag, kFlip, kFan, kTab>(ctx, state, P, tab, ttab, tphase);
     if (l1)
-        return launch_t<kC64, true, false, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
+        return launch_t<kC64, true, false, kDiag, kFlip, kFan, kTab>(ctx, state, P, tab, ttab, tphase);
     if (l2)
-        return launch_t<kC64, false, true, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
-    return launch_t<kC64, false, false, kDiag, kFlip, kFan>(ctx, state, P, tab, ttab, tphase);
+        return launch_t<kC64, false, true, kDiag, kFlip, kFan, kTab>(ctx, state, P, tab, ttab, tphase);
+    return launch_t<kC64, false, false, kDiag, kFlip, kFan, kTab>(ctx, state, P, tab, ttab, tphase);
 }
 
-// fan: complex128 passes without flips only (P.fphase set)
+// fan: complex128 passes without flips only (P.fphase set); win: ttab is a
+// window table (TTab::win), with or without flips
 template <bool kC64>
 int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip, const TTab *tab,
-             const double *ttab, const double *tphase, bool fan = false)
+             const double *ttab, const double *tphase, bool fan = false, bool win = false)
 {
+    if (win)
+    {
+        if (fan)
+        {
+            set_error("tile pass: fans with a window table");
+            return EINVAL_;
+        }
+        if (flip)
+            return launch_l<kC64, false, true, false, true>(ctx, state, P, l1, l2, tab, ttab, nullptr);
+        return launch_l<kC64, false, false, false, true>(ctx, state, P, l1, l2, tab, ttab, nullptr);
+    }
     if constexpr (!kC64)
         if (fan)
         {
@@ -1406,10 +1521,16 @@ int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip,
 // epilogue; flip: per-pair selectors).
 template <bool kC64>
 void launch_u_prec(Ctx &ctx, void *state, const TPass &P, const TTab *tab, const double *ttab,
-                   const double *tphase, bool flip)
+                   const double *tphase, bool flip, bool win)
 {
     const dim3 grid(unsigned(P.ntiles));
-    if (ttab)
+    if (win && flip)
+        hipLaunchKernelGGL((upper_kernel<kC64, false, true, true>), grid, dim3(kThreads), 0, ctx.stream, state, P,
+                           tab, ttab, nullptr);
+    else if (win)
+        hipLaunchKernelGGL((upper_kernel<kC64, false, false, true>), grid, dim3(kThreads), 0, ctx.stream, state, P,
+                           tab, ttab, nullptr);
+    else if (ttab)
         hipLaunchKernelGGL((upper_kernel<kC64, true, false>), grid, dim3(kThreads), 0, ctx.stream, state, P, tab,
                            ttab, tphase);
     else if (flip)
@@ -1421,17 +1542,17 @@ void launch_u_prec(Ctx &ctx, void *state, const TPass &P, const TTab *tab, const
 }
 
 int launch_u(Ctx &ctx, bool c64, void *state, const TPass &P, const TTab *tab, const double *ttab,
-             const double *tphase, bool flip)
+             const double *tphase, bool flip, bool win)
 {
-    if (flip && ttab)
+    if (flip && ttab && !win)
     {
-        set_error("tile pass: diagonal epilogue with per-pair selectors");
+        set_error("tile pass: phase-polynomial epilogue with per-pair selectors");
         return EINVAL_;
     }
     if (c64)
-        launch_u_prec<true>(ctx, state, P, tab, ttab, tphase, flip);
+        launch_u_prec<true>(ctx, state, P, tab, ttab, tphase, flip, win);
     else
-        launch_u_prec<false>(ctx, state, P, tab, ttab, tphase, flip);
+        launch_u_prec<false>(ctx, state, P, tab, ttab, tphase, flip, win);
     return hip_check(hipGetLastError(), "upper_kernel launch");
 }
 
@@ -1811,6 +1932,83 @@ int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const Ga
     return OK;
 }
 
+// A window's selector forms reduced to a basis (see TWin): fully reduced
+// rows, row k the only one with bit piv[k] set, so a form's coordinate k is
+// its bit piv[k].  False when the forms span more than kWinBits dimensions
+// or reach above the state.
+struct WinBasis
+{
+    int rank = 0;
+    uint64_t a[kWinBits] = {};
+    int piv[kWinBits] = {};
+};
+
+bool win_basis(unsigned n, const DiagGate *gates, size_t ngates, WinBasis &B)
+{
+    const uint64_t amask = (uint64_t(1) << 63) - 1;
+    B = WinBasis();
+    for (size_t i = 0; i < ngates; i++)
+        for (int k = 0; k < 2; k++)
+        {
+            const uint64_t s = k == 0 ? gates[i].s0 : gates[i].s1;
+            if (k == 1 && s == kNoSel)
+                continue;
+            uint64_t v = s & amask;
+            if (n < 64 && (v >> n) != 0)
+                return false;
+            for (int r = 0; r < B.rank; r++)
+                if ((v >> B.piv[r]) & 1)
+                    v ^= B.a[r];
+            if (v == 0)
+                continue;
+            if (B.rank == kWinBits)
+                return false;
+            const int p = 63 - __builtin_clzll(v);
+            for (int r = 0; r < B.rank; r++)
+                if ((B.a[r] >> p) & 1)
+                    B.a[r] ^= v;
+            B.a[B.rank] = v;
+            B.piv[B.rank] = p;
+            B.rank++;
+        }
+    return true;
+}
+
+// The window's phase table over y (2^rank entries of (cos, sin)): theta(y)
+// sums each gate's angle where its selectors hold, in gate order.
+void win_table(const WinBasis &B, const DiagGate *gates, size_t ngates, std::vector<double> &tab)
+{
+    const uint64_t amask = (uint64_t(1) << 63) - 1;
+    auto coord = [&](uint64_t s) {
+        uint32_t c = 0;
+        for (int r = 0; r < B.rank; r++)
+            c |= uint32_t(((s & amask) >> B.piv[r]) & 1) << r;
+        return c;
+    };
+    std::vector<uint32_t> c0(ngates), c1(ngates);
+    for (size_t i = 0; i < ngates; i++)
+    {
+        c0[i] = coord(gates[i].s0);
+        c1[i] = gates[i].s1 == kNoSel ? 0 : coord(gates[i].s1);
+    }
+    const uint32_t nent = uint32_t(1) << B.rank;
+    tab.assign(2 * size_t(nent), 0.0);
+    for (uint32_t y = 0; y < nent; y++)
+    {
+        double th = 0.0;
+        for (size_t i = 0; i < ngates; i++)
+        {
+            const bool on0 = ((__builtin_popcount(c0[i] & y) & 1) ^ int(gates[i].s0 >> 63)) != 0;
+            const bool on1 = gates[i].s1 == kNoSel ||
+                             ((__builtin_popcount(c1[i] & y) & 1) ^ int(gates[i].s1 >> 63)) != 0;
+            if (on0 && on1)
+                th += gates[i].theta;
+        }
+        tab[2 * size_t(y)] = std::cos(th);
+        tab[2 * size_t(y) + 1] = std::sin(th);
+    }
+}
+
 // One tile pass over the given targets, then the diagonal window (if any).
 int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
                   const DiagGate *diag = nullptr, size_t ndiag = 0, int geometry = -1,
@@ -1961,14 +2159,35 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     }
     const double *ttab = nullptr;
     const double *tphase = nullptr;
+    // the window: a phase polynomial on a pass without per-pair selectors
+    // (and with every fan), else a table over its selector forms
+    bool win = false;
+    std::vector<double> wtab;
+    std::memset(&G.win, 0, sizeof(G.win));
     if (ndiag > 0)
     {
-        if (flip)
+        if (!flip && !wide && (any_fan || window_is_polynomial(n, diag, ndiag)))
         {
-            set_error("tile pass: diagonal epilogue with per-pair selectors");
-            return EINVAL_;
+            ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase));
         }
-        ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase));
+        else
+        {
+            WinBasis B;
+            if (any_fan || !win_basis(n, diag, ndiag, B))
+            {
+                set_error("tile pass: the diagonal window is neither a phase polynomial nor tabulable");
+                return EINVAL_;
+            }
+            win_table(B, diag, ndiag, wtab);
+            win = true;
+            G.win.rank = uint32_t(B.rank);
+            for (int k = 0; k < B.rank; k++)
+            {
+                G.win.a[k] = B.a[k] >> shift;
+                if (c64)
+                    G.win.yin |= uint32_t(B.a[k] & 1u) << k;
+            }
+        }
     }
     if (any_fan)
         ZXA_TRY(setup_fans(ctx, n, gs, fans, pl, P, G));
@@ -2009,17 +2228,31 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
             }
         }
     }
+    if (win)
+    {
+        // y of every register offset, over the final (filler-cleaned) offsets
+        for (int r = 0; r < kRegs; r++)
+        {
+            uint32_t y = 0;
+            for (int k = 0; k < kWinBits; k++)
+                y |= uint32_t(__builtin_popcountll(G.roff[r] & G.win.a[k]) & 1) << k;
+            G.win.yr[r] = y;
+        }
+        void *dw = nullptr;
+        ZXA_TRY(ctx.ring.stage(wtab.data(), wtab.size() * sizeof(double), ctx.stream, &dw));
+        ttab = static_cast<const double *>(dw);
+    }
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     const TTab *tab = static_cast<const TTab *>(dtab);
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
     if (upper)
-        ZXA_TRY(launch_u(ctx, c64, state, P, tab, ttab, tphase, flip));
+        ZXA_TRY(launch_u(ctx, c64, state, P, tab, ttab, tphase, flip, win));
     else if (c64)
-        ZXA_TRY(launch_p<true>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase));
+        ZXA_TRY(launch_p<true>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase, false, win));
     else
-        ZXA_TRY(launch_p<false>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase, any_fan));
+        ZXA_TRY(launch_p<false>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase, any_fan, win));
     ZXA_TRY(ctx.toc(ndiag > 0 || any_fan ? F_FUSED : F_U3TILE, bytes));
     if (upper && ctx.timing)
         ctx.fam[F_UPPER].launches++;
@@ -2502,15 +2735,26 @@ bool plan_passes(int prec, int geometry, std::vector<const U3Gate *> rest,
     return true;
 }
 
-int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates)
+bool window_tabulable(unsigned n, const DiagGate *gates, size_t ngates)
+{
+    WinBasis B;
+    return n <= 40 && win_basis(n, gates, ngates, B);
+}
+
+int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates, const DiagGate *win,
+             size_t nwin)
 {
     const bool c64 = prec == C64;
+    // the window rides the last tile pass when it is tabulable; else (or
+    // when no tile pass runs) it gets a sweep of its own
+    const bool ride = nwin > 0 && window_tabulable(n, win, nwin);
+    auto rest = [&]() { return nwin > 0 ? diag_apply(ctx, state, prec, n, win, nwin) : OK; };
     if (!independent_batch(gates, ngates))
     {
         // gates that do not commute: one sweep each, in order
         for (size_t i = 0; i < ngates; i++)
             ZXA_TRY(u3_pair_apply(ctx, state, prec, n, gates[i]));
-        return OK;
+        return rest();
     }
     std::vector<const U3Gate *> single, wide;
     for (size_t i = 0; i < ngates; i++)
@@ -2526,7 +2770,7 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
             ZXA_TRY(u3_pair_apply(ctx, state, prec, n, g));
     }
     if (single.empty() && wide.empty())
-        return OK;
+        return rest();
     std::sort(single.begin(), single.end(), [](const U3Gate *a, const U3Gate *b) { return a->delta < b->delta; });
     std::vector<const U3Gate *> order = single;
     order.insert(order.end(), wide.begin(), wide.end());
@@ -2548,9 +2792,13 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
         set_error("u3_apply: dependent pair offsets in one batch");
         return EINVAL_;
     }
-    for (const std::vector<SingleGate> &gs : best)
-        ZXA_TRY(run_tile_pass(ctx, state, prec, n, gs, nullptr, 0, geo));
-    return OK;
+    for (size_t k = 0; k < best.size(); k++)
+    {
+        const bool last = k + 1 == best.size();
+        ZXA_TRY(run_tile_pass(ctx, state, prec, n, best[k], last && ride ? win : nullptr, last && ride ? nwin : 0,
+                              geo));
+    }
+    return ride ? OK : rest();
 }
 
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates,
@@ -2561,9 +2809,13 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
         set_error("fused pass: state narrower than a tile");
         return EINVAL_;
     }
-    if (ndiag > 0 && !window_is_polynomial(n, diag, ndiag))
+    bool any_fan = false;
+    for (size_t i = 0; fans && i < ngates; i++)
+        any_fan = any_fan || fans[i].on;
+    if (ndiag > 0 && !window_is_polynomial(n, diag, ndiag) && (any_fan || !window_tabulable(n, diag, ndiag)))
     {
-        set_error("fused pass: diagonal window is not a phase polynomial");
+        set_error(any_fan ? "fused pass: a pass with fans needs a phase-polynomial window"
+                          : "fused pass: diagonal window is neither a phase polynomial nor tabulable");
         return EINVAL_;
     }
     const unsigned low = tile_low_bits(prec, geometry);
