@@ -185,6 +185,16 @@ int zxh_k_u3_batch(void *state, int precision, size_t local_bits, const zxh_u3_d
  * counterpart: the reference runs these as separate sweeps) */
 int zxh_k_fused_pass(void *state, int precision, size_t local_bits, const zxh_u3_desc *descs, size_t ndesc,
                      const uint64_t *sel0, const uint64_t *sel1, const double *theta, size_t ndiag);
+/* The table a fused tile pass applies for a diagonal window beyond the
+ * phase-polynomial form (host only; needs no device and no init).  The
+ * window's selector forms alpha (sel & ~(1 << 63)) are reduced to a basis
+ * a_0 .. a_{r-1} over GF(2), r <= 10, written to basis[0 .. 9] (zero past r)
+ * and *rank; with y(x) = sum_k parity(a_k & x) << k the window's phase at
+ * amplitude address x is table[2 y] + i table[2 y + 1] (2^r entries; the
+ * caller provides 2048 doubles).  ZXH_EINVAL when the forms span more than
+ * 10 dimensions or reach above nbits.  No reference counterpart. */
+int zxh_window_table(size_t nbits, const uint64_t *sel0, const uint64_t *sel1, const double *theta, size_t ngates,
+                     uint64_t *basis, int *rank, double *table);
 /* diag_pending_push_p / try_push_cp / flush (kernels.h:36-41) as one call:
  * ngates entries, sel1[i] == ~0 marks a P gate (Z/Rz already lowered to P). */
 int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,

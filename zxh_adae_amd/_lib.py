@@ -74,6 +74,7 @@ _SIGS = [
     ("zxh_k_diag", C.c_int, [_vp, C.c_int, _sz, _u64p, _u64p, _dp, _sz]),
     ("zxh_k_diag_noalloc", C.c_int, [_vp, C.c_int, _sz, _u64p, _u64p, _dp, _sz]),
     ("zxh_k_fused_pass", C.c_int, [_vp, C.c_int, _sz, _vp, _sz, _u64p, _u64p, _dp, _sz]),
+    ("zxh_window_table", C.c_int, [_sz, _u64p, _u64p, _dp, _sz, _u64p, C.POINTER(C.c_int), _dp]),
     ("zxh_k_x", C.c_int, [_vp, C.c_int, _sz, _u64, _u64]),
     ("zxh_k_cx", C.c_int, [_vp, C.c_int, _sz, _u64, _u64, _u64]),
     ("zxh_k_cp", C.c_int, [_vp, C.c_int, _sz, _u64, _u64, C.c_double]),

@@ -471,6 +471,24 @@ int zxh_k_fused_pass(void *state, int precision, size_t local_bits, const zxh_u3
                                            reinterpret_cast<const zxa::U3Gate *>(descs), ndesc, g.data(), ndiag));
 }
 
+int zxh_window_table(size_t nbits, const uint64_t *sel0, const uint64_t *sel1, const double *theta, size_t ngates,
+                     uint64_t *basis, int *rank, double *table)
+{
+    if ((ngates && (!sel0 || !sel1 || !theta)) || !basis || !rank || !table)
+        return fail(ZXH_EINVAL, "null argument");
+    if (nbits > 40)
+        return fail(ZXH_EINVAL, "window table: state wider than 40 bits");
+    std::vector<zxa::DiagGate> g(ngates);
+    for (size_t i = 0; i < ngates; i++)
+        g[i] = {sel0[i], sel1[i], theta[i]};
+    std::vector<double> tab;
+    if (!zxa::window_table_host(unsigned(nbits), g.data(), ngates, basis, rank, tab))
+        return fail(ZXH_EINVAL, "window table: the selector forms span more than 10 dimensions or reach above the "
+                                "state");
+    std::memcpy(table, tab.data(), tab.size() * sizeof(double));
+    return ZXH_OK;
+}
+
 int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
                const double *theta, size_t ngates)
 {

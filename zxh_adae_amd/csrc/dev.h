@@ -260,6 +260,11 @@ bool tile_staged_fits(int prec, unsigned n, const std::vector<unsigned> &targets
 // transport) and whether or not the pass has per-pair selectors.
 constexpr int kWinRankMax = 10;
 bool window_tabulable(unsigned nbits, const DiagGate *gates, size_t ngates);
+// The table a tile pass would use (host only): the basis forms (rank of
+// them, over amplitude addresses) and 2^rank (cos, sin) entries.  False when
+// the window is not tabulable.
+bool window_table_host(unsigned nbits, const DiagGate *gates, size_t ngates, uint64_t *basis, int *rank,
+                       std::vector<double> &table);
 // Applies commuting gates on distinct targets (a fused U3 cluster), then the
 // diagonal window win (may be empty): as the last tile pass's epilogue when
 // the window is tabulable (or a phase polynomial on a pass without per-pair

@@ -2741,6 +2741,19 @@ bool window_tabulable(unsigned n, const DiagGate *gates, size_t ngates)
     return n <= 40 && win_basis(n, gates, ngates, B);
 }
 
+bool window_table_host(unsigned n, const DiagGate *gates, size_t ngates, uint64_t *basis, int *rank,
+                       std::vector<double> &table)
+{
+    WinBasis B;
+    if (n > 40 || !win_basis(n, gates, ngates, B))
+        return false;
+    win_table(B, gates, ngates, table);
+    for (int k = 0; k < kWinBits; k++)
+        basis[k] = k < B.rank ? B.a[k] : 0;
+    *rank = B.rank;
+    return true;
+}
+
 int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates, const DiagGate *win,
              size_t nwin)
 {
