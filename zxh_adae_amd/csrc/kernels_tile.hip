@@ -1369,7 +1369,7 @@ __device__ __forceinline__ cd cis(double a)
     return {c, s};
 }
 
-// Per-tile phase factors of the diagonal epilogue, one thread per tile.
+// Per-tile phase factors of the diagonal epilogue.
 // Tile index bit i is address bit outer[i].  tab (nE tile element bits, nh
 // outer bits, nR register elements):
 //     cq[k * nh + i] = Q(tile bit k, outer i)   k < nE
@@ -1379,32 +1379,20 @@ __device__ __forceinline__ cd cis(double a)
 //     map[8..8+nb)   = tile element index of register element bit b
 // Output per tile: E_j = exp(i c(thr j)) (8), D[rho] = exp(i (C + rs[rho] +
 // sum_{b in rho} c(reg b))) (nR), as complex pairs.
-__global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__restrict__ tab, int nE, int nh, int nb,
-                                                              double K, uint64_t ntiles, double *__restrict__ out)
+
+// c_k of a tile: the k-th element coefficient summed over the tile's bits.
+__device__ __forceinline__ double tp_coef(const double *cq, int nh, uint64_t tile, int k)
 {
-    extern __shared__ double sh[];
-    const int nR = 1 << nb;
-    const int ntab = nE * nh + nh + nh * nh + nR + 8 + nb;
-    for (int i = threadIdx.x; i < ntab; i += blockDim.x)
-        sh[i] = tab[i];
-    __syncthreads();
-    const uint64_t tile = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    if (tile >= ntiles)
-        return;
-    const double *cq = sh;
-    const double *hL = cq + nE * nh;
-    const double *hQ = hL + nh;
-    const double *rs = hQ + nh * nh;
-    const double *mp = rs + nR;
-    double c[13];
-    for (int k = 0; k < nE; k++)
-    {
-        double s = 0.0;
-        for (int i = 0; i < nh; i++)
-            if ((tile >> i) & 1u)
-                s += cq[k * nh + i];
-        c[k] = s;
-    }
+    double s = 0.0;
+    for (int i = 0; i < nh; i++)
+        if ((tile >> i) & 1u)
+            s += cq[k * nh + i];
+    return s;
+}
+
+// The tile's constant: K plus its linear and quadratic terms.
+__device__ __forceinline__ double tp_const(const double *hL, const double *hQ, int nh, uint64_t tile, double K)
+{
     double C = K;
     for (int i = 0; i < nh; i++)
     {
@@ -1416,7 +1404,60 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
                 s += hQ[i * nh + l];
         C += s;
     }
-    const int stride = 2 * (8 + nR);
+    return C;
+}
+
+// Per-tile phase factors of the diagonal epilogue: 8 E factors and 2^nb D
+// factors per tile.  spread = 0: one thread per tile; spread = 1: one thread
+// per (tile, factor), for states of few tiles (the same sums in the same
+// order, so the same factors).
+__global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__restrict__ tab, int nE, int nh, int nb,
+                                                              double K, uint64_t ntiles, double *__restrict__ out,
+                                                              int spread)
+{
+    extern __shared__ double sh[];
+    const int nR = 1 << nb;
+    const int ntab = nE * nh + nh + nh * nh + nR + 8 + nb;
+    for (int i = threadIdx.x; i < ntab; i += blockDim.x)
+        sh[i] = tab[i];
+    __syncthreads();
+    const double *cq = sh;
+    const double *hL = cq + nE * nh;
+    const double *hQ = hL + nh;
+    const double *rs = hQ + nh * nh;
+    const double *mp = rs + nR;
+    const int per = 8 + nR;
+    const int stride = 2 * per;
+    const uint64_t id = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    if (spread)
+    {
+        if (id >= ntiles * uint64_t(per))
+            return;
+        const uint64_t tile = id / uint64_t(per);
+        const int j = int(id % uint64_t(per));
+        double a;
+        if (j < 8)
+            a = tp_coef(cq, nh, tile, static_cast<int>(mp[j]));
+        else
+        {
+            const int rho = j - 8;
+            a = tp_const(hL, hQ, nh, tile, K) + rs[rho];
+            for (int b = 0; b < nb; b++)
+                if ((rho >> b) & 1)
+                    a += tp_coef(cq, nh, tile, static_cast<int>(mp[8 + b]));
+        }
+        const cd e = cis(a);
+        out[tile * stride + 2 * j] = e.x;
+        out[tile * stride + 2 * j + 1] = e.y;
+        return;
+    }
+    const uint64_t tile = id;
+    if (tile >= ntiles)
+        return;
+    double c[13];
+    for (int k = 0; k < nE; k++)
+        c[k] = tp_coef(cq, nh, tile, k);
+    const double C = tp_const(hL, hQ, nh, tile, K);
     double *o = out + tile * stride;
     for (int j = 0; j < 8; j++)
     {
@@ -1434,6 +1475,19 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
         o[16 + 2 * rho] = d.x;
         o[16 + 2 * rho + 1] = d.y;
     }
+}
+
+// Launches tile_phase_kernel, spread over (tile, factor) pairs when the state
+// has few tiles (below 4096 one thread per tile leaves most CUs idle and each
+// thread runs 8 + 2^nb sincos in a row).
+void launch_tile_phase(hipStream_t stream, const double *tab, size_t tab_doubles, int nE, int nh, int nb, double K,
+                       uint64_t ntiles, double *out)
+{
+    const int spread = ntiles < 4096 ? 1 : 0;
+    const uint64_t threads = spread ? ntiles * uint64_t(8 + (1 << nb)) : ntiles;
+    const unsigned grid = unsigned((threads + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), tab_doubles * sizeof(double), stream, tab, nE,
+                       nh, nb, K, ntiles, out, spread);
 }
 
 template <typename K>
@@ -1805,10 +1859,8 @@ int setup_epilogue(Ctx &ctx, int prec, unsigned n, const std::vector<unsigned> &
     ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
     const size_t stride = 2 * (8 + (size_t(1) << nb));
     ZXA_TRY(ctx.scratch_a.ensure(ntiles * stride * sizeof(double)));
-    const unsigned grid = unsigned((ntiles + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), pt.size() * sizeof(double), ctx.stream,
-                       static_cast<const double *>(dpt), int(pos.size()), nh, nb, poly.K, ntiles,
-                       static_cast<double *>(ctx.scratch_a.ptr));
+    launch_tile_phase(ctx.stream, static_cast<const double *>(dpt), pt.size(), int(pos.size()), nh, nb, poly.K, ntiles,
+                      static_cast<double *>(ctx.scratch_a.ptr));
     ZXA_HIP(hipGetLastError());
     *ttab = static_cast<const double *>(dtt);
     *tphase = static_cast<const double *>(ctx.scratch_a.ptr);
@@ -1850,13 +1902,17 @@ void fill_pass(int prec, unsigned n, const Plan &pl, TPass &P, TTab &G, bool use
 // Per-tile factors of the fans: out[tile * kFanStride + 2 s] = exp(i (c_s +
 // sum_i theta_s[outer i] * tile bit i)), one thread per tile; tab holds, per
 // slot s < kUnitBits + 1, c_s then the nh outer coefficients.
+// spread = 1: one thread per (tile, slot) instead of one per tile (few tiles).
 __global__ __launch_bounds__(kThreads) void fan_phase_kernel(const double *__restrict__ tab, int nh, uint64_t ntiles,
-                                                             double *__restrict__ out)
+                                                             double *__restrict__ out, int spread)
 {
-    const uint64_t tile = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t id = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+    const uint64_t tile = spread ? id / uint64_t(kUnitBits + 1) : id;
     if (tile >= ntiles)
         return;
-    for (int sl = 0; sl < kUnitBits + 1; sl++)
+    const int first = spread ? int(id % uint64_t(kUnitBits + 1)) : 0;
+    const int last = spread ? first + 1 : kUnitBits + 1;
+    for (int sl = first; sl < last; sl++)
     {
         const double *row = tab + size_t(sl) * (1 + nh);
         double ph = row[0];
@@ -1924,9 +1980,11 @@ int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const Ga
     void *dt = nullptr;
     ZXA_TRY(ctx.ring.stage(tab.data(), tab.size() * sizeof(double), ctx.stream, &dt));
     ZXA_TRY(ctx.scratch_g.ensure(P.ntiles * kFanStride * sizeof(double)));
-    const unsigned grid = unsigned((P.ntiles + kThreads - 1) / kThreads);
+    const int spread = P.ntiles < 4096 ? 1 : 0;
+    const uint64_t threads = spread ? P.ntiles * uint64_t(kUnitBits + 1) : P.ntiles;
+    const unsigned grid = unsigned((threads + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(fan_phase_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<const double *>(dt),
-                       nh, P.ntiles, static_cast<double *>(ctx.scratch_g.ptr));
+                       nh, P.ntiles, static_cast<double *>(ctx.scratch_g.ptr), spread);
     ZXA_HIP(hipGetLastError());
     P.fphase = static_cast<const double *>(ctx.scratch_g.ptr);
     return OK;
@@ -3122,10 +3180,8 @@ int batch_prepare(Ctx &ctx, int prec, bool adj, const std::vector<BatchPassSpec>
         double *dpt = nullptr;
         ZXA_TRY(dmalloc(&dpt, pt.size() * sizeof(double)));
         ZXA_HIP(hipMemcpy(dpt, pt.data(), pt.size() * sizeof(double), hipMemcpyHostToDevice));
-        const unsigned grid = unsigned((it.ntiles + kThreads - 1) / kThreads);
-        hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), pt.size() * sizeof(double), ctx.stream, dpt,
-                           int(plans[i].ebit.size()), nh, nb2, poly.K, it.ntiles,
-                           reinterpret_cast<double *>(depi + epi_off[i]));
+        launch_tile_phase(ctx.stream, dpt, pt.size(), int(plans[i].ebit.size()), nh, nb2, poly.K, it.ntiles,
+                          reinterpret_cast<double *>(depi + epi_off[i]));
         hipError_t e = hipGetLastError();
         if (e == hipSuccess)
             e = hipStreamSynchronize(ctx.stream);
