@@ -45,6 +45,7 @@ def test_tuning_knobs_ranges():
              "diag.nt.c128": ([0, 1], [2]), "diag.nt.c64": ([0, 1], [2]),
              "tile.seg.c128": ([3, 4, 5, 6], [2, 7]), "tile.seg.c64": ([3, 4, 5, 6], [0, 12]),
              "tile.bufs.c128": ([2, 3], [1, 4]), "tile.bufs.c64": ([2, 3], [0, 5]),
+             "tile.wide.c128": ([0, 1], [2, -1]), "tile.wide.c64": ([0, 1], [3]),
              "tile.upper.c128": ([0, 18, 64], [-1, 65]), "tile.upper.c64": ([0, 17, 64], [-2, 100]),
              "block.variant.c128": ([0, 3, 4, 7], [-1, 8]), "block.variant.c64": ([0, 5, 7], [8, 9]),
              "block.tiles.c128": ([0, 1, 4, 4096], [-1, 4097]), "block.tiles.c64": ([0, 8], [5000])}

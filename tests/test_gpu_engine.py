@@ -442,3 +442,27 @@ def test_transported_windows_ride_the_tile_passes(gpu_engine, n, width, seed):
     assert np.max(np.abs(r.get_state_array() - want)) < TOL128
     c = make_sim(Z, n, False, False, gs, prec="c64")
     assert np.max(np.abs(c.get_state_array() - want)) < 2e-5
+
+
+@pytest.mark.parametrize("n,prec", [(16, "c128"), (21, "c128"), (22, "c64"), (24, "c128")])
+def test_qft_with_the_wide_tile(gpu_engine, n, prec):
+    """The fused policy with the wide (8192-unit) tile on: passes of up to
+    nine targets above the contiguous run, the last with the controlled-phase
+    window as its epilogue; the QFT against the oracle."""
+    from zxh_adae_amd import autotune as A
+    Z = gpu_engine
+    old = {k: A.tune_get(k) for k in ("tile.wide.c128", "tile.wide.c64")}
+    try:
+        A.tune_set("tile.wide.c128", 1)
+        A.tune_set("tile.wide.c64", 1)
+        gs = W.qft(n)
+        o = O.OracleSim(n, False, True)
+        o.gates(gs)
+        o.execute()
+        want = o.get_state()
+        s = make_sim(Z, n, False, True, gs, prec=prec)
+        got = s.get_state_array()
+        assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else 2e-5)
+    finally:
+        for k, v in old.items():
+            A.tune_set(k, v)

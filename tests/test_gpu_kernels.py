@@ -368,7 +368,12 @@ def test_fused_pass_matches_u3_then_diag(gpu_engine, tile_seg, n, low, high, ndi
     want = oracle_diag(oracle_u3(psi, n, descs) if descs else psi, n, s0, s1, th) if ndiag else oracle_u3(psi, n, descs)
     d = dev(gpu_engine, psi.astype(np.complex64) if prec == "c64" else psi, prec)
     lowbits = tile_seg + (1 if prec == "c64" else 0)
-    if n < lowbits + 12 - tile_seg or sum(q >= lowbits for q in low + high) > 12 - tile_seg:
+    shift = 1 if prec == "c64" else 0
+    from zxh_adae_amd import autotune
+    # the wide (8192-unit) tile takes up to nine targets above a 16-unit run
+    wide = (autotune.tune_get(f"tile.wide.{prec}") == 1 and n >= 13 + shift and len(low + high) <= 9
+            and all(q >= 4 + shift for q in low + high))
+    if (n < lowbits + 12 - tile_seg or sum(q >= lowbits for q in low + high) > 12 - tile_seg) and not wide:
         from zxh_adae_amd import ZXHError
         with pytest.raises(ZXHError):
             d.fused_pass(descs, s0, s1, th)
@@ -530,14 +535,13 @@ def test_upper_kernel_on_affine_maps(upper_knobs, n, cx, prec):
 @pytest.mark.parametrize("prec", ["c128", "c64"])
 def test_autotuners_follow_their_rules(upper_knobs, prec):
     """tune_diag keeps the fastest diagonal variant; tune_tile keeps the
-    geometry with the longest contiguous runs whose layer is within
-    TILE_TOLERANCE of the fastest, with the pass count of layer_passes.  Both
-    set their knobs, and a U3 batch under the tuned geometry matches the
-    oracle."""
+    fastest layer among the geometries with and without the wide tile, with
+    the pass count of layer_passes.  Both set their knobs, and a U3 batch
+    under the tuned geometry matches the oracle."""
     from zxh_adae_amd import kernels as K
     A = upper_knobs
     n = 20
-    diag_knobs = (f"diag.ept.{prec}", f"diag.nt.{prec}")
+    diag_knobs = (f"diag.ept.{prec}", f"diag.nt.{prec}", f"tile.wide.{prec}")
     old = {k: A.tune_get(k) for k in diag_knobs}
     try:
         st = K.DeviceState(n, prec)
@@ -547,14 +551,18 @@ def test_autotuners_follow_their_rules(upper_knobs, prec):
         assert d["ms"][key] == min(d["ms"].values())
         assert (A.tune_get(diag_knobs[0]), A.tune_get(diag_knobs[1])) == (d["best"]["ept"], d["best"]["nt"])
         t = A.tune_tile(st)
-        ms = {int(k[3:]): v for k, v in t["layer_ms"].items()}
-        assert sorted(ms) == sorted(A.TILE_SEGS)
-        bound = min(ms.values()) * (1.0 + A.TILE_TOLERANCE) + 1e-4  # layer_ms is rounded to 0.1 us
-        assert ms[t["best"]] <= bound
-        assert all(ms[s] > min(ms.values()) * (1.0 + A.TILE_TOLERANCE) - 1e-4 for s in ms if s > t["best"])
+
+        def key(k):
+            return (int(k[3:].split("_")[0]), 1 if k.endswith("_wide") else 0)
+        ms = {key(k): v for k, v in t["layer_ms"].items()}
+        assert sorted(ms) == sorted((s, w) for s in A.TILE_SEGS for w in (0, 1))
+        best = (t["best"], t["wide"])
+        assert ms[best] <= min(ms.values()) + 1e-4  # layer_ms is rounded to 0.1 us
         assert A.tune_get(f"tile.seg.{prec}") == t["best"]
-        for s in A.TILE_SEGS:
-            assert t["passes"][f"seg{s}"] == len(A.layer_passes(n, s, prec == "c64"))
+        assert A.tune_get(f"tile.wide.{prec}") == t["wide"]
+        for (s, w) in ms:
+            name = f"seg{s}" + ("_wide" if w else "")
+            assert t["passes"][name] == len(A.layer_passes(n, s, prec == "c64", bool(w)))
         st.free()
         rng = np.random.default_rng(5)
         psi = rand_state(rng, n)
@@ -642,14 +650,76 @@ def test_fused_pass_with_a_tabulated_window(upper_knobs, prec, seg, n, targets, 
     assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
 
 
+@pytest.fixture
+def wide_knobs(gpu_engine):
+    """Turns the wide (8192-unit) tile on for a test and restores the knobs."""
+    from zxh_adae_amd import autotune
+    knobs = ("tile.wide.c128", "tile.wide.c64", "tile.seg.c128", "tile.seg.c64")
+    old = {k: autotune.tune_get(k) for k in knobs}
+    autotune.tune_set("tile.wide.c128", 1)
+    autotune.tune_set("tile.wide.c64", 1)
+    yield autotune
+    for k, v in old.items():
+        autotune.tune_set(k, v)
+
+
+@pytest.mark.parametrize("prec,n,targets,ndiag", [
+    ("c128", 13, [4, 5, 6, 7, 8, 9, 10, 11, 12], 0), ("c128", 16, [15, 4, 9, 12, 13, 7, 10, 11, 14], 40),
+    ("c128", 18, [17, 16, 5], 64), ("c128", 20, [12, 13, 14, 15, 16, 17, 18, 19, 11], 100),
+    ("c64", 14, [5, 6, 7, 8, 9, 10, 11, 12, 13], 0), ("c64", 17, [16, 5, 9, 12, 13, 7, 10, 11, 14], 40),
+    ("c64", 21, [20, 19, 18, 17, 16, 15, 14, 13, 12], 64)])
+def test_wide_tile_pass_matches_the_oracle(wide_knobs, prec, n, targets, ndiag):
+    """Nine targets above the 16-unit run exceed the 4096-unit tile's slots
+    at any geometry with a run of 16 or more units; with the wide tile on they
+    run as one 8192-unit tile pass, then the polynomial window, and match the
+    oracle's U3 batch and diagonal sweep (complex128 1e-12, complex64 float
+    rounding)."""
+    wide_knobs.tune_set(f"tile.seg.{prec}", 4)
+    rng = np.random.default_rng(n * 3 + ndiag)
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in targets]
+    s0, s1, th = diag_window(rng, n, ndiag) if ndiag else ([], [], [])
+    want = oracle_u3(psi, n, descs)
+    if ndiag:
+        want = oracle_diag(want, n, s0, s1, th)
+    d = dev(wide_knobs, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    d.fused_pass(descs, s0, s1, th)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+
+
+@pytest.mark.parametrize("prec,n", [("c128", 20), ("c128", 23), ("c64", 21)])
+def test_u3_batch_on_every_qubit_with_the_wide_tile(wide_knobs, prec, n):
+    """All targets of a layer: the first pass as the 4096-unit tile takes it,
+    the rest in wide passes of nine; against the oracle."""
+    from zxh_adae_amd import kernels as K
+    rng = np.random.default_rng(n)
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in range(n)]
+    want = oracle_u3(psi, n, descs)
+    d = dev(wide_knobs, psi.astype(np.complex64) if prec == "c64" else psi, prec)
+    K.reset_timing()
+    K.set_timing(True)
+    try:
+        d.u3_batch(descs)
+        passes = K.timing("u3_tile")["launches"]
+    finally:
+        K.set_timing(False)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL128 if prec == "c128" else TOL64)
+    assert passes >= 2
+
+
 def test_fused_pass_rejects_multibit_and_overfull(gpu_engine):
     from zxh_adae_amd import ZXHError
     rng = np.random.default_rng(1)
     d = dev(gpu_engine, rand_state(rng, 13))
     with pytest.raises(ZXHError):
         d.fused_pass([(pack(3, False), 1, rand_u(rng))], [], [], [])
+    # more targets above the low bits than either tile takes (the 4096-unit
+    # tile: 12 - seg; the wide tile: nine, none below unit bit 4)
     with pytest.raises(ZXHError):
-        d.fused_pass([(pack(1 << q, False), 1 << q, rand_u(rng)) for q in range(4, 13)], [], [], [])
+        d.fused_pass([(pack(1 << q, False), 1 << q, rand_u(rng)) for q in range(3, 13)], [], [], [])
 
 
 @pytest.mark.parametrize("prec", ["c128", "c64"])

@@ -71,15 +71,27 @@ def tune_diag(state: "K.DeviceState", repeats: int = 3, batch: int = 64) -> Dict
 TILE_SEGS = (3, 4, 5, 6)
 
 
-def layer_passes(n: int, seg: int, c64: bool):
+def layer_passes(n: int, seg: int, c64: bool, wide: bool = False):
     """Targets of each tile pass of a full-width single-qubit layer under
     geometry seg: the low bits ride in the first pass, then 12 - seg upper
-    targets per pass."""
+    targets per pass.  With the wide (8192-unit) tile, as the engine forms
+    them: the first pass takes the low bits and the 12 - seg highest targets,
+    then passes of nine ascending targets, each above the 16-unit run (else
+    12 - seg)."""
     low = seg + (1 if c64 else 0)
     slots = 12 - seg
     upper = list(range(low, n))
-    chunks = [upper[i:i + slots] for i in range(0, len(upper), slots)] or [[]]
-    chunks[0] = list(range(low)) + chunks[0]
+    if not wide or len(upper) <= slots:
+        chunks = [upper[i:i + slots] for i in range(0, len(upper), slots)] or [[]]
+        chunks[0] = list(range(low)) + chunks[0]
+        return chunks
+    chunks = [list(range(low)) + upper[len(upper) - slots:]]
+    rest = upper[:len(upper) - slots]
+    i = 0
+    while i < len(rest):
+        take = 9 if rest[i] >= 4 + (1 if c64 else 0) else slots
+        chunks.append(rest[i:i + take])
+        i += take
     return chunks
 
 
@@ -87,29 +99,30 @@ TILE_TOLERANCE = 0.03
 
 
 def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dict[str, object]:
-    """Times a full-width layer per tile geometry on state: U3 on every qubit,
-    split into the geometry's tile passes, each pass fused with a window of
-    P/CP gates.  Fewer contiguous bits per tile leave room for more targets
-    per pass (fewer passes) at shorter contiguous runs, so the choice is by
-    the layer's total time: the geometry with the longest runs whose layer
-    is within TILE_TOLERANCE of the fastest (longer runs keep passes with
-    fewer targets fast, and a difference of a few percent is within
-    run-to-run variation).  Sets that geometry; the state is changed by the
-    passes."""
+    """Times a full-width layer per tile geometry on state, with and without
+    the wide (8192-unit) tile: U3 on every qubit, split into the geometry's
+    tile passes (layer_passes), each pass fused with a window of P/CP gates.
+    Fewer contiguous bits per tile leave room for more targets per pass
+    (fewer passes) at shorter contiguous runs, so the choice is by the
+    layer's total time: the fastest.  Sets the geometry and the wide-tile
+    knob; the state is changed by the passes."""
     c64 = state.prec == _lib.ZXH_C64
     knob = "tile.seg.c64" if c64 else "tile.seg.c128"
+    wknob = "tile.wide.c64" if c64 else "tile.wide.c128"
     n = state.nbits
     old = tune_get(knob)
+    old_wide = tune_get(wknob)
     gs = W.diag_batch(n, window, 4343)
     s0 = [(1 << g[2]) for g in gs]
     s1 = [K.NO_SEL if g[0] == W.P else (1 << g[1]) for g in gs]
     th = [g[3] for g in gs]
-    results: Dict[int, float] = {}
-    npasses: Dict[int, int] = {}
-    for seg in TILE_SEGS:
+    results: Dict[tuple, float] = {}
+    npasses: Dict[tuple, int] = {}
+    for seg, wide in [(sg, w) for w in (0, 1) for sg in TILE_SEGS]:
         tune_set(knob, seg)
+        tune_set(wknob, wide)
         passes = [[((1 << q), (1 << q), W.u3_matrix(0.3 + 0.01 * q, 0.2, 0.1)) for q in chunk]
-                  for chunk in layer_passes(n, seg, c64)]
+                  for chunk in layer_passes(n, seg, c64, bool(wide))]
         for descs in passes:  # warm
             state.fused_pass(descs, s0, s1, th)
         K.reset_timing()
@@ -119,14 +132,21 @@ def tune_tile(state: "K.DeviceState", repeats: int = 2, window: int = 32) -> Dic
                 state.fused_pass(descs, s0, s1, th)
         t = K.timing("fused")
         K.set_timing(False)
-        results[seg] = t["ms"] / repeats
-        npasses[seg] = len(passes)
-    fastest = min(results.values())
-    best = max(k for k, v in results.items() if v <= fastest * (1.0 + TILE_TOLERANCE))
-    tune_set(knob, best)
-    return {"knob": knob, "best": best, "previous": old,
-            "layer_ms": {f"seg{k}": round(v, 4) for k, v in results.items()},
-            "passes": {f"seg{k}": v for k, v in npasses.items()}}
+        results[(seg, wide)] = t["ms"] / repeats
+        npasses[(seg, wide)] = len(passes)
+    # the fastest layer (the wide tile changes the pass split, so the
+    # longest-runs preference among near ties no longer picks the faster
+    # steps: at 30 qubits c128 seg 6 with the wide tile is within 3% of seg 4
+    # on this layer and 4.5% slower on the QFT step)
+    best = min(results, key=lambda k: results[k])
+    tune_set(knob, best[0])
+    tune_set(wknob, best[1])
+
+    def name(k):
+        return f"seg{k[0]}" + ("_wide" if k[1] else "")
+    return {"knob": knob, "best": best[0], "wide": best[1], "previous": old, "previous_wide": old_wide,
+            "layer_ms": {name(k): round(v, 4) for k, v in results.items()},
+            "passes": {name(k): v for k, v in npasses.items()}}
 
 
 def tune_batch(sets, precision: str = "c64", sample: int = 8, repeats: int = 2, seed: int = 11) -> Dict[str, object]:

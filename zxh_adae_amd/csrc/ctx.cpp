@@ -298,6 +298,8 @@ Knob g_knobs[] = {
     {"tile.seg.c64", 4, 3, 6},
     {"tile.bufs.c128", 2, 2, 3},
     {"tile.bufs.c64", 2, 2, 3},
+    {"tile.wide.c128", 1, 0, 1},
+    {"tile.wide.c64", 1, 0, 1},
     {"tile.upper.c128", 18, 0, 64},
     {"tile.upper.c64", 18, 0, 64},
     {"block.variant.c128", 4, 0, 7},

@@ -199,6 +199,8 @@ struct Ctx
 //   diag.nt.c128 / diag.nt.c64     nontemporal loads/stores in the flat sweep
 //   tile.seg.c128 / tile.seg.c64   contiguous unit-address bits of a tile (3..6)
 //   tile.bufs.c128 / tile.bufs.c64 register buffers per tile-pass workgroup (2, 3)
+//   tile.wide.c128 / tile.wide.c64 1: passes of up to nine targets above a
+//                                  16-unit run may use the 8192-unit tile
 //   tile.upper.c128 / .c64         a pass with every target on unit bits 6..11
 //                                  whose highest target unit-address bit is at
 //                                  least this runs on the LDS-free upper kernel
@@ -251,6 +253,11 @@ unsigned tile_seg(int prec, int geometry = -1);
 unsigned tile_low_bits(int prec, int geometry = -1);
 unsigned tile_upper_slots(int prec, int geometry = -1);
 unsigned tile_min_bits(int prec);
+// True when single-bit targets (amplitude address bits) fit one wide tile
+// pass (knob tile.wide.<prec> = 1): at most nine, distinct, all above the
+// 16-unit contiguous run, on a state of at least 13 unit bits.  Such a pass
+// moves 256-byte runs where the 4096-unit tile would move 128-byte ones.
+bool tile_wide_fits(int prec, unsigned n, const std::vector<unsigned> &targets);
 // True when single-bit targets in this (circuit) order fit one staged tile
 // pass of a 2^n complex128 state: the order a pass with fans keeps.
 bool tile_staged_fits(int prec, unsigned n, const std::vector<unsigned> &targets, int geometry = -1);

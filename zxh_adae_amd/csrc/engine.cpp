@@ -336,7 +336,7 @@ int Sim::flush_diag(std::vector<DiagGate> &window)
     int rc = OK;
     bool ride = fusable() && window_is_polynomial(active_bits(), window.data(), window.size()) &&
                 window_is_polynomial(active_bits(), pass_diag_.data(), pass_diag_.size());
-    if (fusable() && !ride && !pass_has_fan_)
+    if (fusable() && !ride && !pass_has_fan_ && unsigned(pass_high_) <= tile_upper_slots(prec_))
     {
         // selectors beyond the polynomial form: the pass's window as a table
         std::vector<DiagGate> all = pass_diag_;
@@ -525,6 +525,23 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
     // circuit order (tile_staged_fits).  Otherwise a gate the window touches
     // starts a new pass.
     const bool can_fan = prec_ == C128 && !disable_x_;
+    // A batch that opens on an empty pass (no window: no fans) and needs more
+    // than one pass, with the wide tile on: the first pass takes the targets
+    // below the low bits and the highest ones (its runs are the longest the
+    // geometry makes, which the widest strides need most), the wide passes
+    // the rest in ascending order.  The gates act on distinct qubits of one
+    // batch, so any order is the same product.
+    if (pass_u3_.empty() && pass_diag_.empty() && descs.size() > tile_upper_slots(prec_) &&
+        tune_get(prec_ == C64 ? "tile.wide.c64" : "tile.wide.c128") == 1)
+    {
+        auto tgt = [](const U3Gate &g) { return unsigned(__builtin_ctzll(g.delta)); };
+        std::stable_sort(descs.begin(), descs.end(), [&](const U3Gate &a, const U3Gate &b) { return tgt(a) < tgt(b); });
+        const unsigned lowb = tile_low_bits(prec_);
+        const size_t nlow = size_t(std::count_if(descs.begin(), descs.end(),
+                                                 [&](const U3Gate &g) { return tgt(g) < lowb; }));
+        const size_t ntop = std::min(descs.size() - nlow, size_t(tile_upper_slots(prec_)));
+        std::rotate(descs.begin() + long(nlow), descs.end() - long(ntop), descs.end());
+    }
     auto order_with = [&](unsigned t) {
         std::vector<unsigned> v;
         for (const U3Gate &p : pass_u3_)
@@ -580,8 +597,16 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
             continue;
         }
         const bool high = t >= low;
-        const bool fits = (pass_has_fan_ || fan.on) ? tile_staged_fits(prec_, n, order_with(t))
-                                                    : !(high && unsigned(pass_high_) >= tile_upper_slots(prec_));
+        // (a pass already wider than the 4096-unit tile takes no low target)
+        bool fits = (pass_has_fan_ || fan.on) ? tile_staged_fits(prec_, n, order_with(t))
+                                              : unsigned(pass_high_) + (high ? 1u : 0u) <= tile_upper_slots(prec_);
+        if (!fits && !pass_has_fan_ && !fan.on &&
+            window_is_polynomial(n, pass_diag_.data(), pass_diag_.size()))
+        {
+            // the wide tile: up to nine targets above a 16-unit run
+            std::vector<unsigned> ts = order_with(t);
+            fits = tile_wide_fits(prec_, n, ts);
+        }
         if (!fits)
         {
             // (a fan was taken only where the staged order fits)

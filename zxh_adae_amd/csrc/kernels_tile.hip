@@ -505,12 +505,12 @@ __device__ __forceinline__ cd cmul(cd a, cd b)
 // with ST, G tile-independent (per-thread table) and E, D per tile (written
 // by tile_phase_kernel).  rho is visited in Gray-code order so that the last
 // product costs one complex multiplication per amplitude.
-template <bool kC64>
+template <bool kC64, int kThr = 8>
 struct Epi
 {
     static constexpr int kNB = kC64 ? 5 : 4;         // register element bits
     static constexpr int kNR = 1 << kNB;             // register elements
-    static constexpr int kStride = 2 * (8 + kNR);    // doubles per tile in tphase
+    static constexpr int kStride = 2 * (kThr + kNR); // doubles per tile in tphase
     static constexpr int kThrStride = 2 * (1 + kNB); // doubles per thread in ttab
     cd st;
     cd g[kNB];
@@ -560,7 +560,7 @@ struct Epi
     {
         cd w = st;
 #pragma unroll
-        for (int j = 0; j < 8; j++)
+        for (int j = 0; j < kThr; j++)
         {
             const cd e = {tp[2 * j], tp[2 * j + 1]};
             const cd we = cmul(w, e);
@@ -568,7 +568,7 @@ struct Epi
             w.x = on ? we.x : w.x;
             w.y = on ? we.y : w.y;
         }
-        ZXA_CONST double *D = tp + 16;
+        ZXA_CONST double *D = tp + 2 * kThr;
 #pragma unroll
         for (int k = 0; k < kNR; k++)
         {
@@ -997,6 +997,177 @@ __global__ __launch_bounds__(kThreads) void upper_kernel(void *__restrict__ stat
 }
 
 // ---------------------------------------------------------------------------
+// Wide tile pass: 8192-unit tiles (13 unit bits) for passes of up to nine
+// targets above the contiguous run.  Unit bits 0..3 are the run (256-byte
+// runs of 16-byte units, where the 4096-unit tile with nine such targets
+// has 128-byte runs); unit bits 4..12 are the nine slots.  512 threads hold
+// 16 units each:
+//   layout 0 (load, store): thread bits = unit bits 0..8, register bits =
+//     unit bits 9..12; unit bit 4 is lane bit 4, reached by exchanging it
+//     with register bit 0 (v_permlane16_swap);
+//   layout 1: thread bits = unit bits 0..4 and 9..12, register bits = unit
+//     bits 5..8, through LDS (the whole tile, 128 KiB: one workgroup per CU).
+// Each thread writes back exactly the LDS units it read, so a transpose
+// needs one barrier, after its writes.  Pipelined as tile_kernel.
+constexpr int kW13Threads = 512;
+constexpr int kW13Bits = 13;
+constexpr int kW13Units = 1 << kW13Bits;
+constexpr int kW13Slots = 9; // unit bits 4..12
+
+struct P13
+{
+    uint32_t sorted[kW13Bits]; // the tile's unit-address bits, ascending
+    uint64_t ntiles;
+    uint64_t tmask;
+    uint64_t gstep;
+};
+
+struct Tab13
+{
+    uint64_t roff[kRegs];      // layout-0 register offsets (unit bits 9..12)
+    uint64_t toffv[9];         // unit-address offset of each thread bit (unit bits 0..8)
+    TGate g[kW13Slots];        // slot s: unit bit 4 + s (the identity where no gate)
+};
+
+__host__ __device__ __forceinline__ uint64_t deposit13(uint64_t tile, const uint32_t *ub)
+{
+    for (int k = 0; k < kW13Bits; k++)
+    {
+        const unsigned p = ub[k];
+        tile = ((tile >> p) << (p + 1)) | (tile & ((uint64_t(1) << p) - 1));
+    }
+    return tile;
+}
+
+__device__ __forceinline__ int w13_l1(int t, int r)
+{
+    return (t & 31) | (r << 5) | ((t >> 5) << 9);
+}
+
+template <bool kC64, bool kDiag>
+__device__ __forceinline__ void w13_work(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds,
+                                         ZXA_CONST Tab13 *G, const double *tphase, const Epi<kC64, 9> &epi, int t,
+                                         uint64_t tile)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        gate_on_reg_bit<kC64, false>(v, i, G->g[5 + i], 0, 0);
+#pragma unroll
+    for (int r = 0; r < kRegs; r += 2)
+        lane_swap<false>(v[r], v[r + 1]);
+    gate_on_reg_bit<kC64, false>(v, 0, G->g[0], 0, 0);
+#pragma unroll
+    for (int r = 0; r < kRegs; r += 2)
+        lane_swap<false>(v[r], v[r + 1]);
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        lds[t | (r << 9)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        v[r] = lds[w13_l1(t, r)];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        gate_on_reg_bit<kC64, false>(v, i, G->g[1 + i], 0, 0);
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        lds[w13_l1(t, r)] = v[r];
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        v[r] = lds[t | (r << 9)];
+    if constexpr (kDiag)
+        epi.apply(v, (ZXA_CONST double *)(tphase + tile * Epi<kC64, 9>::kStride), t);
+}
+
+template <bool kC64>
+__device__ __forceinline__ void w13_load(typename UnitT<kC64>::T (&v)[kRegs], const typename UnitT<kC64>::T *p,
+                                         uint64_t at, ZXA_CONST Tab13 *G)
+{
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        v[r] = __builtin_nontemporal_load(p + (at ^ G->roff[r]));
+}
+
+template <bool kC64>
+__device__ __forceinline__ void w13_store(const typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *p,
+                                          uint64_t at, ZXA_CONST Tab13 *G)
+{
+    asm volatile("" : "+v"(at));
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+        __builtin_nontemporal_store(v[r], p + (at ^ G->roff[r]));
+}
+
+__device__ __forceinline__ ZXA_CONST Tab13 *tables13(const Tab13 *tabp)
+{
+    asm volatile("" : "+s"(tabp));
+    return (ZXA_CONST Tab13 *)tabp;
+}
+
+__device__ __forceinline__ uint64_t next_base13(uint64_t base, const P13 &P)
+{
+    return ((base | P.tmask) + P.gstep) & ~P.tmask;
+}
+
+template <bool kC64, bool kDiag>
+__global__ __launch_bounds__(kW13Threads) __attribute__((amdgpu_waves_per_eu(2))) void wide13_kernel(
+    void *__restrict__ state, const P13 P, const Tab13 *__restrict__ tabp, const double *__restrict__ ttab,
+    const double *__restrict__ tphase)
+{
+    using T = typename UnitT<kC64>::T;
+    __shared__ T lds[kW13Units];
+    const int t = threadIdx.x;
+    T *p = static_cast<T *>(state);
+    ZXA_CONST Tab13 *G = tables13(tabp);
+    uint64_t toff = 0;
+#pragma unroll
+    for (int j = 0; j < 9; j++)
+        toff ^= ((t >> j) & 1) ? G->toffv[j] : 0;
+    Epi<kC64, 9> epi;
+    if constexpr (kDiag)
+        epi.init(ttab, t);
+    const uint64_t grid = gridDim.x;
+    T a[kRegs], b[kRegs];
+    uint64_t t0 = blockIdx.x;
+    uint64_t b0 = deposit13(t0, P.sorted);
+    w13_load<kC64>(a, p, b0 ^ toff, G);
+    uint64_t t1 = t0 + grid;
+    bool m1 = t1 < P.ntiles;
+    uint64_t b1 = next_base13(b0, P);
+    w13_load<kC64>(b, p, (m1 ? b1 : b0) ^ toff, G);
+    w13_work<kC64, kDiag>(a, lds, G, tphase, epi, t, t0);
+    for (;;)
+    {
+        G = tables13(tabp);
+        w13_store<kC64>(a, p, b0 ^ toff, G);
+        if (!m1)
+            break;
+        const uint64_t t2 = t1 + grid;
+        const bool m2 = t2 < P.ntiles;
+        const uint64_t b2 = next_base13(b1, P);
+        w13_load<kC64>(a, p, (m2 ? b2 : b1) ^ toff, G);
+        w13_work<kC64, kDiag>(b, lds, G, tphase, epi, t, t1);
+
+        G = tables13(tabp);
+        w13_store<kC64>(b, p, b1 ^ toff, G);
+        if (!m2)
+            break;
+        const uint64_t t3 = t2 + grid;
+        const bool m3 = t3 < P.ntiles;
+        const uint64_t b3 = next_base13(b2, P);
+        w13_load<kC64>(b, p, (m3 ? b3 : b2) ^ toff, G);
+        w13_work<kC64, kDiag>(a, lds, G, tphase, epi, t, t2);
+
+        t0 = t2;
+        b0 = b2;
+        t1 = t3;
+        b1 = b3;
+        m1 = m3;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint pass: the reverse of one forward tile pass, on two states at once.
 // psi is the ansatz state and lam = (later gates)^dagger H psi, both taken at
 // the point just after the forward pass's single-qubit gates.  A rotation's
@@ -1413,11 +1584,11 @@ __device__ __forceinline__ double tp_const(const double *hL, const double *hQ, i
 // order, so the same factors).
 __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__restrict__ tab, int nE, int nh, int nb,
                                                               double K, uint64_t ntiles, double *__restrict__ out,
-                                                              int spread)
+                                                              int spread, int nthr)
 {
     extern __shared__ double sh[];
     const int nR = 1 << nb;
-    const int ntab = nE * nh + nh + nh * nh + nR + 8 + nb;
+    const int ntab = nE * nh + nh + nh * nh + nR + nthr + nb;
     for (int i = threadIdx.x; i < ntab; i += blockDim.x)
         sh[i] = tab[i];
     __syncthreads();
@@ -1426,7 +1597,7 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
     const double *hQ = hL + nh;
     const double *rs = hQ + nh * nh;
     const double *mp = rs + nR;
-    const int per = 8 + nR;
+    const int per = nthr + nR;
     const int stride = 2 * per;
     const uint64_t id = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (spread)
@@ -1436,15 +1607,15 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
         const uint64_t tile = id / uint64_t(per);
         const int j = int(id % uint64_t(per));
         double a;
-        if (j < 8)
+        if (j < nthr)
             a = tp_coef(cq, nh, tile, static_cast<int>(mp[j]));
         else
         {
-            const int rho = j - 8;
+            const int rho = j - nthr;
             a = tp_const(hL, hQ, nh, tile, K) + rs[rho];
             for (int b = 0; b < nb; b++)
                 if ((rho >> b) & 1)
-                    a += tp_coef(cq, nh, tile, static_cast<int>(mp[8 + b]));
+                    a += tp_coef(cq, nh, tile, static_cast<int>(mp[nthr + b]));
         }
         const cd e = cis(a);
         out[tile * stride + 2 * j] = e.x;
@@ -1454,12 +1625,12 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
     const uint64_t tile = id;
     if (tile >= ntiles)
         return;
-    double c[13];
+    double c[14];
     for (int k = 0; k < nE; k++)
         c[k] = tp_coef(cq, nh, tile, k);
     const double C = tp_const(hL, hQ, nh, tile, K);
     double *o = out + tile * stride;
-    for (int j = 0; j < 8; j++)
+    for (int j = 0; j < nthr; j++)
     {
         const cd e = cis(c[static_cast<int>(mp[j])]);
         o[2 * j] = e.x;
@@ -1470,10 +1641,10 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
         double a = C + rs[rho];
         for (int b = 0; b < nb; b++)
             if ((rho >> b) & 1)
-                a += c[static_cast<int>(mp[8 + b])];
+                a += c[static_cast<int>(mp[nthr + b])];
         const cd d = cis(a);
-        o[16 + 2 * rho] = d.x;
-        o[16 + 2 * rho + 1] = d.y;
+        o[2 * nthr + 2 * rho] = d.x;
+        o[2 * nthr + 2 * rho + 1] = d.y;
     }
 }
 
@@ -1481,13 +1652,13 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
 // has few tiles (below 4096 one thread per tile leaves most CUs idle and each
 // thread runs 8 + 2^nb sincos in a row).
 void launch_tile_phase(hipStream_t stream, const double *tab, size_t tab_doubles, int nE, int nh, int nb, double K,
-                       uint64_t ntiles, double *out)
+                       uint64_t ntiles, double *out, int nthr = 8)
 {
     const int spread = ntiles < 4096 ? 1 : 0;
-    const uint64_t threads = spread ? ntiles * uint64_t(8 + (1 << nb)) : ntiles;
+    const uint64_t threads = spread ? ntiles * uint64_t(nthr + (1 << nb)) : ntiles;
     const unsigned grid = unsigned((threads + kThreads - 1) / kThreads);
     hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), tab_doubles * sizeof(double), stream, tab, nE,
-                       nh, nb, K, ntiles, out, spread);
+                       nh, nb, K, ntiles, out, spread, nthr);
 }
 
 template <typename K>
@@ -1760,31 +1931,34 @@ bool plan_tile(int prec, unsigned n, const std::vector<SingleGate> &gs, Plan &pl
 // Host tables of the diagonal epilogue: the per-thread table (ST, G) and the
 // packed input of tile_phase_kernel.
 void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, const Poly &poly,
-                     std::vector<double> &ttab, std::vector<double> &ptab, int *nh_out, int *nb_out)
-// pos: address bit of each tile element bit (Plan::ebit)
+                     std::vector<double> &ttab, std::vector<double> &ptab, int *nh_out, int *nb_out, int nthr = 8)
+// pos: address bit of each tile element bit (Plan::ebit); nthr thread bits
+// (element bits 0.. of the thread index, above the complex64 in-unit bit),
+// the register bits above them
 {
     const bool c64 = prec == C64;
     const int nE = int(pos.size());
     const int nb = c64 ? 5 : 4;
     const int nR = 1 << nb;
-    int thr_k[8], reg_k[5];
-    for (int j = 0; j < 8; j++)
+    const int nthreads = 1 << nthr;
+    int thr_k[9], reg_k[5];
+    for (int j = 0; j < nthr; j++)
         thr_k[j] = c64 ? j + 1 : j;
     for (int b = 0; b < nb; b++)
-        reg_k[b] = c64 ? (b == 0 ? 0 : 8 + b) : 8 + b;
+        reg_k[b] = c64 ? (b == 0 ? 0 : nthr + b) : nthr + b;
     auto Lt = [&](int k) { return poly.L[pos[k]]; };
     auto Qt = [&](int k, int l) { return poly.q(int(pos[k]), int(pos[l])); };
 
-    ttab.assign(size_t(kThreads) * 2 * (1 + nb), 0.0);
-    for (int t = 0; t < kThreads; t++)
+    ttab.assign(size_t(nthreads) * 2 * (1 + nb), 0.0);
+    for (int t = 0; t < nthreads; t++)
     {
         double s = 0.0;
-        for (int j = 0; j < 8; j++)
+        for (int j = 0; j < nthr; j++)
         {
             if (!((t >> j) & 1))
                 continue;
             s += Lt(thr_k[j]);
-            for (int l = j + 1; l < 8; l++)
+            for (int l = j + 1; l < nthr; l++)
                 if ((t >> l) & 1)
                     s += Qt(thr_k[j], thr_k[l]);
         }
@@ -1794,7 +1968,7 @@ void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, con
         for (int b = 0; b < nb; b++)
         {
             double d = 0.0;
-            for (int j = 0; j < 8; j++)
+            for (int j = 0; j < nthr; j++)
                 if ((t >> j) & 1)
                     d += Qt(thr_k[j], reg_k[b]);
             a[2 + 2 * b] = std::cos(d);
@@ -1807,7 +1981,7 @@ void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, con
         if (std::find(pos.begin(), pos.end(), b) == pos.end())
             outer.push_back(int(b));
     const int nh = int(outer.size());
-    ptab.assign(size_t(nE) * nh + nh + size_t(nh) * nh + nR + 8 + nb, 0.0);
+    ptab.assign(size_t(nE) * nh + nh + size_t(nh) * nh + nR + nthr + nb, 0.0);
     double *cq = ptab.data();
     double *hL = cq + nE * nh;
     double *hQ = hL + nh;
@@ -1836,10 +2010,10 @@ void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, con
         }
         rs[rho] = s;
     }
-    for (int j = 0; j < 8; j++)
+    for (int j = 0; j < nthr; j++)
         mp[j] = thr_k[j];
     for (int b = 0; b < nb; b++)
-        mp[8 + b] = reg_k[b];
+        mp[nthr + b] = reg_k[b];
     *nh_out = nh;
     *nb_out = nb;
 }
@@ -1847,20 +2021,20 @@ void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, con
 // Stages the diagonal epilogue of a pass: the per-thread table and the
 // per-tile phase factors (tile_phase_kernel into scratch_a).
 int setup_epilogue(Ctx &ctx, int prec, unsigned n, const std::vector<unsigned> &pos, const DiagGate *diag,
-                   size_t ndiag, uint64_t ntiles, const double **ttab, const double **tphase)
+                   size_t ndiag, uint64_t ntiles, const double **ttab, const double **tphase, int nthr = 8)
 {
     Poly poly;
     window_polynomial(n, diag, ndiag, poly);
     std::vector<double> tt, pt;
     int nh = 0, nb = 0;
-    epilogue_tables(prec, n, pos, poly, tt, pt, &nh, &nb);
+    epilogue_tables(prec, n, pos, poly, tt, pt, &nh, &nb, nthr);
     void *dtt = nullptr, *dpt = nullptr;
     ZXA_TRY(ctx.ring.stage(tt.data(), tt.size() * sizeof(double), ctx.stream, &dtt));
     ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
-    const size_t stride = 2 * (8 + (size_t(1) << nb));
+    const size_t stride = 2 * (size_t(nthr) + (size_t(1) << nb));
     ZXA_TRY(ctx.scratch_a.ensure(ntiles * stride * sizeof(double)));
     launch_tile_phase(ctx.stream, static_cast<const double *>(dpt), pt.size(), int(pos.size()), nh, nb, poly.K, ntiles,
-                      static_cast<double *>(ctx.scratch_a.ptr));
+                      static_cast<double *>(ctx.scratch_a.ptr), nthr);
     ZXA_HIP(hipGetLastError());
     *ttab = static_cast<const double *>(dtt);
     *tphase = static_cast<const double *>(ctx.scratch_a.ptr);
@@ -2065,6 +2239,140 @@ void win_table(const WinBasis &B, const DiagGate *gates, size_t ngates, std::vec
         tab[2 * size_t(y)] = std::cos(th);
         tab[2 * size_t(y) + 1] = std::sin(th);
     }
+}
+
+// The wide (13-unit-bit) tile: single-bit gates (alpha == delta) on distinct
+// targets, all above the 16-unit contiguous run, at most nine; a state of
+// at least 13 unit bits.  Slots take the targets' unit bits and, where fewer
+// than nine, the lowest free unit bits above the run (which lengthen the
+// runs), ascending.  ebit: the address bit of each tile element bit (as
+// Plan::ebit) for the epilogue tables.
+bool wide13_plan(int prec, unsigned n, const std::vector<SingleGate> &gs, P13 &P, Tab13 &G,
+                 std::vector<unsigned> &ebit)
+{
+    const bool c64 = prec == C64;
+    const unsigned shift = c64 ? 1u : 0u;
+    if (n < unsigned(kW13Bits) + shift || n > 40 || gs.size() > size_t(kW13Slots))
+        return false;
+    const unsigned nu = n - shift;
+    std::vector<unsigned> S;
+    for (const SingleGate &g : gs)
+    {
+        const uint64_t alpha = g.g->packed_sel & ((uint64_t(1) << 63) - 1);
+        if (g.g->delta != (uint64_t(1) << g.target) || alpha != g.g->delta || g.target < 4 + shift)
+            return false;
+        const unsigned u = g.target - shift;
+        if (std::find(S.begin(), S.end(), u) != S.end())
+            return false;
+        S.push_back(u);
+    }
+    for (unsigned u = 4; u < nu && S.size() < size_t(kW13Slots); u++)
+        if (std::find(S.begin(), S.end(), u) == S.end())
+            S.push_back(u);
+    if (S.size() != size_t(kW13Slots))
+        return false;
+    std::sort(S.begin(), S.end());
+    uint32_t ub[kW13Bits];
+    for (int u = 0; u < 4; u++)
+        ub[u] = uint32_t(u);
+    for (int k = 0; k < kW13Slots; k++)
+        ub[4 + k] = S[size_t(k)];
+    std::memset(&P, 0, sizeof(P));
+    std::memset(&G, 0, sizeof(G));
+    for (int k = 0; k < kW13Bits; k++)
+    {
+        P.sorted[k] = ub[k];
+        P.tmask |= uint64_t(1) << ub[k];
+    }
+    std::sort(P.sorted, P.sorted + kW13Bits);
+    P.ntiles = uint64_t(1) << (nu - unsigned(kW13Bits));
+    for (int j = 0; j < 9; j++)
+        G.toffv[j] = uint64_t(1) << ub[j];
+    for (int r = 0; r < kRegs; r++)
+        for (int i = 0; i < 4; i++)
+            if ((r >> i) & 1)
+                G.roff[r] ^= uint64_t(1) << ub[9 + i];
+    for (TGate &tg : G.g)
+    {
+        tg.u[0] = tg.u[6] = 1.0;
+        tg.uf[0] = tg.uf[6] = 1.0f;
+    }
+    for (const SingleGate &g : gs)
+    {
+        const unsigned u = g.target - shift;
+        const int slot = int(std::find(S.begin(), S.end(), u) - S.begin());
+        const double *m = g.g->u;
+        // beta = 1: the pair's first element is the target-bit-one amplitude
+        const bool swap = (g.g->packed_sel >> 63) != 0;
+        TGate &tg = G.g[slot];
+        for (int e = 0; e < 8; e++)
+        {
+            tg.u[e] = swap ? m[e ^ 6] : m[e];
+            tg.uf[e] = float(tg.u[e]);
+        }
+    }
+    ebit.clear();
+    if (c64)
+        ebit.push_back(0);
+    for (int u = 0; u < kW13Bits; u++)
+        ebit.push_back(ub[u] + shift);
+    return true;
+}
+
+template <bool kC64, bool kDiag>
+int launch_wide13(Ctx &ctx, void *state, const P13 &P0, const Tab13 *tab, const double *ttab, const double *tphase)
+{
+    auto k = wide13_kernel<kC64, kDiag>;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, kW13Threads, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    const uint64_t cap = uint64_t(per_cu) * uint64_t(ctx.num_cus > 0 ? ctx.num_cus : 256);
+    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(P0.ntiles, cap));
+    P13 P = P0;
+    P.gstep = deposit13(grid, P.sorted);
+    hipLaunchKernelGGL(k, dim3(unsigned(grid)), dim3(kW13Threads), 0, ctx.stream, state, P, tab, ttab, tphase);
+    return hip_check(hipGetLastError(), "wide13_kernel launch");
+}
+
+// One wide tile pass, then the window (a phase polynomial) as its epilogue.
+int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
+                    const DiagGate *diag, size_t ndiag)
+{
+    const bool c64 = prec == C64;
+    P13 P;
+    Tab13 G;
+    std::vector<unsigned> ebit;
+    if (!wide13_plan(prec, n, gs, P, G, ebit))
+    {
+        set_error("wide tile pass: the gates do not fit the wide tile");
+        return EINVAL_;
+    }
+    const double *ttab = nullptr;
+    const double *tphase = nullptr;
+    if (ndiag > 0)
+    {
+        if (!window_is_polynomial(n, diag, ndiag))
+        {
+            set_error("wide tile pass: the window is not a phase polynomial");
+            return EINVAL_;
+        }
+        ZXA_TRY(setup_epilogue(ctx, prec, n, ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 9));
+    }
+    void *dtab = nullptr;
+    ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
+    const Tab13 *tab = static_cast<const Tab13 *>(dtab);
+    const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
+    ZXA_TRY(ctx.tic());
+    int rc;
+    if (c64)
+        rc = ndiag ? launch_wide13<true, true>(ctx, state, P, tab, ttab, tphase)
+                   : launch_wide13<true, false>(ctx, state, P, tab, ttab, tphase);
+    else
+        rc = ndiag ? launch_wide13<false, true>(ctx, state, P, tab, ttab, tphase)
+                   : launch_wide13<false, false>(ctx, state, P, tab, ttab, tphase);
+    ZXA_TRY(rc);
+    ZXA_TRY(ctx.toc(ndiag > 0 ? F_FUSED : F_U3TILE, bytes));
+    return ctx.ring.done(ctx.stream);
 }
 
 // One tile pass over the given targets, then the diagonal window (if any).
@@ -2689,6 +2997,24 @@ unsigned tile_upper_slots(int prec, int geometry)
     return unsigned(kUnitBits) - tile_seg(prec, geometry);
 }
 
+bool tile_wide_fits(int prec, unsigned n, const std::vector<unsigned> &targets)
+{
+    const unsigned shift = prec == C64 ? 1u : 0u;
+    if (tune_get(prec == C64 ? "tile.wide.c64" : "tile.wide.c128") != 1)
+        return false;
+    if (n < unsigned(kW13Bits) + shift || n > 40 || targets.size() > size_t(kW13Slots))
+        return false;
+    for (size_t i = 0; i < targets.size(); i++)
+    {
+        if (targets[i] < 4 + shift || targets[i] >= n)
+            return false;
+        for (size_t j = 0; j < i; j++)
+            if (targets[j] == targets[i])
+                return false;
+    }
+    return true;
+}
+
 unsigned tile_min_bits(int prec)
 {
     return unsigned(kUnitBits) + (prec == C64 ? 1u : 0u);
@@ -2863,13 +3189,63 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
         set_error("u3_apply: dependent pair offsets in one batch");
         return EINVAL_;
     }
+    // single-bit batches: the first pass as planned, the remaining targets in
+    // wide passes of nine when that takes no more passes (their runs are
+    // longer)
+    std::vector<bool> wide13(best.size(), false);
+    if (wide.empty() && best.size() >= 2)
+    {
+        // the first pass: the targets below the low bits and the highest
+        // ones (its runs are as long as the geometry makes them, which the
+        // widest strides need most); the wide passes: the rest, ascending
+        std::vector<SingleGate> all;
+        for (const std::vector<SingleGate> &ps : best)
+            all.insert(all.end(), ps.begin(), ps.end());
+        std::sort(all.begin(), all.end(), [](const SingleGate &a, const SingleGate &b) { return a.target < b.target; });
+        const unsigned lowb = tile_low_bits(prec, geo);
+        const size_t nlow = size_t(std::count_if(all.begin(), all.end(),
+                                                 [&](const SingleGate &g) { return g.target < lowb; }));
+        const size_t ntop = std::min(all.size() - nlow, size_t(tile_upper_slots(prec, geo)));
+        std::vector<SingleGate> first(all.begin(), all.begin() + long(nlow));
+        first.insert(first.end(), all.end() - long(ntop), all.end());
+        std::vector<SingleGate> later(all.begin() + long(nlow), all.end() - long(ntop));
+        std::vector<std::vector<SingleGate>> alt(1, first);
+        bool ok = true;
+        for (size_t i = 0; i < later.size() && ok; i += size_t(kW13Slots))
+        {
+            std::vector<SingleGate> chunk(later.begin() + long(i),
+                                          later.begin() + long(std::min(later.size(), i + size_t(kW13Slots))));
+            std::vector<unsigned> ts;
+            for (const SingleGate &g : chunk)
+                ts.push_back(g.target);
+            ok = tile_wide_fits(prec, n, ts);
+            alt.push_back(chunk);
+        }
+        if (ok && alt.size() <= best.size())
+        {
+            best.swap(alt);
+            wide13.assign(best.size(), true);
+            wide13[0] = false;
+        }
+    }
+    bool rode = false;
     for (size_t k = 0; k < best.size(); k++)
     {
         const bool last = k + 1 == best.size();
-        ZXA_TRY(run_tile_pass(ctx, state, prec, n, best[k], last && ride ? win : nullptr, last && ride ? nwin : 0,
-                              geo));
+        if (wide13[k])
+        {
+            const bool take = last && ride && window_is_polynomial(n, win, nwin);
+            ZXA_TRY(run_wide13_pass(ctx, state, prec, n, best[k], take ? win : nullptr, take ? nwin : 0));
+            rode = rode || take;
+        }
+        else
+        {
+            ZXA_TRY(run_tile_pass(ctx, state, prec, n, best[k], last && ride ? win : nullptr,
+                                  last && ride ? nwin : 0, geo));
+            rode = rode || (last && ride);
+        }
     }
-    return ride ? OK : rest();
+    return rode ? OK : rest();
 }
 
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates,
@@ -2914,6 +3290,12 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
     }
     if (unsigned(high) > tile_upper_slots(prec, geometry))
     {
+        // the wide tile: nine targets above a 16-unit run
+        std::vector<unsigned> ts;
+        for (const SingleGate &g : gs)
+            ts.push_back(g.target);
+        if (!any_fan && tile_wide_fits(prec, n, ts) && (ndiag == 0 || window_is_polynomial(n, diag, ndiag)))
+            return run_wide13_pass(ctx, state, prec, n, gs, diag, ndiag);
         set_error("fused pass: more targets above the tile's low bits than the tile has slots");
         return EINVAL_;
     }
