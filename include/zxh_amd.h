@@ -195,6 +195,16 @@ int zxh_k_fused_pass(void *state, int precision, size_t local_bits, const zxh_u3
  * 10 dimensions or reach above nbits.  No reference counterpart. */
 int zxh_window_table(size_t nbits, const uint64_t *sel0, const uint64_t *sel1, const double *theta, size_t ngates,
                      uint64_t *basis, int *rank, double *table);
+/* How a batch of single-qubit gates on distinct amplitude address bits
+ * (targets[0 .. ntargets-1]) is split into tile passes when it needs more
+ * than one and the wide tile is on (knob tile.wide.<precision>), under tile
+ * geometry 3..6 (anything else: the knob tile.seg.<precision>): pass_of[i]
+ * is the pass of targets[i], pass 0 a 4096-unit tile pass, the others wide
+ * passes; *npasses = 0 when the batch fits one pass or no such split
+ * applies (pass_of untouched).  The engine and zxh_k_u3_batch split batches
+ * this way.  Host only; needs no device.  No reference counterpart. */
+int zxh_tile_layer_plan(int precision, size_t nbits, int geometry, const uint32_t *targets, size_t ntargets,
+                        uint32_t *pass_of, size_t *npasses);
 /* diag_pending_push_p / try_push_cp / flush (kernels.h:36-41) as one call:
  * ngates entries, sel1[i] == ~0 marks a P gate (Z/Rz already lowered to P). */
 int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,

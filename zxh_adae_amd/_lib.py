@@ -75,6 +75,8 @@ _SIGS = [
     ("zxh_k_diag_noalloc", C.c_int, [_vp, C.c_int, _sz, _u64p, _u64p, _dp, _sz]),
     ("zxh_k_fused_pass", C.c_int, [_vp, C.c_int, _sz, _vp, _sz, _u64p, _u64p, _dp, _sz]),
     ("zxh_window_table", C.c_int, [_sz, _u64p, _u64p, _dp, _sz, _u64p, C.POINTER(C.c_int), _dp]),
+    ("zxh_tile_layer_plan", C.c_int, [C.c_int, _sz, C.c_int, C.POINTER(C.c_uint32), _sz, C.POINTER(C.c_uint32),
+                                      C.POINTER(_sz)]),
     ("zxh_k_x", C.c_int, [_vp, C.c_int, _sz, _u64, _u64]),
     ("zxh_k_cx", C.c_int, [_vp, C.c_int, _sz, _u64, _u64, _u64]),
     ("zxh_k_cp", C.c_int, [_vp, C.c_int, _sz, _u64, _u64, C.c_double]),

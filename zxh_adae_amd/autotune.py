@@ -71,27 +71,44 @@ def tune_diag(state: "K.DeviceState", repeats: int = 3, batch: int = 64) -> Dict
 TILE_SEGS = (3, 4, 5, 6)
 
 
+def tile_layer_plan(n: int, targets, c64: bool, seg: int = -1):
+    """The passes the engine splits a batch of single-qubit gates on distinct
+    address bits into when it needs more than one tile pass and the wide
+    tile is on (zxh_tile_layer_plan): a list of target lists, the first for a
+    4096-unit tile pass, the others wide; [] when the batch fits one pass or
+    no such split applies.  Reads the tile.wide knob as it stands."""
+    ts = (C.c_uint32 * max(1, len(targets)))(*targets)
+    out = (C.c_uint32 * max(1, len(targets)))()
+    npass = C.c_size_t(0)
+    prec = _lib.ZXH_C64 if c64 else _lib.ZXH_C128
+    check(_lib.load().zxh_tile_layer_plan(prec, n, seg, ts, len(targets), out, C.byref(npass)))
+    passes = [[] for _ in range(npass.value)]
+    for i, t in enumerate(targets):
+        if npass.value:
+            passes[out[i]].append(int(t))
+    return passes
+
+
 def layer_passes(n: int, seg: int, c64: bool, wide: bool = False):
     """Targets of each tile pass of a full-width single-qubit layer under
     geometry seg: the low bits ride in the first pass, then 12 - seg upper
-    targets per pass.  With the wide (8192-unit) tile, as the engine forms
-    them: the first pass takes the low bits and the 12 - seg highest targets,
-    then passes of nine ascending targets, each above the 16-unit run (else
-    12 - seg)."""
+    targets per pass.  With the wide (8192-unit) tile, the split the engine
+    makes with its knob on (tile_layer_plan)."""
     low = seg + (1 if c64 else 0)
     slots = 12 - seg
     upper = list(range(low, n))
-    if not wide or len(upper) <= slots:
-        chunks = [upper[i:i + slots] for i in range(0, len(upper), slots)] or [[]]
-        chunks[0] = list(range(low)) + chunks[0]
-        return chunks
-    chunks = [list(range(low)) + upper[len(upper) - slots:]]
-    rest = upper[:len(upper) - slots]
-    i = 0
-    while i < len(rest):
-        take = 9 if rest[i] >= 4 + (1 if c64 else 0) else slots
-        chunks.append(rest[i:i + take])
-        i += take
+    if wide:
+        wknob = "tile.wide.c64" if c64 else "tile.wide.c128"
+        old = tune_get(wknob)
+        tune_set(wknob, 1)
+        try:
+            plan = tile_layer_plan(n, list(range(n)), c64, seg)
+        finally:
+            tune_set(wknob, old)
+        if plan:
+            return plan
+    chunks = [upper[i:i + slots] for i in range(0, len(upper), slots)] or [[]]
+    chunks[0] = list(range(low)) + chunks[0]
     return chunks
 
 

@@ -489,6 +489,27 @@ int zxh_window_table(size_t nbits, const uint64_t *sel0, const uint64_t *sel1, c
     return ZXH_OK;
 }
 
+int zxh_tile_layer_plan(int precision, size_t nbits, int geometry, const uint32_t *targets, size_t ntargets,
+                        uint32_t *pass_of, size_t *npasses)
+{
+    if ((ntargets && (!targets || !pass_of)) || !npasses)
+        return fail(ZXH_EINVAL, "null argument");
+    if (precision != ZXH_C128 && precision != ZXH_C64)
+        return fail(ZXH_EINVAL, "precision must be ZXH_C128 or ZXH_C64");
+    if (nbits > 40)
+        return fail(ZXH_EINVAL, "layer plan: state wider than 40 bits");
+    *npasses = 0;
+    const std::vector<unsigned> ts(targets, targets + ntargets);
+    std::vector<std::vector<unsigned>> plan;
+    if (!zxa::tile_layer_plan(precision, unsigned(nbits), geometry, ts, plan))
+        return ZXH_OK;
+    for (size_t k = 0; k < plan.size(); k++)
+        for (unsigned t : plan[k])
+            pass_of[size_t(std::find(ts.begin(), ts.end(), t) - ts.begin())] = uint32_t(k);
+    *npasses = plan.size();
+    return ZXH_OK;
+}
+
 int zxh_k_diag(void *state, int precision, size_t local_bits, const uint64_t *sel0, const uint64_t *sel1,
                const double *theta, size_t ngates)
 {

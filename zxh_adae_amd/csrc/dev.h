@@ -258,6 +258,15 @@ unsigned tile_min_bits(int prec);
 // 16-unit contiguous run, on a state of at least 13 unit bits.  Such a pass
 // moves 256-byte runs where the 4096-unit tile would move 128-byte ones.
 bool tile_wide_fits(int prec, unsigned n, const std::vector<unsigned> &targets);
+// The passes of a batch of single-bit gates on distinct targets that needs
+// more than one 4096-unit tile pass under geometry, with the wide tile on:
+// passes[0] for a 4096-unit tile pass (every target below tile_low_bits and
+// at most tile_upper_slots others), then as few wide passes as the batch
+// allows, each fitting tile_wide_fits.  The slots are spread over address
+// bits, not taken in runs of consecutive bits (which move the state slower).
+// False when the batch fits one pass or the wide tile does not apply.
+bool tile_layer_plan(int prec, unsigned n, int geometry, const std::vector<unsigned> &targets,
+                     std::vector<std::vector<unsigned>> &passes);
 // True when single-bit targets in this (circuit) order fit one staged tile
 // pass of a 2^n complex128 state: the order a pass with fans keeps.
 bool tile_staged_fits(int prec, unsigned n, const std::vector<unsigned> &targets, int geometry = -1);

@@ -526,21 +526,29 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
     // starts a new pass.
     const bool can_fan = prec_ == C128 && !disable_x_;
     // A batch that opens on an empty pass (no window: no fans) and needs more
-    // than one pass, with the wide tile on: the first pass takes the targets
-    // below the low bits and the highest ones (its runs are the longest the
-    // geometry makes, which the widest strides need most), the wide passes
-    // the rest in ascending order.  The gates act on distinct qubits of one
-    // batch, so any order is the same product.
+    // than one pass, with the wide tile on: the passes tile_layer_plan forms,
+    // each closed where the plan ends it.  The gates act on distinct qubits
+    // of one batch, so any order is the same product.
+    std::vector<size_t> breaks;
     if (pass_u3_.empty() && pass_diag_.empty() && descs.size() > tile_upper_slots(prec_) &&
         tune_get(prec_ == C64 ? "tile.wide.c64" : "tile.wide.c128") == 1)
     {
-        auto tgt = [](const U3Gate &g) { return unsigned(__builtin_ctzll(g.delta)); };
-        std::stable_sort(descs.begin(), descs.end(), [&](const U3Gate &a, const U3Gate &b) { return tgt(a) < tgt(b); });
-        const unsigned lowb = tile_low_bits(prec_);
-        const size_t nlow = size_t(std::count_if(descs.begin(), descs.end(),
-                                                 [&](const U3Gate &g) { return tgt(g) < lowb; }));
-        const size_t ntop = std::min(descs.size() - nlow, size_t(tile_upper_slots(prec_)));
-        std::rotate(descs.begin() + long(nlow), descs.end() - long(ntop), descs.end());
+        std::vector<unsigned> ts;
+        for (const U3Gate &g : descs)
+            ts.push_back(unsigned(__builtin_ctzll(g.delta)));
+        std::vector<std::vector<unsigned>> plan;
+        if (tile_layer_plan(prec_, n, -1, ts, plan))
+        {
+            std::vector<U3Gate> ordered;
+            for (const std::vector<unsigned> &ps : plan)
+            {
+                if (!ordered.empty())
+                    breaks.push_back(ordered.size());
+                for (unsigned t : ps)
+                    ordered.push_back(descs[size_t(std::find(ts.begin(), ts.end(), t) - ts.begin())]);
+            }
+            descs.swap(ordered);
+        }
     }
     auto order_with = [&](unsigned t) {
         std::vector<unsigned> v;
@@ -549,8 +557,11 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
         v.push_back(t);
         return v;
     };
-    for (const U3Gate &g0 : descs)
+    for (size_t i = 0; i < descs.size(); i++)
     {
+        if (std::binary_search(breaks.begin(), breaks.end(), i))
+            ZXA_TRY(flush_pass());
+        const U3Gate &g0 = descs[i];
         U3Gate g = g0;
         if (g.packed_sel >> 63)
         {

@@ -3015,6 +3015,91 @@ bool tile_wide_fits(int prec, unsigned n, const std::vector<unsigned> &targets)
     return true;
 }
 
+// Which address bits a pass's slots sit on matters more than its run length.
+// Nine slots on consecutive address bits that begin 8-12 unit bits above the
+// run (strides 64 KiB-1 MiB) move the state at about 51% of HBM; nine slots
+// two bits apart, anywhere, at 61-64%; nine consecutive slots right above the
+// run (one contiguous tile) at 70%.  So a batch that needs two or more wide
+// passes gives its nine lowest upper targets, when consecutive from the
+// bottom of the wide tile's slots, a wide pass of their own, and deals the
+// others out in turn, ascending, among the remaining wide passes and the
+// first pass.
+bool tile_layer_plan(int prec, unsigned n, int geometry, const std::vector<unsigned> &targets,
+                     std::vector<std::vector<unsigned>> &passes)
+{
+    passes.clear();
+    std::vector<unsigned> ts(targets);
+    std::sort(ts.begin(), ts.end());
+    if (std::adjacent_find(ts.begin(), ts.end()) != ts.end() || (!ts.empty() && ts.back() >= n))
+        return false;
+    const unsigned lowb = tile_low_bits(prec, geometry);
+    const size_t slots = tile_upper_slots(prec, geometry);
+    const unsigned wlow = 4 + (prec == C64 ? 1u : 0u);
+    std::vector<unsigned> first, pool;
+    size_t first_high = 0;
+    for (unsigned t : ts)
+    {
+        if (t < lowb)
+            first.push_back(t);
+        else if (t < wlow)
+        {
+            first.push_back(t);
+            first_high++;
+        }
+        else
+            pool.push_back(t);
+    }
+    const size_t nhigh = first_high + pool.size();
+    if (nhigh <= slots || first_high > slots)
+        return false;
+    const size_t W = size_t(kW13Slots);
+    const size_t nw = (nhigh - slots + W - 1) / W;
+    std::vector<std::vector<unsigned>> wide(nw);
+    size_t w0 = 0;
+    if (nw >= 2 && pool.size() >= W && pool[W - 1] == wlow + unsigned(W) - 1)
+    {
+        wide[0].assign(pool.begin(), pool.begin() + long(W));
+        pool.erase(pool.begin(), pool.begin() + long(W));
+        w0 = 1;
+    }
+    // in turn, in proportion to the room each pass has (the wide passes w0..,
+    // then the first pass): target j goes to the pass furthest behind its
+    // share (j + 1) cap / total, the earlier pass among equals
+    const size_t np = nw - w0 + 1;
+    std::vector<int64_t> cap(np, int64_t(W)), got(np, 0);
+    cap[np - 1] = int64_t(slots - first_high);
+    int64_t total = 0;
+    for (int64_t c : cap)
+        total += c;
+    if (total < int64_t(pool.size()))
+        return false;
+    for (size_t j = 0; j < pool.size(); j++)
+    {
+        size_t best = np;
+        int64_t lead = 0;
+        for (size_t p = 0; p < np; p++)
+        {
+            const int64_t d = cap[p] * int64_t(j + 1) - got[p] * total;
+            if (got[p] < cap[p] && (best == np || d > lead))
+            {
+                best = p;
+                lead = d;
+            }
+        }
+        got[best]++;
+        if (best + 1 < np)
+            wide[w0 + best].push_back(pool[j]);
+        else
+            first.push_back(pool[j]);
+    }
+    for (const std::vector<unsigned> &w : wide)
+        if (w.empty() || !tile_wide_fits(prec, n, w))
+            return false;
+    passes.push_back(first);
+    passes.insert(passes.end(), wide.begin(), wide.end());
+    return true;
+}
+
 unsigned tile_min_bits(int prec)
 {
     return unsigned(kUnitBits) + (prec == C64 ? 1u : 0u);
@@ -3195,31 +3280,23 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
     std::vector<bool> wide13(best.size(), false);
     if (wide.empty() && best.size() >= 2)
     {
-        // the first pass: the targets below the low bits and the highest
-        // ones (its runs are as long as the geometry makes them, which the
-        // widest strides need most); the wide passes: the rest, ascending
+        // the first pass as a 4096-unit tile pass, the others wide
+        // (tile_layer_plan)
         std::vector<SingleGate> all;
         for (const std::vector<SingleGate> &ps : best)
             all.insert(all.end(), ps.begin(), ps.end());
-        std::sort(all.begin(), all.end(), [](const SingleGate &a, const SingleGate &b) { return a.target < b.target; });
-        const unsigned lowb = tile_low_bits(prec, geo);
-        const size_t nlow = size_t(std::count_if(all.begin(), all.end(),
-                                                 [&](const SingleGate &g) { return g.target < lowb; }));
-        const size_t ntop = std::min(all.size() - nlow, size_t(tile_upper_slots(prec, geo)));
-        std::vector<SingleGate> first(all.begin(), all.begin() + long(nlow));
-        first.insert(first.end(), all.end() - long(ntop), all.end());
-        std::vector<SingleGate> later(all.begin() + long(nlow), all.end() - long(ntop));
-        std::vector<std::vector<SingleGate>> alt(1, first);
-        bool ok = true;
-        for (size_t i = 0; i < later.size() && ok; i += size_t(kW13Slots))
+        std::vector<unsigned> ts;
+        for (const SingleGate &g : all)
+            ts.push_back(g.target);
+        std::vector<std::vector<unsigned>> plan;
+        std::vector<std::vector<SingleGate>> alt;
+        const bool ok = tile_layer_plan(prec, n, geo, ts, plan);
+        for (size_t k = 0; ok && k < plan.size(); k++)
         {
-            std::vector<SingleGate> chunk(later.begin() + long(i),
-                                          later.begin() + long(std::min(later.size(), i + size_t(kW13Slots))));
-            std::vector<unsigned> ts;
-            for (const SingleGate &g : chunk)
-                ts.push_back(g.target);
-            ok = tile_wide_fits(prec, n, ts);
-            alt.push_back(chunk);
+            alt.emplace_back();
+            for (unsigned t : plan[k])
+                alt.back().push_back(*std::find_if(all.begin(), all.end(),
+                                                   [&](const SingleGate &g) { return g.target == t; }));
         }
         if (ok && alt.size() <= best.size())
         {
