@@ -1286,19 +1286,61 @@ struct TGrad
     double c[kSums];
 };
 
-// The three sums of one amplitude pair, accumulated.
+// The two off-diagonal sums of one amplitude pair (p0, p1), (l0, l1),
+// accumulated: S_e and S_f.
 template <typename R>
-__device__ __forceinline__ void pair_sums(R p0x, R p0y, R p1x, R p1y, R l0x, R l0y, R l1x, R l1y, R &sd, R &se,
-                                          R &sf)
+__device__ __forceinline__ void pair_cross(R p0x, R p0y, R p1x, R p1y, R l0x, R l0y, R l1x, R l1y, R &se, R &sf)
 {
-    sd += l0x * p0y - l0y * p0x - l1x * p1y + l1y * p1x;
     se += l0x * p1y - l0y * p1x + l1x * p0y - l1y * p0x;
     sf += l0x * p1x + l0y * p1y - l1x * p0x - l1y * p0y;
 }
 
+// Im(conj(l) p) of one amplitude.
+template <typename R>
+__device__ __forceinline__ R im_lp(R px, R py, R lx, R ly)
+{
+    return lx * py - ly * px;
+}
+
+// One halving step on lane bit 32 or 16 (halve's result) through the lane
+// exchange of two lists' words: lanes with the bit keep hi plus the partner's
+// hi, the others lo plus the partner's lo (the partner's copy added first
+// there; the sums are the same).
+template <int kLen, int kMask, typename R>
+__device__ __forceinline__ void halve_swap(const R (&x)[kLen], R (&y)[half_up(kLen)])
+{
+    static_assert(kMask == 32 || kMask == 16, "lane exchange steps");
+    constexpr int kW = sizeof(R) / 4;
+    struct Words
+    {
+        uint32_t w[kW];
+    };
+    constexpr int kH = half_up(kLen);
+#pragma unroll
+    for (int i = 0; i < kH; i++)
+    {
+        Words lo = __builtin_bit_cast(Words, x[i]);
+        Words hi = __builtin_bit_cast(Words, i + kH < kLen ? x[i + kH] : R(0));
+#pragma unroll
+        for (int k = 0; k < kW; k++)
+        {
+            const auto r = kMask == 32 ? __builtin_amdgcn_permlane32_swap(lo.w[k], hi.w[k], false, false)
+                                       : __builtin_amdgcn_permlane16_swap(lo.w[k], hi.w[k], false, false);
+            lo.w[k] = r[0];
+            hi.w[k] = r[1];
+        }
+        y[i] = __builtin_bit_cast(R, lo) + __builtin_bit_cast(R, hi);
+    }
+}
+
 // Slots of layout L (register bit i at list position 3 i; the complex64
-// in-unit slot of layout 0 at 12): sums, then M on both states; the list's
-// first two halving steps into rest.
+// in-unit slot of layout 0 at 12): every slot's sums on the state as it
+// enters the layout, then the slots' matrices M on both states; the list's
+// first two halving steps into rest.  A slot's sums do not depend on which
+// of the other slots' M were applied before them: each M acts on another
+// qubit, commutes with the slot's operator and is the same unitary on both
+// states.  So S_d, a signed sum of Im(conj(l) p) over the registers by the
+// slot's bit, shares one product per register among the slots.
 template <bool kC64, int L>
 __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
                                            ZXA_CONST TTab *G, Real<kC64> (&rest)[group_rest(kC64, L)], int lane)
@@ -1306,28 +1348,77 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
     using R = Real<kC64>;
     constexpr int kN = group_len(kC64, L);
     R s[kN];
+    // S_e, S_f per slot
 #pragma unroll
     for (int i = 0; i < 4; i++)
     {
-        const int k = reg_bit(L, i);
-        R sd = 0, se = 0, sf = 0;
+        R se = 0, sf = 0;
 #pragma unroll
         for (int r = 0; r < kRegs; r++)
         {
             if ((r >> i) & 1)
                 continue;
             const int r1 = r | (1 << i);
-            pair_sums<R>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x, w[r1].y, sd, se, sf);
+            pair_cross<R>(v[r].x, v[r].y, v[r1].x, v[r1].y, w[r].x, w[r].y, w[r1].x, w[r1].y, se, sf);
             if constexpr (kC64)
-                pair_sums<float>(v[r].z, v[r].w, v[r1].z, v[r1].w, w[r].z, w[r].w, w[r1].z, w[r1].w, sd, se, sf);
+                pair_cross<float>(v[r].z, v[r].w, v[r1].z, v[r1].w, w[r].z, w[r].w, w[r1].z, w[r1].w, se, sf);
         }
-        // the sums read the slot's inputs: formed here (not sunk towards
-        // their use after the stores, which keeps old copies of the states)
-        asm volatile("" : "+v"(sd), "+v"(se), "+v"(sf));
-        __builtin_amdgcn_sched_barrier(0);
-        s[i * kSums + 0] = sd;
         s[i * kSums + 1] = se;
         s[i * kSums + 2] = sf;
+    }
+    // S_d per slot: a butterfly over the registers' Im(conj(l) p), bit 0
+    // first (d: pair sums, e: pair differences)
+    R d[kRegs];
+    R dlo = 0, dhi = 0; // complex64: each amplitude's part, for the in-unit slot
+#pragma unroll
+    for (int r = 0; r < kRegs; r++)
+    {
+        d[r] = im_lp<R>(v[r].x, v[r].y, w[r].x, w[r].y);
+        if constexpr (kC64)
+        {
+            const float dh = im_lp<float>(v[r].z, v[r].w, w[r].z, w[r].w);
+            if constexpr (L == 0)
+            {
+                dlo += d[r];
+                dhi += dh;
+            }
+            d[r] += dh;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        const int len = kRegs >> i;
+        R e = 0;
+#pragma unroll
+        for (int r = 0; r < len; r += 2)
+        {
+            e += d[r] - d[r + 1];
+            d[r >> 1] = d[r] + d[r + 1];
+        }
+        s[i * kSums + 0] = e;
+    }
+    if constexpr (kC64 && L == 0)
+    {
+        // the in-unit slot: pairs (first, second amplitude) of every unit
+        R se = 0, sf = 0;
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+            pair_cross<float>(v[r].x, v[r].y, v[r].z, v[r].w, w[r].x, w[r].y, w[r].z, w[r].w, se, sf);
+        s[4 * kSums + 0] = dlo - dhi;
+        s[4 * kSums + 1] = se;
+        s[4 * kSums + 2] = sf;
+    }
+    // the sums read the layout's inputs: formed here (not sunk towards their
+    // use after the matrices, which would keep old copies of the states)
+#pragma unroll
+    for (int j = 0; j < kN; j++)
+        asm volatile("" : "+v"(s[j]));
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+    {
+        const int k = reg_bit(L, i);
         R m[8];
         load_matrix<kC64>(m, G->g[k]);
         if constexpr (kC64)
@@ -1358,15 +1449,6 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
     }
     if constexpr (kC64 && L == 0)
     {
-        float sd = 0, se = 0, sf = 0;
-#pragma unroll
-        for (int r = 0; r < kRegs; r++)
-            pair_sums<float>(v[r].x, v[r].y, v[r].z, v[r].w, w[r].x, w[r].y, w[r].z, w[r].w, sd, se, sf);
-        asm volatile("" : "+v"(sd), "+v"(se), "+v"(sf));
-        __builtin_amdgcn_sched_barrier(0);
-        s[4 * kSums + 0] = sd;
-        s[4 * kSums + 1] = se;
-        s[4 * kSums + 2] = sf;
         float m[8];
         load_matrix<true>(m, G->g[kIntra]);
         const MRow mr = mfma_row(m);
@@ -1378,8 +1460,8 @@ __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], 
         }
     }
     R h[half_up(kN)];
-    halve<kN, 32, R>(s, h, lane);
-    halve<half_up(kN), 16, R>(h, rest, lane);
+    halve_swap<kN, 32, R>(s, h);
+    halve_swap<half_up(kN), 16, R>(h, rest);
 #pragma unroll
     for (int i = 0; i < group_rest(kC64, L); i++)
         asm volatile("" : "+v"(rest[i]));
@@ -1461,9 +1543,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     T *pp = static_cast<T *>(psi);
     T *pl = static_cast<T *>(lam);
     const uint64_t toff = thr_off<0>(t, P);
-    Epi<kC64> epi;
-    if constexpr (kDiag)
-        epi.init(ttab, t);
     double acc = 0.0;
     uint64_t base = deposit(uint64_t(blockIdx.x), P.sorted);
     for (uint64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x)
@@ -1475,7 +1554,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         AdjRest<kC64> R;
         adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
         if constexpr (kDiag)
+        {
+            // the per-thread factors (a few KiB, cached) read again per tile
+            // rather than held across it: the sums and both states fill the
+            // registers
+            const double *et = ttab;
+            asm volatile("" : "+s"(et));
+            Epi<kC64> epi;
+            epi.init(et, t);
             epi.template apply_n<true>(v, w, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), t);
+        }
         store_tile<kC64>(v, pp, base ^ toff, G);
         store_tile<kC64>(w, pl, base ^ toff, G);
         acc += R.total(t & 63);
@@ -2914,9 +3002,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         AdjRest<kC64> R;
         adj_tile<kC64, kL1, kL2>(v, w, lds, G, t, R);
         // unconditional (identity tables where the item has no window): a
-        // branch around the epilogue made the compiler spill about 90 VGPRs
+        // branch around the epilogue made the compiler spill about 90 VGPRs.
+        // The per-thread factors are read here, not held across the tile.
+        const double *et = reinterpret_cast<const double *>(I.ttab);
+        asm volatile("" : "+s"(et));
         Epi<kC64> epi;
-        epi.init(reinterpret_cast<const double *>(I.ttab), t);
+        epi.init(et, t);
         epi.template apply_n<true>(
             v, w, (ZXA_CONST double *)(reinterpret_cast<const double *>(I.tphase) + local * I.tstride), t);
         store_tile<kC64>(v, pp, base ^ toff, G);
