@@ -200,8 +200,9 @@ int zxh_window_table(size_t nbits, const uint64_t *sel0, const uint64_t *sel1, c
  * than one and the wide tile is on (knob tile.wide.<precision>), under tile
  * geometry 3..6 (anything else: the knob tile.seg.<precision>): pass_of[i]
  * is the pass of targets[i], pass 0 a 4096-unit tile pass, the others wide
- * passes; *npasses = 0 when the batch fits one pass or no such split
- * applies (pass_of untouched).  The engine and zxh_k_u3_batch split batches
+ * passes (every pass wide when no target lies below the wide tile's 16-unit
+ * run and that takes fewer passes); *npasses = 0 when the batch fits one
+ * 4096-unit tile pass or no such split applies (pass_of untouched).  The engine and zxh_k_u3_batch split batches
  * this way.  Host only; needs no device.  No reference counterpart. */
 int zxh_tile_layer_plan(int precision, size_t nbits, int geometry, const uint32_t *targets, size_t ntargets,
                         uint32_t *pass_of, size_t *npasses);

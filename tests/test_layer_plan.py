@@ -34,17 +34,25 @@ def wide_on():
 def check_plan(n, targets, c64, seg, passes):
     shift = 1 if c64 else 0
     lowb = seg + shift
+    wlow = 4 + shift
     slots = 12 - seg
     flat = sorted(t for p in passes for t in p)
     assert flat == sorted(targets)
-    first = passes[0]
-    assert set(t for t in targets if t < lowb) <= set(first)
-    assert sum(1 for t in first if t >= lowb) <= slots
-    for p in passes[1:]:
-        assert 1 <= len(p) <= 9
-        assert min(p) >= 4 + shift
     nhigh = sum(1 for t in targets if t >= lowb)
-    assert len(passes) == 1 + -(-(nhigh - slots) // 9)
+    with_first = 1 + -(-(nhigh - slots) // 9)
+    if min(targets) >= wlow and -(-len(targets) // 9) < with_first:
+        # no target below the wide run: wide passes only, fewer of them
+        assert len(passes) == -(-len(targets) // 9)
+        wide = passes
+    else:
+        first = passes[0]
+        assert set(t for t in targets if t < lowb) <= set(first)
+        assert sum(1 for t in first if t >= lowb) <= slots
+        assert len(passes) == with_first
+        wide = passes[1:]
+    for p in wide:
+        assert 1 <= len(p) <= 9
+        assert min(p) >= wlow
 
 
 @pytest.mark.parametrize("c64", [False, True])
@@ -96,6 +104,14 @@ def test_random_batches(wide_on, seed):
             assert passes == []
             continue
         check_plan(n, targets, c64, seg, passes)
+
+
+def test_wide_only_batches(wide_on):
+    # nine targets above the run: one wide pass; eighteen: two
+    assert plan(30, range(4, 13)) == [list(range(4, 13))]
+    assert plan(30, range(12, 30)) == [list(range(12, 29, 2)), list(range(13, 30, 2))]
+    # a target below the run needs the 4096-unit first pass
+    assert len(plan(30, [0] + list(range(4, 13)))) == 2
 
 
 def test_no_plan_cases(wide_on):

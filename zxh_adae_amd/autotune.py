@@ -75,8 +75,10 @@ def tile_layer_plan(n: int, targets, c64: bool, seg: int = -1):
     """The passes the engine splits a batch of single-qubit gates on distinct
     address bits into when it needs more than one tile pass and the wide
     tile is on (zxh_tile_layer_plan): a list of target lists, the first for a
-    4096-unit tile pass, the others wide; [] when the batch fits one pass or
-    no such split applies.  Reads the tile.wide knob as it stands."""
+    4096-unit tile pass, the others wide (all wide when no target lies below
+    the wide tile's run and that takes fewer passes); [] when the batch fits
+    one 4096-unit pass or no such split applies.  Reads the tile.wide knob as
+    it stands."""
     ts = (C.c_uint32 * max(1, len(targets)))(*targets)
     out = (C.c_uint32 * max(1, len(targets)))()
     npass = C.c_size_t(0)

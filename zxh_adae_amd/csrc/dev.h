@@ -262,7 +262,8 @@ bool tile_wide_fits(int prec, unsigned n, const std::vector<unsigned> &targets);
 // more than one 4096-unit tile pass under geometry, with the wide tile on:
 // passes[0] for a 4096-unit tile pass (every target below tile_low_bits and
 // at most tile_upper_slots others), then as few wide passes as the batch
-// allows, each fitting tile_wide_fits.  The slots are spread over address
+// allows, each fitting tile_wide_fits; when no target lies below the wide
+// tile's run and wide passes alone take fewer passes, wide passes only.  The slots are spread over address
 // bits, not taken in runs of consecutive bits (which move the state slower).
 // False when the batch fits one pass or the wide tile does not apply.
 bool tile_layer_plan(int prec, unsigned n, int geometry, const std::vector<unsigned> &targets,
@@ -303,9 +304,12 @@ struct GateFan
 // then a diagonal window (a phase polynomial, or tabulable and without
 // fans), in one sweep.  Without fans the gates must commute; with fans (complex128, every
 // target at or above tile_low_bits) they are applied in the given order,
-// each after its fan.
+// each after its fan.  fresh: the state is |0...0> of 2^nbits amplitudes but
+// not yet written (a reset left pending); the pass then forms its input
+// instead of reading it where it can, and resets the state first where not.
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned nbits, const U3Gate *gates, size_t ngates,
-                     const DiagGate *diag, size_t ndiag, int geometry = -1, const GateFan *fans = nullptr);
+                     const DiagGate *diag, size_t ndiag, int geometry = -1, const GateFan *fans = nullptr,
+                     bool fresh = false);
 // One target of an adjoint pass: the uncompute matrix m (= U^dagger of the
 // forward pass's fused single-qubit gate on this qubit, (re, im) x 4), up to
 // two transition operators a[k] (2x2 traceless Hermitian; zero when unused)

@@ -295,7 +295,22 @@ int Sim::resize_reset(size_t M)
     }
     M_ = M;
     used_bits_ = 0;
+    if (policy_ == POLICY_FUSED)
+    {
+        fresh_ = true;
+        fresh_elems_ = cap;
+        return OK;
+    }
+    fresh_ = false;
     return state_reset(ctx_, dstate_, prec_, cap);
+}
+
+int Sim::settle()
+{
+    if (!fresh_)
+        return OK;
+    fresh_ = false;
+    return state_reset(ctx_, dstate_, prec_, fresh_elems_);
 }
 
 int Sim::expand()
@@ -360,6 +375,8 @@ int Sim::flush_diag(std::vector<DiagGate> &window)
     {
         rc = flush_pass();
         if (!rc)
+            rc = settle();
+        if (!rc)
             rc = diag_apply(ctx_, dstate_, prec_, active_bits(), window.data(), window.size());
     }
     window.clear();
@@ -376,6 +393,7 @@ int Sim::flush_group()
 {
     if (exp_group_.empty())
         return OK;
+    ZXA_TRY(settle());
     const int rc = u3_expand_group_apply(ctx_, dstate_, prec_, exp_base_, exp_group_.data(), exp_group_.size());
     exp_group_.clear();
     return rc;
@@ -388,7 +406,9 @@ int Sim::flush_wide(std::vector<DiagGate> *window)
 {
     std::vector<U3Gate> descs;
     descs.swap(wide_u3_);
-    int rc = OK;
+    int rc = descs.empty() && !(window && !window->empty()) ? OK : settle();
+    if (rc)
+        return rc;
     if (window && !window->empty())
     {
         rc = u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size(), window->data(), window->size());
@@ -405,9 +425,16 @@ int Sim::flush_pass()
     if (pass_u3_.empty() && pass_diag_.empty())
         return OK;
     ZXA_TRY(flush_group());
+    // the first pass after the reset forms |0...0> itself when it covers the
+    // whole reset state and has no fans (it resets the state where its kernel
+    // cannot)
+    const bool fresh = fresh_ && !pass_has_fan_ && (size_t(1) << active_bits()) == fresh_elems_;
+    if (!fresh)
+        ZXA_TRY(settle());
+    fresh_ = false;
     int rc = fused_pass_apply(ctx_, dstate_, prec_, active_bits(), pass_u3_.data(), pass_u3_.size(),
                               pass_diag_.data(), pass_diag_.size(), -1,
-                              pass_has_fan_ ? pass_fan_.data() : nullptr);
+                              pass_has_fan_ ? pass_fan_.data() : nullptr, fresh);
     pass_u3_.clear();
     pass_fan_.clear();
     pass_diag_.clear();
@@ -514,6 +541,7 @@ int Sim::emit_u3(std::vector<U3Gate> &descs)
             wide_u3_ = descs;
             return OK;
         }
+        ZXA_TRY(settle());
         return u3_apply(ctx_, dstate_, prec_, active_bits(), descs.data(), descs.size());
     }
     ZXA_TRY(flush_wide());
@@ -878,6 +906,7 @@ int Sim::execute()
                     continue;
                 }
                 ZXA_TRY(flush_group());
+                ZXA_TRY(settle());
                 if (top)
                     ZXA_TRY(u3_expand_apply(ctx_, dstate_, prec_, active_bits(), eg));
                 else
@@ -927,6 +956,7 @@ int Sim::execute()
                 if (rc)
                     return rc;
                 ZXA_TRY(flush_group());
+                ZXA_TRY(settle());
                 ZXA_TRY(x_apply(ctx_, dstate_, prec_, active_bits(), d & low_mask(active_bits()),
                                 local_selector(g.q)));
             }
@@ -943,6 +973,7 @@ int Sim::execute()
                 if (rc)
                     return rc;
                 ZXA_TRY(flush_group());
+                ZXA_TRY(settle());
                 ZXA_TRY(cx_apply(ctx_, dstate_, prec_, active_bits(), d & low_mask(active_bits()),
                                  local_selector(g.cq), local_selector(g.q)));
             }
@@ -965,6 +996,7 @@ int Sim::execute()
                 ZXA_TRY(flush_all());
                 ZXA_TRY(flush_pass());
                 ZXA_TRY(flush_group());
+                ZXA_TRY(settle());
                 ZXA_TRY(cp_apply(ctx_, dstate_, prec_, active_bits(), scq, sq, g.theta));
             }
             else
@@ -978,6 +1010,7 @@ int Sim::execute()
     ZXA_TRY(flush_all());
     ZXA_TRY(flush_pass());
     ZXA_TRY(flush_group());
+    ZXA_TRY(settle());
     ZXA_HIP(hipStreamSynchronize(ctx_.stream));
     executed_ = true;
     return OK;

@@ -133,6 +133,10 @@ class Sim
     int flush_pass();
     bool fusable() const;
     int resize_reset(size_t M);
+    // The reset of execute() is left pending under the fused policy: the
+    // first tile pass forms |0...0> itself (fused_pass_apply's fresh input);
+    // any other first use of the state runs the reset (settle).
+    int settle();
 
     size_t N_;
     bool disable_x_, eager_;
@@ -147,6 +151,8 @@ class Sim
     size_t m_ = 0;
     // physical state
     void *dstate_ = nullptr;
+    bool fresh_ = false;      // the state is |0...0> of fresh_elems_ amplitudes, not yet written
+    size_t fresh_elems_ = 0;
     size_t alloc_elems_ = 0;
     size_t M_ = 0;
     size_t used_bits_ = 0;

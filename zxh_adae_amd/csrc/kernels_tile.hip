@@ -1759,6 +1759,52 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t tiles)
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
 }
 
+// Fresh tile pass: the first pass over a state just reset to |0...0>
+// (amplitude 0 is 1, every other 0).  Each tile's input is formed in
+// registers instead of read: the reset's sweep and the pass's read merge into
+// the pass's store, and every amplitude still goes through the pass's gates,
+// transposes and window before it is stored.  Stores only, so no prefetch.
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kTab>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void tile_fresh_kernel(
+    void *__restrict__ state, const TPass P, const TTab *__restrict__ tabp, const double *__restrict__ ttab,
+    const double *__restrict__ tphase)
+{
+    static_assert(!(kTab && kDiag), "one epilogue kind");
+    using T = typename UnitT<kC64>::T;
+    __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
+    __shared__ WinT<kC64> wt[kTab ? (1 << kWinBits) : 1];
+    const int t = threadIdx.x;
+    T *p = static_cast<T *>(state);
+    const uint64_t toff = thr_off<0>(t, P);
+    Epi<kC64> epi;
+    if constexpr (kDiag)
+        epi.init(ttab, t);
+    uint32_t yt = 0;
+    if constexpr (kTab)
+    {
+        ZXA_CONST TTab *G0 = tables(tabp);
+        win_load<kC64>(wt, ttab, G0->win, t);
+        yt = win_y(G0->win, toff);
+        __syncthreads();
+    }
+    T one{}, zero{};
+    one.x = 1;
+    uint64_t base = deposit(uint64_t(blockIdx.x), P.sorted);
+    for (uint64_t tile = blockIdx.x; tile < P.ntiles; tile += gridDim.x)
+    {
+        ZXA_CONST TTab *G = tables(tabp);
+        const uint64_t at = base ^ toff;
+        T v[kRegs];
+#pragma unroll
+        for (int r = 0; r < kRegs; r++)
+            v[r] = (at ^ G->roff[r]) == 0 ? one : zero;
+        tile_work<kC64, kL1, kL2, kDiag, false, false, kTab>(v, lds, P, G, tphase, epi, t, toff, tile, base, nullptr,
+                                                            wt, yt);
+        store_tile<kC64>(v, p, at, G);
+        base = next_base(base, P);
+    }
+}
+
 template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kFlip, bool kFan = false, bool kTab = false>
 int launch_t(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const double *ttab, const double *tphase)
 {
@@ -1828,6 +1874,42 @@ int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip,
     if (flip)
         return launch_l<kC64, false, true>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
     return launch_l<kC64, false, false>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
+}
+
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kTab>
+int launch_f(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const double *ttab, const double *tphase)
+{
+    auto k = tile_fresh_kernel<kC64, kL1, kL2, kDiag, kTab>;
+    const int grid = persistent_grid(ctx, k, P0.ntiles);
+    TPass P = P0;
+    P.gstep = deposit(uint64_t(grid), P.sorted);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, ctx.stream, state, P, tab, ttab, tphase);
+    return hip_check(hipGetLastError(), "tile_fresh_kernel launch");
+}
+
+template <bool kC64, bool kDiag, bool kTab>
+int launch_fl(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, const TTab *tab, const double *ttab,
+              const double *tphase)
+{
+    if (l1 && l2)
+        return launch_f<kC64, true, true, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
+    if (l1)
+        return launch_f<kC64, true, false, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
+    if (l2)
+        return launch_f<kC64, false, true, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
+    return launch_f<kC64, false, false, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
+}
+
+// A fresh pass (no per-pair selectors, no fans): win as in launch_p.
+template <bool kC64>
+int launch_fresh(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, const TTab *tab, const double *ttab,
+                 const double *tphase, bool win)
+{
+    if (win)
+        return launch_fl<kC64, false, true>(ctx, state, P, l1, l2, tab, ttab, nullptr);
+    if (ttab)
+        return launch_fl<kC64, true, false>(ctx, state, P, l1, l2, tab, ttab, tphase);
+    return launch_fl<kC64, false, false>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
 }
 
 // One upper_kernel workgroup per tile (ttab and tphase set: the diagonal
@@ -2464,9 +2546,12 @@ int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vect
 }
 
 // One tile pass over the given targets, then the diagonal window (if any).
+// fresh: the state is the reset |0...0> of 2^n amplitudes, not yet written
+// (see tile_fresh_kernel; passes with per-pair selectors or fans reset it
+// first).
 int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
                   const DiagGate *diag = nullptr, size_t ndiag = 0, int geometry = -1,
-                  const GateFan *fans = nullptr)
+                  const GateFan *fans = nullptr, bool fresh = false)
 {
     const bool c64 = prec == C64;
     bool any_fan = false;
@@ -2650,7 +2735,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     // is at least tile.upper.<prec>: spread over more than 2^k units, a tile's
     // runs are fetched faster with more tiles in flight per CU than the LDS
     // allows the tile kernel
-    bool upper = !any_fan && P.ntiles <= uint64_t(INT_MAX);
+    bool upper = !fresh && !any_fan && P.ntiles <= uint64_t(INT_MAX);
     unsigned high = 0;
     for (size_t gi = 0; gi < gs.size() && upper; gi++)
     {
@@ -2699,9 +2784,20 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     const TTab *tab = static_cast<const TTab *>(dtab);
-    const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
+    const bool fresh_k = fresh && !flip && !any_fan;
+    if (fresh && !fresh_k)
+        ZXA_TRY(state_reset(ctx, state, prec, size_t(1) << n));
+    // algorithmic bytes: the state read and written, or written only
+    const uint64_t bytes = (fresh_k ? 1ull : 2ull) * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
-    if (upper)
+    if (fresh_k)
+    {
+        if (c64)
+            ZXA_TRY(launch_fresh<true>(ctx, state, P, use[1], use[2], tab, ttab, tphase, win));
+        else
+            ZXA_TRY(launch_fresh<false>(ctx, state, P, use[1], use[2], tab, ttab, tphase, win));
+    }
+    else if (upper)
         ZXA_TRY(launch_u(ctx, c64, state, P, tab, ttab, tphase, flip, win));
     else if (c64)
         ZXA_TRY(launch_p<true>(ctx, state, P, use[1], use[2], flip, tab, ttab, tphase, false, win));
@@ -3144,7 +3240,17 @@ bool tile_layer_plan(int prec, unsigned n, int geometry, const std::vector<unsig
     if (nhigh <= slots || first_high > slots)
         return false;
     const size_t W = size_t(kW13Slots);
-    const size_t nw = (nhigh - slots + W - 1) / W;
+    size_t nw = (nhigh - slots + W - 1) / W;
+    // no target below the wide tile's run: wide passes alone when they take
+    // fewer passes (nine targets above the run are one wide pass)
+    const bool wide_only = !ts.empty() && ts.front() >= wlow && (ts.size() + W - 1) / W < 1 + nw;
+    if (wide_only)
+    {
+        nw = (ts.size() + W - 1) / W;
+        pool = ts;
+        first.clear();
+        first_high = 0;
+    }
     std::vector<std::vector<unsigned>> wide(nw);
     size_t w0 = 0;
     if (nw >= 2 && pool.size() >= W && pool[W - 1] == wlow + unsigned(W) - 1)
@@ -3156,9 +3262,10 @@ bool tile_layer_plan(int prec, unsigned n, int geometry, const std::vector<unsig
     // in turn, in proportion to the room each pass has (the wide passes w0..,
     // then the first pass): target j goes to the pass furthest behind its
     // share (j + 1) cap / total, the earlier pass among equals
-    const size_t np = nw - w0 + 1;
+    const size_t np = nw - w0 + (wide_only ? 0 : 1);
     std::vector<int64_t> cap(np, int64_t(W)), got(np, 0);
-    cap[np - 1] = int64_t(slots - first_high);
+    if (!wide_only)
+        cap[np - 1] = int64_t(slots - first_high);
     int64_t total = 0;
     for (int64_t c : cap)
         total += c;
@@ -3178,7 +3285,7 @@ bool tile_layer_plan(int prec, unsigned n, int geometry, const std::vector<unsig
             }
         }
         got[best]++;
-        if (best + 1 < np)
+        if (wide_only || best + 1 < np)
             wide[w0 + best].push_back(pool[j]);
         else
             first.push_back(pool[j]);
@@ -3186,7 +3293,8 @@ bool tile_layer_plan(int prec, unsigned n, int geometry, const std::vector<unsig
     for (const std::vector<unsigned> &w : wide)
         if (w.empty() || !tile_wide_fits(prec, n, w))
             return false;
-    passes.push_back(first);
+    if (!wide_only)
+        passes.push_back(first);
     passes.insert(passes.end(), wide.begin(), wide.end());
     return true;
 }
@@ -3392,8 +3500,10 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
         if (ok && alt.size() <= best.size())
         {
             best.swap(alt);
-            wide13.assign(best.size(), true);
-            wide13[0] = false;
+            // the wide passes (every pass but a 4096-unit first pass)
+            wide13.assign(best.size(), false);
+            for (size_t k = 0; k < plan.size(); k++)
+                wide13[k] = tile_wide_fits(prec, n, plan[k]);
         }
     }
     bool rode = false;
@@ -3417,7 +3527,7 @@ int u3_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, s
 }
 
 int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *gates, size_t ngates,
-                     const DiagGate *diag, size_t ndiag, int geometry, const GateFan *fans)
+                     const DiagGate *diag, size_t ndiag, int geometry, const GateFan *fans, bool fresh)
 {
     if (n < tile_min_bits(prec) || n > 40)
     {
@@ -3463,13 +3573,21 @@ int fused_pass_apply(Ctx &ctx, void *state, int prec, unsigned n, const U3Gate *
         for (const SingleGate &g : gs)
             ts.push_back(g.target);
         if (!any_fan && tile_wide_fits(prec, n, ts) && (ndiag == 0 || window_is_polynomial(n, diag, ndiag)))
+        {
+            if (fresh)
+                ZXA_TRY(state_reset(ctx, state, prec, size_t(1) << n));
             return run_wide13_pass(ctx, state, prec, n, gs, diag, ndiag);
+        }
         set_error("fused pass: more targets above the tile's low bits than the tile has slots");
         return EINVAL_;
     }
     if (gs.empty())
+    {
+        if (fresh)
+            ZXA_TRY(state_reset(ctx, state, prec, size_t(1) << n));
         return diag_apply(ctx, state, prec, n, diag, ndiag);
-    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag, geometry, fans);
+    }
+    return run_tile_pass(ctx, state, prec, n, gs, diag, ndiag, geometry, fans, fresh);
 }
 
 } // namespace zxa
