@@ -5,8 +5,11 @@ target followed by CP(pi/2^k) to all lower qubits, no swaps: 30 H + 435 CP)
 in complex128 through the engine's drop-in ZXH object with the default fused
 policy: the reference's own U3 clusters and diagonal windows, merged into
 tile passes where they commute.  The state is resident in HBM; execute()
-includes the reset of the 16 GiB state, as the reference's execute() does.
-value = gates / second of the timed steps.
+starts from the reset state |0...0> of 16 GiB, as the reference's execute()
+does: the fused policy merges the reset into the first tile pass, which forms
+that state in registers instead of reading it (every amplitude still goes
+through the pass's gates and is stored).  value = gates / second of the timed
+steps.
 
 Reported beside it (same JSON line):
   roofline      dominant kernel of the step: algorithmic bytes per launch /
