@@ -1182,7 +1182,7 @@ __global__ __launch_bounds__(kW13Threads) __attribute__((amdgpu_waves_per_eu(2))
 //     S_f = Re(conj(l0) p1) - Re(conj(l1) p0)
 // so the kernel accumulates the three sums of every slot (whatever the
 // operators), and the collection combines them with each operator's (a, b).
-// After a slot's sums the pass applies the slot's uncompute matrix
+// After a layout's sums the pass applies its slots' uncompute matrices
 // M = U^dagger to both states.  The diagonal epilogue (if any) is the
 // previous forward pass's window with negated angles, on both states.
 // Per tile the sums are reduced over the wave by recursive halving: a step
@@ -1343,7 +1343,7 @@ __device__ __forceinline__ void halve_swap(const R (&x)[kLen], R (&y)[half_up(kL
 // slot's bit, shares one product per register among the slots.
 template <bool kC64, int L>
 __device__ __forceinline__ void adj_layout(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
-                                           ZXA_CONST TTab *G, Real<kC64> (&rest)[group_rest(kC64, L)], int lane)
+                                           ZXA_CONST TTab *G, Real<kC64> (&rest)[group_rest(kC64, L)])
 {
     using R = Real<kC64>;
     constexpr int kN = group_len(kC64, L);
@@ -1494,24 +1494,23 @@ template <bool kC64, bool kL1, bool kL2>
 __device__ __forceinline__ void adj_tile(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T (&w)[kRegs],
                                          typename UnitT<kC64>::T *lds, ZXA_CONST TTab *G, int t, AdjRest<kC64> &R)
 {
-    const int lane = t & 63;
 #pragma unroll
     for (int i = 0; i < group_rest(kC64, 1); i++)
         R.r1[i] = 0;
 #pragma unroll
     for (int i = 0; i < group_rest(kC64, 2); i++)
         R.r2[i] = 0;
-    adj_layout<kC64, 0>(v, w, G, R.r0, lane);
+    adj_layout<kC64, 0>(v, w, G, R.r0);
     if constexpr (kL1)
     {
         transpose<kC64, 0, 1>(v, lds, t);
         transpose<kC64, 0, 1>(w, lds, t);
-        adj_layout<kC64, 1>(v, w, G, R.r1, lane);
+        adj_layout<kC64, 1>(v, w, G, R.r1);
         if constexpr (kL2)
         {
             transpose<kC64, 1, 2>(v, lds, t);
             transpose<kC64, 1, 2>(w, lds, t);
-            adj_layout<kC64, 2>(v, w, G, R.r2, lane);
+            adj_layout<kC64, 2>(v, w, G, R.r2);
             transpose<kC64, 2, 0>(v, lds, t);
             transpose<kC64, 2, 0>(w, lds, t);
         }
@@ -1525,7 +1524,7 @@ __device__ __forceinline__ void adj_tile(typename UnitT<kC64>::T (&v)[kRegs], ty
     {
         transpose<kC64, 0, 2>(v, lds, t);
         transpose<kC64, 0, 2>(w, lds, t);
-        adj_layout<kC64, 2>(v, w, G, R.r2, lane);
+        adj_layout<kC64, 2>(v, w, G, R.r2);
         transpose<kC64, 2, 0>(v, lds, t);
         transpose<kC64, 2, 0>(w, lds, t);
     }
