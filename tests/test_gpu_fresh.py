@@ -130,3 +130,34 @@ def test_sampling_after_a_fresh_first_pass(gpu_engine):
     real = s.sample_real(4096)
     want = O.oracle_sample_state(s.phys_state(), min(n, 18), O.mix_seed(77, 0), 4096)
     assert np.array_equal(real, want)
+
+
+def textbook_qft(n, seed=None):
+    """The QFT in the reference's qft workload order (for t from the top down:
+    the controlled phases onto t, then H(t), or a random U3 with a seed): the
+    phases before a gate on t become that gate's fan."""
+    rng = np.random.default_rng(seed) if seed is not None else None
+    gs = []
+    for t in range(n - 1, -1, -1):
+        for k in range(n - 1, t, -1):
+            gs.append((CP, k, t, np.pi / 2 ** (k - t), 0.0, 0.0))
+        if rng is None:
+            gs.append((H, 0, t, 0.0, 0.0, 0.0))
+        else:
+            gs.append((U3, 0, t, *(float(x) for x in rng.uniform(-np.pi, np.pi, 3))))
+    return gs
+
+
+@pytest.mark.parametrize("n,seed", [(16, None), (19, 3), (22, None)])
+def test_first_pass_with_fans(gpu_engine, n, seed):
+    """A first pass whose gates carry fans (complex128): fresh, against the
+    oracle and the reference policy."""
+    Z = gpu_engine
+    gates = textbook_qft(n, seed)
+    want = oracle_state(n, gates)
+    s = run(Z, n, gates, "c128", timing=True)
+    assert s.timing("reset")["launches"] == 0
+    got = s.get_state_array()
+    assert np.max(np.abs(got - want)) < TOL["c128"]
+    r = run(Z, n, gates, "c128", policy="reference")
+    assert np.max(np.abs(r.get_state_array() - got)) < TOL["c128"]

@@ -426,9 +426,8 @@ int Sim::flush_pass()
         return OK;
     ZXA_TRY(flush_group());
     // the first pass after the reset forms |0...0> itself when it covers the
-    // whole reset state and has no fans (it resets the state where its kernel
-    // cannot)
-    const bool fresh = fresh_ && !pass_has_fan_ && (size_t(1) << active_bits()) == fresh_elems_;
+    // whole reset state (it resets the state where its kernel cannot)
+    const bool fresh = fresh_ && (size_t(1) << active_bits()) == fresh_elems_;
     if (!fresh)
         ZXA_TRY(settle());
     fresh_ = false;

@@ -1762,15 +1762,18 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t tiles)
 // (amplitude 0 is 1, every other 0).  Each tile's input is formed in
 // registers instead of read: the reset's sweep and the pass's read merge into
 // the pass's store, and every amplitude still goes through the pass's gates,
-// transposes and window before it is stored.  Stores only, so no prefetch.
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kTab>
+// transposes, fans and window before it is stored.  Stores only, so no
+// prefetch.
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kTab, bool kFan = false>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) void tile_fresh_kernel(
     void *__restrict__ state, const TPass P, const TTab *__restrict__ tabp, const double *__restrict__ ttab,
     const double *__restrict__ tphase)
 {
-    static_assert(!(kTab && kDiag), "one epilogue kind");
+    static_assert(!(kTab && (kDiag || kFan)), "one epilogue kind");
+    static_assert(!(kFan && kC64), "fans need complex128");
     using T = typename UnitT<kC64>::T;
     __shared__ T lds[(kL1 || kL2) ? kUnits : 1];
+    __shared__ dvec2 fl[kFan ? (kUnitBits + 1) * 32 : 1];
     __shared__ WinT<kC64> wt[kTab ? (1 << kWinBits) : 1];
     const int t = threadIdx.x;
     T *p = static_cast<T *>(state);
@@ -1778,6 +1781,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
     Epi<kC64> epi;
     if constexpr (kDiag)
         epi.init(ttab, t);
+    if constexpr (kFan)
+    {
+        for (int e = t; e < (kUnitBits + 1) * 32; e += kThreads)
+            fl[e] = dvec2{tabp->fan[e >> 5].nib[e & 31][0], tabp->fan[e >> 5].nib[e & 31][1]};
+        __syncthreads();
+    }
     uint32_t yt = 0;
     if constexpr (kTab)
     {
@@ -1797,8 +1806,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
 #pragma unroll
         for (int r = 0; r < kRegs; r++)
             v[r] = (at ^ G->roff[r]) == 0 ? one : zero;
-        tile_work<kC64, kL1, kL2, kDiag, false, false, kTab>(v, lds, P, G, tphase, epi, t, toff, tile, base, nullptr,
-                                                            wt, yt);
+        tile_work<kC64, kL1, kL2, kDiag, false, kFan, kTab>(v, lds, P, G, tphase, epi, t, toff, tile, base, fl, wt,
+                                                           yt);
         store_tile<kC64>(v, p, at, G);
         base = next_base(base, P);
     }
@@ -1875,10 +1884,10 @@ int launch_p(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, bool flip,
     return launch_l<kC64, false, false>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
 }
 
-template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kTab>
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kTab, bool kFan>
 int launch_f(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const double *ttab, const double *tphase)
 {
-    auto k = tile_fresh_kernel<kC64, kL1, kL2, kDiag, kTab>;
+    auto k = tile_fresh_kernel<kC64, kL1, kL2, kDiag, kTab, kFan>;
     const int grid = persistent_grid(ctx, k, P0.ntiles);
     TPass P = P0;
     P.gstep = deposit(uint64_t(grid), P.sorted);
@@ -1886,24 +1895,35 @@ int launch_f(Ctx &ctx, void *state, const TPass &P0, const TTab *tab, const doub
     return hip_check(hipGetLastError(), "tile_fresh_kernel launch");
 }
 
-template <bool kC64, bool kDiag, bool kTab>
+template <bool kC64, bool kDiag, bool kTab, bool kFan = false>
 int launch_fl(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, const TTab *tab, const double *ttab,
               const double *tphase)
 {
     if (l1 && l2)
-        return launch_f<kC64, true, true, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
+        return launch_f<kC64, true, true, kDiag, kTab, kFan>(ctx, state, P, tab, ttab, tphase);
     if (l1)
-        return launch_f<kC64, true, false, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
+        return launch_f<kC64, true, false, kDiag, kTab, kFan>(ctx, state, P, tab, ttab, tphase);
     if (l2)
-        return launch_f<kC64, false, true, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
-    return launch_f<kC64, false, false, kDiag, kTab>(ctx, state, P, tab, ttab, tphase);
+        return launch_f<kC64, false, true, kDiag, kTab, kFan>(ctx, state, P, tab, ttab, tphase);
+    return launch_f<kC64, false, false, kDiag, kTab, kFan>(ctx, state, P, tab, ttab, tphase);
 }
 
-// A fresh pass (no per-pair selectors, no fans): win as in launch_p.
+// A fresh pass (no per-pair selectors): win and fan as in launch_p.
 template <bool kC64>
 int launch_fresh(Ctx &ctx, void *state, const TPass &P, bool l1, bool l2, const TTab *tab, const double *ttab,
-                 const double *tphase, bool win)
+                 const double *tphase, bool win, bool fan)
 {
+    if (fan)
+    {
+        if constexpr (!kC64)
+        {
+            if (!win)
+                return ttab ? launch_fl<false, true, false, true>(ctx, state, P, l1, l2, tab, ttab, tphase)
+                            : launch_fl<false, false, false, true>(ctx, state, P, l1, l2, tab, nullptr, nullptr);
+        }
+        set_error("fresh tile pass: fans need a complex128 state and a phase-polynomial window");
+        return EINVAL_;
+    }
     if (win)
         return launch_fl<kC64, false, true>(ctx, state, P, l1, l2, tab, ttab, nullptr);
     if (ttab)
@@ -2546,8 +2566,7 @@ int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vect
 
 // One tile pass over the given targets, then the diagonal window (if any).
 // fresh: the state is the reset |0...0> of 2^n amplitudes, not yet written
-// (see tile_fresh_kernel; passes with per-pair selectors or fans reset it
-// first).
+// (see tile_fresh_kernel; passes with per-pair selectors reset it first).
 int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector<SingleGate> &gs,
                   const DiagGate *diag = nullptr, size_t ndiag = 0, int geometry = -1,
                   const GateFan *fans = nullptr, bool fresh = false)
@@ -2783,7 +2802,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     const TTab *tab = static_cast<const TTab *>(dtab);
-    const bool fresh_k = fresh && !flip && !any_fan;
+    const bool fresh_k = fresh && !flip;
     if (fresh && !fresh_k)
         ZXA_TRY(state_reset(ctx, state, prec, size_t(1) << n));
     // algorithmic bytes: the state read and written, or written only
@@ -2792,9 +2811,9 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     if (fresh_k)
     {
         if (c64)
-            ZXA_TRY(launch_fresh<true>(ctx, state, P, use[1], use[2], tab, ttab, tphase, win));
+            ZXA_TRY(launch_fresh<true>(ctx, state, P, use[1], use[2], tab, ttab, tphase, win, any_fan));
         else
-            ZXA_TRY(launch_fresh<false>(ctx, state, P, use[1], use[2], tab, ttab, tphase, win));
+            ZXA_TRY(launch_fresh<false>(ctx, state, P, use[1], use[2], tab, ttab, tphase, win, any_fan));
     }
     else if (upper)
         ZXA_TRY(launch_u(ctx, c64, state, P, tab, ttab, tphase, flip, win));
