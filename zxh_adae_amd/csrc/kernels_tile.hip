@@ -1758,6 +1758,45 @@ int persistent_grid(Ctx &ctx, K kernel, uint64_t tiles)
     return static_cast<int>(std::max<uint64_t>(1, std::min<uint64_t>(tiles, cap)));
 }
 
+// A fresh tile's work: the input |0...0> has its one nonzero element at
+// tile-local unit 0, held by thread 0 in register 0 in every layout (the
+// layouts' thread and register maps are linear), so it is formed directly in
+// the first layout the pass uses beyond layout 0, and the gates run from
+// there, ending in layout 0: one transpose fewer than tile_work.  The gates
+// of a pass without fans commute, so their order across layouts is free; the
+// window comes last, in layout 0, as in tile_work.
+template <bool kC64, bool kL1, bool kL2, bool kDiag, bool kTab>
+__device__ __forceinline__ void fresh_work(typename UnitT<kC64>::T (&v)[kRegs], typename UnitT<kC64>::T *lds,
+                                           const TPass &P, ZXA_CONST TTab *G, const double *tphase,
+                                           const Epi<kC64> &epi, int t, uint64_t toff, uint64_t tile, uint64_t base,
+                                           const WinT<kC64> *wt, uint32_t yt)
+{
+    if constexpr (kL1)
+    {
+        apply_layout<kC64, false, 1>(v, P, G, base, thr_off<1>(t, P), t, nullptr, nullptr);
+        if constexpr (kL2)
+        {
+            transpose<kC64, 1, 2>(v, lds, t);
+            apply_layout<kC64, false, 2>(v, P, G, base, thr_off<2>(t, P), t, nullptr, nullptr);
+            transpose<kC64, 2, 0>(v, lds, t);
+        }
+        else
+        {
+            transpose<kC64, 1, 0>(v, lds, t);
+        }
+    }
+    else if constexpr (kL2)
+    {
+        apply_layout<kC64, false, 2>(v, P, G, base, thr_off<2>(t, P), t, nullptr, nullptr);
+        transpose<kC64, 2, 0>(v, lds, t);
+    }
+    apply_layout<kC64, false, 0>(v, P, G, base, toff, t, nullptr, nullptr);
+    if constexpr (kDiag)
+        epi.apply(v, (ZXA_CONST double *)(tphase + tile * Epi<kC64>::kStride), t);
+    if constexpr (kTab)
+        win_apply<kC64>(v, wt, G->win, win_y(G->win, base) ^ yt);
+}
+
 // Fresh tile pass: the first pass over a state just reset to |0...0>
 // (amplitude 0 is 1, every other 0).  Each tile's input is formed in
 // registers instead of read: the reset's sweep and the pass's read merge into
@@ -1803,11 +1842,24 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(2))) v
         ZXA_CONST TTab *G = tables(tabp);
         const uint64_t at = base ^ toff;
         T v[kRegs];
+        if constexpr (kFan)
+        {
+            // fans keep the circuit's order of the layouts: formed in layout 0
 #pragma unroll
-        for (int r = 0; r < kRegs; r++)
-            v[r] = (at ^ G->roff[r]) == 0 ? one : zero;
-        tile_work<kC64, kL1, kL2, kDiag, false, kFan, kTab>(v, lds, P, G, tphase, epi, t, toff, tile, base, fl, wt,
-                                                           yt);
+            for (int r = 0; r < kRegs; r++)
+                v[r] = (at ^ G->roff[r]) == 0 ? one : zero;
+            tile_work<kC64, kL1, kL2, kDiag, false, kFan, kTab>(v, lds, P, G, tphase, epi, t, toff, tile, base, fl,
+                                                               wt, yt);
+        }
+        else
+        {
+#pragma unroll
+            for (int r = 0; r < kRegs; r++)
+                v[r] = zero;
+            if (base == 0 && t == 0)
+                v[0] = one;
+            fresh_work<kC64, kL1, kL2, kDiag, kTab>(v, lds, P, G, tphase, epi, t, toff, tile, base, wt, yt);
+        }
         store_tile<kC64>(v, p, at, G);
         base = next_base(base, P);
     }
