@@ -103,7 +103,11 @@ int zxh_rx(zxh_sim *s, size_t q, double theta);
 /* one call per gate code (ZXH_G_*), for bulk loaders */
 int zxh_gate(zxh_sim *s, int code, size_t cq, size_t q, double theta, double lambda, double phi);
 
-int zxh_execute(zxh_sim *s); /* ZXH::execute (zxh.cpp:312-553); synchronous */
+/* ZXH::execute (zxh.cpp:312-553); synchronous.  Starts from the reset state
+ * |0...0>, as the reference does; under the fused policy the first tile pass
+ * forms that state in registers instead of a separate zero fill (no family-6
+ * launch then; the pass's algorithmic bytes are one write of the state). */
+int zxh_execute(zxh_sim *s);
 
 int zxh_measure(zxh_sim *s, size_t shots);                        /* ZXH::measure (zxh.cpp:555-559) */
 int zxh_measured_count(const zxh_sim *s, size_t *out);            /* ZXH::measured_count */
