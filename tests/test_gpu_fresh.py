@@ -94,7 +94,8 @@ def test_first_operations_that_reset(gpu_engine, prec, case):
         gates = [(X, 0, q, 0.0, 0.0, 0.0) for q in range(0, n, 3)] + W.hadamard_all(n) + u3_layer(n, 9)
     elif case == "wide_first":
         # nine targets above the run and none below: the wide tile's pass
-        gates = [(H, 0, q, 0.0, 0.0, 0.0) for q in range(4 + (1 if prec == "c64" else 0), 13 + (1 if prec == "c64" else 0))]
+        shift = 1 if prec == "c64" else 0
+        gates = [(H, 0, q, 0.0, 0.0, 0.0) for q in range(4 + shift, 13 + shift)]
     else:
         gates = [(CX, q, q + 1, 0.0, 0.0, 0.0) for q in range(n - 1)] + W.hadamard_all(n) + u3_layer(n, 10)
     want = oracle_state(n, gates)
