@@ -162,3 +162,37 @@ def test_first_pass_with_fans(gpu_engine, n, seed):
     assert np.max(np.abs(got - want)) < TOL["c128"]
     r = run(Z, n, gates, "c128", policy="reference")
     assert np.max(np.abs(r.get_state_array() - got)) < TOL["c128"]
+
+
+@pytest.mark.parametrize("prec", ["c128", "c64"])
+@pytest.mark.parametrize("kind", ["polynomial", "table"])
+def test_fresh_pass_carries_the_window(gpu_engine, prec, kind):
+    """A first batch that fits one tile pass, followed by a diagonal window
+    that does not commute with it and rides it as its epilogue: a phase
+    polynomial (P/CP on single bits), or, after CX transport moved qubits
+    5..10 onto multi-bit rows, a window over multi-bit selector forms (a
+    table, rank 10).  The one pass is the fresh one: no reset and no diagonal sweep of
+    its own."""
+    Z = gpu_engine
+    n = 16
+    gates = []
+    if kind == "table":
+        # affine CX: rows 5..10 become multi-bit forms; qubits 11..15 keep
+        # single-bit rows, so U3 gates on them stay single-bit pairs
+        gates += [(CX, q, q + 5, 0.0, 0.0, 0.0) for q in range(0, 6)]
+        targets = [11, 12, 13, 14, 15]
+        controls = list(range(5, 10))  # with the targets, forms of rank 10 at most
+    else:
+        targets = [0, 1, 2, 3, 4, 11, 12, 13]
+        controls = list(range(5, 11))
+    rng = np.random.default_rng(13 if kind == "table" else 12)
+    gates += [(U3, 0, q, *rng.uniform(-np.pi, np.pi, 3)) for q in targets]
+    gates += [(CP, c, targets[i % len(targets)], 0.2 + 0.1 * i, 0.0, 0.0) for i, c in enumerate(controls)]
+    gates += [(P, 0, q, 0.05 * (q + 1), 0.0, 0.0) for q in controls]
+    want = oracle_state(n, gates)
+    s = run(Z, n, gates, prec, timing=True)
+    assert s.timing("reset")["launches"] == 0
+    assert s.timing("diag")["launches"] == 0
+    assert np.max(np.abs(s.get_state_array() - want)) < TOL[prec]
+    r = run(Z, n, gates, prec, policy="reference")
+    assert np.max(np.abs(r.get_state_array() - s.get_state_array())) < TOL[prec]
