@@ -710,6 +710,38 @@ def test_u3_batch_on_every_qubit_with_the_wide_tile(wide_knobs, prec, n):
     assert passes >= 2
 
 
+@pytest.mark.parametrize("prec,n,seed,count,above_run", [
+    ("c128", 20, 1, 12, False), ("c128", 22, 2, 17, False), ("c128", 22, 3, 13, True),
+    ("c64", 21, 4, 15, False), ("c64", 23, 5, 11, True),
+    ("c128", 23, 3, 18, True), ("c64", 24, 5, 18, True)])
+def test_u3_batch_on_random_targets_follows_the_plan(wide_knobs, prec, n, seed, count, above_run):
+    """Random target subsets (mixed, or all above the wide tile's run, where
+    the split may be wide only): as many passes as the engine's published
+    split, and the oracle's result."""
+    from zxh_adae_amd import kernels as K
+    rng = np.random.default_rng(seed)
+    c64 = prec == "c64"
+    lo = (5 if c64 else 4) if above_run else 0
+    targets = sorted(int(q) for q in rng.choice(np.arange(lo, n), size=count, replace=False))
+    plan = wide_knobs.tile_layer_plan(n, targets, c64)
+    assert sorted(q for p in plan for q in p) == (targets if plan else [])
+    psi = rand_state(rng, n)
+    descs = [(pack(1 << q, False), 1 << q, rand_u(rng)) for q in targets]
+    want = oracle_u3(psi, n, descs)
+    d = dev(wide_knobs, psi.astype(np.complex64) if c64 else psi, prec)
+    K.reset_timing()
+    K.set_timing(True)
+    try:
+        d.u3_batch(descs)
+        passes = K.timing("u3_tile")["launches"]
+    finally:
+        K.set_timing(False)
+    got = d.download().astype(np.complex128)
+    assert np.max(np.abs(got - want)) < (TOL64 if c64 else TOL128)
+    if plan:
+        assert passes == len(plan)
+
+
 def test_fused_pass_rejects_multibit_and_overfull(gpu_engine):
     from zxh_adae_amd import ZXHError
     rng = np.random.default_rng(1)
