@@ -665,7 +665,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--kernels", choices=["c128", "c64"], default=None,
                     help="only launch each hot kernel --steps times on a 30-qubit state (profiling)")
-    ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128", "zxh_workloads", "sampling"],
+    ap.add_argument("--section", choices=["adjoint_c3", "batch_c4", "c5_c64", "c5_c128", "zxh_workloads", "sampling", "c1"],
                     default=None,
                     help="run one extra alone, after the tile-geometry autotuning (profiling)")
     ap.add_argument("--workloads", default=None,
@@ -691,7 +691,7 @@ def main():
         return
     from zxh_adae_amd import autotune
     if args.section:
-        prec = "c128" if args.section in ("c5_c128", "zxh_workloads", "sampling") else "c64"
+        prec = "c128" if args.section in ("c5_c128", "zxh_workloads", "sampling", "c1") else "c64"
         st = K.DeviceState(NQ, prec)
         st.scale(0.0)
         tuned = autotune.tune_tile(st)
@@ -707,6 +707,8 @@ def main():
             out = sampling_section(Z, W, K)
         elif args.section == "zxh_workloads":
             out = zxh_workloads(Z, breakdown=True, names=args.workloads.split(",") if args.workloads else None)
+        elif args.section == "c1":
+            out = c1_gpu(Z, W, repeats=20)
         else:
             out = c5_circuit(W, K, "c128", 32)
         print(json.dumps({args.section: out, "autotune": tuned}), flush=True)

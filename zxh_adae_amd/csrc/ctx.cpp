@@ -121,15 +121,13 @@ void ParamRing::release()
     if (host_)
         (void)hipHostFree(host_);
     dev_ = host_ = nullptr;
+    open_.clear();
+    end_batch();
 }
 
-int ParamRing::stage(const void *src, size_t bytes, hipStream_t stream, void **dev)
+// The next slot, once its last consumer has finished; open until done().
+int ParamRing::open_slot(int *slot)
 {
-    if (bytes > kSlotBytes)
-    {
-        set_error("parameter block exceeds ring slot size");
-        return EINVAL_;
-    }
     const int s = next_;
     for (int o : open_)
         if (o == s)
@@ -140,13 +138,69 @@ int ParamRing::stage(const void *src, size_t bytes, hipStream_t stream, void **d
     next_ = (next_ + 1) % kSlots;
     if (used_[s])
         ZXA_HIP(hipEventSynchronize(ev_[s]));
+    open_.push_back(s);
+    *slot = s;
+    return OK;
+}
+
+int ParamRing::stage(const void *src, size_t bytes, hipStream_t stream, void **dev)
+{
+    if (bytes > kSlotBytes)
+    {
+        set_error("parameter block exceeds ring slot size");
+        return EINVAL_;
+    }
+    if (batch_ >= 0)
+    {
+        size_t off = (boff_ + 255) & ~size_t(255);
+        if (off + bytes > kSlotBytes)
+        {
+            ZXA_TRY(flush(stream));
+            ZXA_TRY(begin_batch());
+            off = 0;
+        }
+        const size_t at = size_t(batch_) * kSlotBytes + off;
+        std::memcpy(static_cast<char *>(host_) + at, src, bytes);
+        boff_ = off + bytes;
+        *dev = static_cast<char *>(dev_) + at;
+        return OK;
+    }
+    int s = 0;
+    ZXA_TRY(open_slot(&s));
     char *h = static_cast<char *>(host_) + size_t(s) * kSlotBytes;
     char *d = static_cast<char *>(dev_) + size_t(s) * kSlotBytes;
     std::memcpy(h, src, bytes);
     ZXA_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
-    open_.push_back(s);
     *dev = d;
     return OK;
+}
+
+int ParamRing::begin_batch()
+{
+    end_batch();
+    int s = 0;
+    ZXA_TRY(open_slot(&s));
+    batch_ = s;
+    return OK;
+}
+
+int ParamRing::flush(hipStream_t stream)
+{
+    const int s = batch_;
+    const size_t bytes = boff_;
+    end_batch();
+    if (s < 0 || bytes == 0)
+        return OK;
+    const size_t at = size_t(s) * kSlotBytes;
+    ZXA_HIP(hipMemcpyAsync(static_cast<char *>(dev_) + at, static_cast<char *>(host_) + at, bytes,
+                           hipMemcpyHostToDevice, stream));
+    return OK;
+}
+
+void ParamRing::end_batch()
+{
+    batch_ = -1;
+    boff_ = 0;
 }
 
 int ParamRing::done(hipStream_t stream)

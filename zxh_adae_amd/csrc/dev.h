@@ -130,14 +130,26 @@ class ParamRing
     // returns the device address.  Call done() after the consuming launches.
     int stage(const void *src, size_t bytes, hipStream_t stream, void **dev);
     int done(hipStream_t stream);
+    // Batched staging: between begin_batch() and flush(), stage() packs the
+    // blocks into one slot at 256-byte aligned offsets and issues no copy (a
+    // block that no longer fits copies the packed ones and opens the next
+    // slot); flush() copies the packed bytes with one async H2D and ends the
+    // batch.  Launches that read a block come after flush().  end_batch()
+    // drops an unflushed batch (error returns).
+    int begin_batch();
+    int flush(hipStream_t stream);
+    void end_batch();
 
   private:
+    int open_slot(int *slot);
     void *host_ = nullptr;
     void *dev_ = nullptr;
     hipEvent_t ev_[kSlots] = {};
     bool used_[kSlots] = {};
     int next_ = 0;
     std::vector<int> open_; // staged since the last done()
+    int batch_ = -1;        // slot being packed, or -1
+    size_t boff_ = 0;       // bytes packed in it
 };
 
 struct Ctx

@@ -33,6 +33,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <type_traits>
 #include <vector>
 
@@ -2259,10 +2260,52 @@ void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, con
     *nb_out = nb;
 }
 
+// Launches that read tables staged in the ring's open batch, run after its
+// flush().
+using Later = std::vector<std::function<int()>>;
+
+// Runs a launch now, or after the batch's flush when later is given.
+int now_or_later(Later *later, std::function<int()> go)
+{
+    if (!later)
+        return go();
+    later->push_back(std::move(go));
+    return OK;
+}
+
+// A pass's tables in one copy: begin() opens the ring's batch, the launches
+// that read its blocks wait in later until run() flushes it; the destructor
+// drops a batch that an error return left open.
+struct PassBatch
+{
+    Ctx &ctx;
+    Later later;
+    explicit PassBatch(Ctx &c) : ctx(c)
+    {
+    }
+    ~PassBatch()
+    {
+        ctx.ring.end_batch();
+    }
+    int begin()
+    {
+        return ctx.ring.begin_batch();
+    }
+    int run()
+    {
+        ZXA_TRY(ctx.ring.flush(ctx.stream));
+        for (const auto &go : later)
+            ZXA_TRY(go());
+        later.clear();
+        return OK;
+    }
+};
+
 // Stages the diagonal epilogue of a pass: the per-thread table and the
 // per-tile phase factors (tile_phase_kernel into scratch_a).
 int setup_epilogue(Ctx &ctx, int prec, unsigned n, const std::vector<unsigned> &pos, const DiagGate *diag,
-                   size_t ndiag, uint64_t ntiles, const double **ttab, const double **tphase, int nthr = 8)
+                   size_t ndiag, uint64_t ntiles, const double **ttab, const double **tphase, int nthr = 8,
+                   Later *later = nullptr)
 {
     Poly poly;
     window_polynomial(n, diag, ndiag, poly);
@@ -2274,9 +2317,16 @@ int setup_epilogue(Ctx &ctx, int prec, unsigned n, const std::vector<unsigned> &
     ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
     const size_t stride = 2 * (size_t(nthr) + (size_t(1) << nb));
     ZXA_TRY(ctx.scratch_a.ensure(ntiles * stride * sizeof(double)));
-    launch_tile_phase(ctx.stream, static_cast<const double *>(dpt), pt.size(), int(pos.size()), nh, nb, poly.K, ntiles,
-                      static_cast<double *>(ctx.scratch_a.ptr), nthr);
-    ZXA_HIP(hipGetLastError());
+    hipStream_t stream = ctx.stream;
+    const double *tab = static_cast<const double *>(dpt);
+    const size_t npt = pt.size();
+    const int npos = int(pos.size());
+    const double K = poly.K;
+    double *out = static_cast<double *>(ctx.scratch_a.ptr);
+    ZXA_TRY(now_or_later(later, [=]() {
+        launch_tile_phase(stream, tab, npt, npos, nh, nb, K, ntiles, out, nthr);
+        return hip_check(hipGetLastError(), "tile_phase_kernel launch");
+    }));
     *ttab = static_cast<const double *>(dtt);
     *tphase = static_cast<const double *>(ctx.scratch_a.ptr);
     return OK;
@@ -2345,7 +2395,7 @@ __global__ __launch_bounds__(kThreads) void fan_phase_kernel(const double *__res
 // fan gate's thread and register factors in its layout, and the per-tile
 // factors over the outer bits (fan_phase_kernel into scratch_g).
 int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const GateFan *fans, const Plan &pl,
-               TPass &P, TTab &G)
+               TPass &P, TTab &G, Later *later = nullptr)
 {
     std::vector<int> outer;
     for (unsigned b = 0; b < n; b++)
@@ -2398,9 +2448,14 @@ int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const Ga
     const int spread = P.ntiles < 4096 ? 1 : 0;
     const uint64_t threads = spread ? P.ntiles * uint64_t(kUnitBits + 1) : P.ntiles;
     const unsigned grid = unsigned((threads + kThreads - 1) / kThreads);
-    hipLaunchKernelGGL(fan_phase_kernel, dim3(grid), dim3(kThreads), 0, ctx.stream, static_cast<const double *>(dt),
-                       nh, P.ntiles, static_cast<double *>(ctx.scratch_g.ptr), spread);
-    ZXA_HIP(hipGetLastError());
+    hipStream_t stream = ctx.stream;
+    const double *ftab = static_cast<const double *>(dt);
+    const uint64_t ntiles = P.ntiles;
+    double *out = static_cast<double *>(ctx.scratch_g.ptr);
+    ZXA_TRY(now_or_later(later, [=]() {
+        hipLaunchKernelGGL(fan_phase_kernel, dim3(grid), dim3(kThreads), 0, stream, ftab, nh, ntiles, out, spread);
+        return hip_check(hipGetLastError(), "fan_phase_kernel launch");
+    }));
     P.fphase = static_cast<const double *>(ctx.scratch_g.ptr);
     return OK;
 }
@@ -2590,6 +2645,8 @@ int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vect
     }
     const double *ttab = nullptr;
     const double *tphase = nullptr;
+    PassBatch batch(ctx);
+    ZXA_TRY(batch.begin());
     if (ndiag > 0)
     {
         if (!window_is_polynomial(n, diag, ndiag))
@@ -2597,11 +2654,12 @@ int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vect
             set_error("wide tile pass: the window is not a phase polynomial");
             return EINVAL_;
         }
-        ZXA_TRY(setup_epilogue(ctx, prec, n, ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 9));
+        ZXA_TRY(setup_epilogue(ctx, prec, n, ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 9, &batch.later));
     }
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     const Tab13 *tab = static_cast<const Tab13 *>(dtab);
+    ZXA_TRY(batch.run());
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
     int rc;
@@ -2773,11 +2831,13 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     bool win = false;
     std::vector<double> wtab;
     std::memset(&G.win, 0, sizeof(G.win));
+    PassBatch batch(ctx);
+    ZXA_TRY(batch.begin());
     if (ndiag > 0)
     {
         if (!flip && !wide && (any_fan || window_is_polynomial(n, diag, ndiag)))
         {
-            ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase));
+            ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase, 8, &batch.later));
         }
         else
         {
@@ -2799,7 +2859,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         }
     }
     if (any_fan)
-        ZXA_TRY(setup_fans(ctx, n, gs, fans, pl, P, G));
+        ZXA_TRY(setup_fans(ctx, n, gs, fans, pl, P, G, &batch.later));
     // passes whose targets all sit on unit bits 6..11 run on upper_kernel (one
     // tile per workgroup, no LDS) when their highest target unit-address bit
     // is at least tile.upper.<prec>: spread over more than 2^k units, a tile's
@@ -2854,6 +2914,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     const TTab *tab = static_cast<const TTab *>(dtab);
+    ZXA_TRY(batch.run());
     const bool fresh_k = fresh && !flip;
     if (fresh && !fresh_k)
         ZXA_TRY(state_reset(ctx, state, prec, size_t(1) << n));
@@ -3725,11 +3786,14 @@ int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, con
     }
     const double *ttab = nullptr;
     const double *tphase = nullptr;
+    PassBatch batch(ctx);
+    ZXA_TRY(batch.begin());
     if (ndiag > 0)
-        ZXA_TRY(setup_epilogue(ctx, prec, n, pl.ebit, diag, ndiag, P.ntiles, &ttab, &tphase));
+        ZXA_TRY(setup_epilogue(ctx, prec, n, pl.ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 8, &batch.later));
     void *dtab = nullptr, *dgr = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     ZXA_TRY(ctx.ring.stage(tg, sizeof(tg), ctx.stream, &dgr));
+    ZXA_TRY(batch.run());
     // per-wave rows of 64 lane accumulators: at most 8 workgroups per CU
     const size_t maxw = size_t(8) * size_t(ctx.num_cus > 0 ? ctx.num_cus : 256) * kWavesPerTile;
     ZXA_TRY(ctx.scratch_f.ensure(maxw * 64 * sizeof(double)));
