@@ -34,7 +34,6 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
-#include <functional>
 #include <type_traits>
 #include <vector>
 
@@ -1671,11 +1670,10 @@ __device__ __forceinline__ double tp_const(const double *hL, const double *hQ, i
 // factors per tile.  spread = 0: one thread per tile; spread = 1: one thread
 // per (tile, factor), for states of few tiles (the same sums in the same
 // order, so the same factors).
-__global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__restrict__ tab, int nE, int nh, int nb,
-                                                              double K, uint64_t ntiles, double *__restrict__ out,
-                                                              int spread, int nthr)
+__device__ __forceinline__ void tile_phase_work(const double *__restrict__ tab, int nE, int nh, int nb, double K,
+                                                uint64_t ntiles, double *__restrict__ out, int spread, int nthr,
+                                                uint64_t id, double *sh)
 {
-    extern __shared__ double sh[];
     const int nR = 1 << nb;
     const int ntab = nE * nh + nh + nh * nh + nR + nthr + nb;
     for (int i = threadIdx.x; i < ntab; i += blockDim.x)
@@ -1688,7 +1686,6 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
     const double *mp = rs + nR;
     const int per = nthr + nR;
     const int stride = 2 * per;
-    const uint64_t id = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
     if (spread)
     {
         if (id >= ntiles * uint64_t(per))
@@ -1737,15 +1734,30 @@ __global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__re
     }
 }
 
+__global__ __launch_bounds__(kThreads) void tile_phase_kernel(const double *__restrict__ tab, int nE, int nh, int nb,
+                                                              double K, uint64_t ntiles, double *__restrict__ out,
+                                                              int spread, int nthr)
+{
+    extern __shared__ double sh[];
+    tile_phase_work(tab, nE, nh, nb, K, ntiles, out, spread, nthr,
+                    blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x, sh);
+}
+
+unsigned tile_phase_blocks(uint64_t ntiles, int nb, int nthr, int *spread)
+{
+    *spread = ntiles < 4096 ? 1 : 0;
+    const uint64_t threads = *spread ? ntiles * uint64_t(nthr + (1 << nb)) : ntiles;
+    return unsigned((threads + kThreads - 1) / kThreads);
+}
+
 // Launches tile_phase_kernel, spread over (tile, factor) pairs when the state
 // has few tiles (below 4096 one thread per tile leaves most CUs idle and each
 // thread runs 8 + 2^nb sincos in a row).
 void launch_tile_phase(hipStream_t stream, const double *tab, size_t tab_doubles, int nE, int nh, int nb, double K,
                        uint64_t ntiles, double *out, int nthr = 8)
 {
-    const int spread = ntiles < 4096 ? 1 : 0;
-    const uint64_t threads = spread ? ntiles * uint64_t(nthr + (1 << nb)) : ntiles;
-    const unsigned grid = unsigned((threads + kThreads - 1) / kThreads);
+    int spread = 0;
+    const unsigned grid = tile_phase_blocks(ntiles, nb, nthr, &spread);
     hipLaunchKernelGGL(tile_phase_kernel, dim3(grid), dim3(kThreads), tab_doubles * sizeof(double), stream, tab, nE,
                        nh, nb, K, ntiles, out, spread, nthr);
 }
@@ -2275,26 +2287,118 @@ void epilogue_tables(int prec, unsigned n, const std::vector<unsigned> &pos, con
     *nb_out = nb;
 }
 
-// Launches that read tables staged in the ring's open batch, run after its
-// flush().
-using Later = std::vector<std::function<int()>>;
-
-// Runs a launch now, or after the batch's flush when later is given.
-int now_or_later(Later *later, std::function<int()> go)
+// Per-tile factors of the fans: out[tile * kFanStride + 2 s] = exp(i (c_s +
+// sum_i theta_s[outer i] * tile bit i)), one thread per tile; tab holds, per
+// slot s < kUnitBits + 1, c_s then the nh outer coefficients.
+// spread = 1: one thread per (tile, slot) instead of one per tile (few tiles).
+__device__ __forceinline__ void fan_phase_work(const double *__restrict__ tab, int nh, uint64_t ntiles,
+                                               double *__restrict__ out, int spread, uint64_t id)
 {
-    if (!later)
-        return go();
-    later->push_back(std::move(go));
+    const uint64_t tile = spread ? id / uint64_t(kUnitBits + 1) : id;
+    if (tile >= ntiles)
+        return;
+    const int first = spread ? int(id % uint64_t(kUnitBits + 1)) : 0;
+    const int last = spread ? first + 1 : kUnitBits + 1;
+    for (int sl = first; sl < last; sl++)
+    {
+        const double *row = tab + size_t(sl) * (1 + nh);
+        double ph = row[0];
+        for (int i = 0; i < nh; i++)
+            if ((tile >> i) & 1u)
+                ph += row[1 + i];
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        out[tile * kFanStride + 2 * sl] = cs;
+        out[tile * kFanStride + 2 * sl + 1] = sn;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void fan_phase_kernel(const double *__restrict__ tab, int nh, uint64_t ntiles,
+                                                             double *__restrict__ out, int spread)
+{
+    fan_phase_work(tab, nh, ntiles, out, spread, blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x);
+}
+
+unsigned fan_phase_blocks(uint64_t ntiles, int *spread)
+{
+    *spread = ntiles < 4096 ? 1 : 0;
+    const uint64_t threads = *spread ? ntiles * uint64_t(kUnitBits + 1) : ntiles;
+    return unsigned((threads + kThreads - 1) / kThreads);
+}
+
+// Both per-tile factor kernels of a pass with an epilogue and fans in one
+// launch: blocks below tblocks run tile_phase_kernel's work, the others
+// fan_phase_kernel's (a block-uniform branch, so the epilogue's LDS staging
+// and barrier see whole blocks).
+__global__ __launch_bounds__(kThreads) void pass_phase_kernel(const double *__restrict__ ttab, int nE, int nh, int nb,
+                                                              double K, uint64_t ntiles, double *__restrict__ tout,
+                                                              int tspread, int nthr, unsigned tblocks,
+                                                              const double *__restrict__ ftab, int fnh,
+                                                              double *__restrict__ fout, int fspread)
+{
+    extern __shared__ double sh[];
+    if (blockIdx.x < tblocks)
+        tile_phase_work(ttab, nE, nh, nb, K, ntiles, tout, tspread, nthr,
+                        blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x, sh);
+    else
+        fan_phase_work(ftab, fnh, ntiles, fout, fspread,
+                       (blockIdx.x - tblocks) * static_cast<uint64_t>(blockDim.x) + threadIdx.x);
+}
+
+// The per-tile factor launches of a pass, recorded while its tables are
+// packed (setup_epilogue, setup_fans) and made after the batch's flush: one
+// pass_phase_kernel when the pass has both.
+struct PhaseLaunches
+{
+    bool tile = false;
+    const double *ttab = nullptr;
+    size_t tdoubles = 0;
+    int nE = 0, nh = 0, nb = 0, nthr = 8;
+    double K = 0.0;
+    uint64_t tntiles = 0;
+    double *tout = nullptr;
+    bool fan = false;
+    const double *ftab = nullptr;
+    int fnh = 0;
+    uint64_t fntiles = 0;
+    double *fout = nullptr;
+};
+
+int launch_phases(hipStream_t stream, const PhaseLaunches &ph)
+{
+    if (ph.tile && ph.fan && ph.tntiles == ph.fntiles)
+    {
+        int ts = 0, fs = 0;
+        const unsigned tb = tile_phase_blocks(ph.tntiles, ph.nb, ph.nthr, &ts);
+        const unsigned fb = fan_phase_blocks(ph.fntiles, &fs);
+        hipLaunchKernelGGL(pass_phase_kernel, dim3(tb + fb), dim3(kThreads), ph.tdoubles * sizeof(double), stream,
+                           ph.ttab, ph.nE, ph.nh, ph.nb, ph.K, ph.tntiles, ph.tout, ts, ph.nthr, tb, ph.ftab, ph.fnh,
+                           ph.fout, fs);
+        return hip_check(hipGetLastError(), "pass_phase_kernel launch");
+    }
+    if (ph.tile)
+    {
+        launch_tile_phase(stream, ph.ttab, ph.tdoubles, ph.nE, ph.nh, ph.nb, ph.K, ph.tntiles, ph.tout, ph.nthr);
+        ZXA_TRY(hip_check(hipGetLastError(), "tile_phase_kernel launch"));
+    }
+    if (ph.fan)
+    {
+        int fs = 0;
+        const unsigned fb = fan_phase_blocks(ph.fntiles, &fs);
+        hipLaunchKernelGGL(fan_phase_kernel, dim3(fb), dim3(kThreads), 0, stream, ph.ftab, ph.fnh, ph.fntiles, ph.fout,
+                           fs);
+        ZXA_TRY(hip_check(hipGetLastError(), "fan_phase_kernel launch"));
+    }
     return OK;
 }
 
-// A pass's tables in one copy: begin() opens the ring's batch, the launches
-// that read its blocks wait in later until run() flushes it; the destructor
-// drops a batch that an error return left open.
+// A pass's tables in one copy: begin() opens the ring's batch, the per-tile
+// factor launches that read its blocks wait in ph until run() flushes it; the
+// destructor drops a batch that an error return left open.
 struct PassBatch
 {
     Ctx &ctx;
-    Later later;
+    PhaseLaunches ph;
     explicit PassBatch(Ctx &c) : ctx(c)
     {
     }
@@ -2309,18 +2413,18 @@ struct PassBatch
     int run()
     {
         ZXA_TRY(ctx.ring.flush(ctx.stream));
-        for (const auto &go : later)
-            ZXA_TRY(go());
-        later.clear();
-        return OK;
+        const PhaseLaunches ph_now = ph;
+        ph = PhaseLaunches();
+        return launch_phases(ctx.stream, ph_now);
     }
 };
 
 // Stages the diagonal epilogue of a pass: the per-thread table and the
-// per-tile phase factors (tile_phase_kernel into scratch_a).
+// per-tile phase factors (tile_phase_kernel into scratch_a, recorded in ph
+// and launched after the batch's flush).
 int setup_epilogue(Ctx &ctx, int prec, unsigned n, const std::vector<unsigned> &pos, const DiagGate *diag,
-                   size_t ndiag, uint64_t ntiles, const double **ttab, const double **tphase, int nthr = 8,
-                   Later *later = nullptr)
+                   size_t ndiag, uint64_t ntiles, const double **ttab, const double **tphase, int nthr,
+                   PhaseLaunches &ph)
 {
     Poly poly;
     window_polynomial(n, diag, ndiag, poly);
@@ -2332,16 +2436,16 @@ int setup_epilogue(Ctx &ctx, int prec, unsigned n, const std::vector<unsigned> &
     ZXA_TRY(ctx.ring.stage(pt.data(), pt.size() * sizeof(double), ctx.stream, &dpt));
     const size_t stride = 2 * (size_t(nthr) + (size_t(1) << nb));
     ZXA_TRY(ctx.scratch_a.ensure(ntiles * stride * sizeof(double)));
-    hipStream_t stream = ctx.stream;
-    const double *tab = static_cast<const double *>(dpt);
-    const size_t npt = pt.size();
-    const int npos = int(pos.size());
-    const double K = poly.K;
-    double *out = static_cast<double *>(ctx.scratch_a.ptr);
-    ZXA_TRY(now_or_later(later, [=]() {
-        launch_tile_phase(stream, tab, npt, npos, nh, nb, K, ntiles, out, nthr);
-        return hip_check(hipGetLastError(), "tile_phase_kernel launch");
-    }));
+    ph.tile = true;
+    ph.ttab = static_cast<const double *>(dpt);
+    ph.tdoubles = pt.size();
+    ph.nE = int(pos.size());
+    ph.nh = nh;
+    ph.nb = nb;
+    ph.nthr = nthr;
+    ph.K = poly.K;
+    ph.tntiles = ntiles;
+    ph.tout = static_cast<double *>(ctx.scratch_a.ptr);
     *ttab = static_cast<const double *>(dtt);
     *tphase = static_cast<const double *>(ctx.scratch_a.ptr);
     return OK;
@@ -2379,38 +2483,12 @@ void fill_pass(int prec, unsigned n, const Plan &pl, TPass &P, TTab &G, bool use
         use[slot == kIntra ? 0 : layout_of_unit_bit(slot)] = true;
 }
 
-// Per-tile factors of the fans: out[tile * kFanStride + 2 s] = exp(i (c_s +
-// sum_i theta_s[outer i] * tile bit i)), one thread per tile; tab holds, per
-// slot s < kUnitBits + 1, c_s then the nh outer coefficients.
-// spread = 1: one thread per (tile, slot) instead of one per tile (few tiles).
-__global__ __launch_bounds__(kThreads) void fan_phase_kernel(const double *__restrict__ tab, int nh, uint64_t ntiles,
-                                                             double *__restrict__ out, int spread)
-{
-    const uint64_t id = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-    const uint64_t tile = spread ? id / uint64_t(kUnitBits + 1) : id;
-    if (tile >= ntiles)
-        return;
-    const int first = spread ? int(id % uint64_t(kUnitBits + 1)) : 0;
-    const int last = spread ? first + 1 : kUnitBits + 1;
-    for (int sl = first; sl < last; sl++)
-    {
-        const double *row = tab + size_t(sl) * (1 + nh);
-        double ph = row[0];
-        for (int i = 0; i < nh; i++)
-            if ((tile >> i) & 1u)
-                ph += row[1 + i];
-        double sn, cs;
-        sincos(ph, &sn, &cs);
-        out[tile * kFanStride + 2 * sl] = cs;
-        out[tile * kFanStride + 2 * sl + 1] = sn;
-    }
-}
-
 // Fan tables of a complex128 pass (single-bit slots, planned staged): each
 // fan gate's thread and register factors in its layout, and the per-tile
-// factors over the outer bits (fan_phase_kernel into scratch_g).
+// factors over the outer bits (fan_phase_kernel into scratch_g, recorded in
+// ph and launched after the batch's flush).
 int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const GateFan *fans, const Plan &pl,
-               TPass &P, TTab &G, Later *later = nullptr)
+               TPass &P, TTab &G, PhaseLaunches &ph)
 {
     std::vector<int> outer;
     for (unsigned b = 0; b < n; b++)
@@ -2460,17 +2538,11 @@ int setup_fans(Ctx &ctx, unsigned n, const std::vector<SingleGate> &gs, const Ga
     void *dt = nullptr;
     ZXA_TRY(ctx.ring.stage(tab.data(), tab.size() * sizeof(double), ctx.stream, &dt));
     ZXA_TRY(ctx.scratch_g.ensure(P.ntiles * kFanStride * sizeof(double)));
-    const int spread = P.ntiles < 4096 ? 1 : 0;
-    const uint64_t threads = spread ? P.ntiles * uint64_t(kUnitBits + 1) : P.ntiles;
-    const unsigned grid = unsigned((threads + kThreads - 1) / kThreads);
-    hipStream_t stream = ctx.stream;
-    const double *ftab = static_cast<const double *>(dt);
-    const uint64_t ntiles = P.ntiles;
-    double *out = static_cast<double *>(ctx.scratch_g.ptr);
-    ZXA_TRY(now_or_later(later, [=]() {
-        hipLaunchKernelGGL(fan_phase_kernel, dim3(grid), dim3(kThreads), 0, stream, ftab, nh, ntiles, out, spread);
-        return hip_check(hipGetLastError(), "fan_phase_kernel launch");
-    }));
+    ph.fan = true;
+    ph.ftab = static_cast<const double *>(dt);
+    ph.fnh = nh;
+    ph.fntiles = P.ntiles;
+    ph.fout = static_cast<double *>(ctx.scratch_g.ptr);
     P.fphase = static_cast<const double *>(ctx.scratch_g.ptr);
     return OK;
 }
@@ -2669,7 +2741,7 @@ int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vect
             set_error("wide tile pass: the window is not a phase polynomial");
             return EINVAL_;
         }
-        ZXA_TRY(setup_epilogue(ctx, prec, n, ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 9, &batch.later));
+        ZXA_TRY(setup_epilogue(ctx, prec, n, ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 9, batch.ph));
     }
     void *dtab = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
@@ -2852,7 +2924,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
     {
         if (!flip && !wide && (any_fan || window_is_polynomial(n, diag, ndiag)))
         {
-            ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase, 8, &batch.later));
+            ZXA_TRY(setup_epilogue(ctx, prec, n, pos, diag, ndiag, P.ntiles, &ttab, &tphase, 8, batch.ph));
         }
         else
         {
@@ -2874,7 +2946,7 @@ int run_tile_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vector
         }
     }
     if (any_fan)
-        ZXA_TRY(setup_fans(ctx, n, gs, fans, pl, P, G, &batch.later));
+        ZXA_TRY(setup_fans(ctx, n, gs, fans, pl, P, G, batch.ph));
     // passes whose targets all sit on unit bits 6..11 run on upper_kernel (one
     // tile per workgroup, no LDS) when their highest target unit-address bit
     // is at least tile.upper.<prec>: spread over more than 2^k units, a tile's
@@ -3804,7 +3876,7 @@ int adjoint_pass_apply(Ctx &ctx, void *psi, void *lam, int prec, unsigned n, con
     PassBatch batch(ctx);
     ZXA_TRY(batch.begin());
     if (ndiag > 0)
-        ZXA_TRY(setup_epilogue(ctx, prec, n, pl.ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 8, &batch.later));
+        ZXA_TRY(setup_epilogue(ctx, prec, n, pl.ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 8, batch.ph));
     void *dtab = nullptr, *dgr = nullptr;
     ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
     ZXA_TRY(ctx.ring.stage(tg, sizeof(tg), ctx.stream, &dgr));
