@@ -1100,10 +1100,16 @@ __device__ __forceinline__ void w13_store(const typename UnitT<kC64>::T (&v)[kRe
         __builtin_nontemporal_store(v[r], p + (at ^ G->roff[r]));
 }
 
-__device__ __forceinline__ ZXA_CONST Tab13 *tables13(const Tab13 *tabp)
+// The wide pass's table travels as the kernel's first argument, so it sits at
+// offset 0 of the kernel-argument segment (explicit arguments are laid out
+// from offset 0 in declaration order) and reaches the device with the launch
+// itself: no staging copy.  It is read in place through the segment pointer
+// (scalar loads); the empty asm keeps each re-read from being hoisted.
+__device__ __forceinline__ ZXA_CONST Tab13 *tables13()
 {
-    asm volatile("" : "+s"(tabp));
-    return (ZXA_CONST Tab13 *)tabp;
+    uint64_t a = reinterpret_cast<uint64_t>(__builtin_amdgcn_kernarg_segment_ptr());
+    asm volatile("" : "+s"(a));
+    return reinterpret_cast<ZXA_CONST Tab13 *>(a);
 }
 
 __device__ __forceinline__ uint64_t next_base13(uint64_t base, const P13 &P)
@@ -1113,14 +1119,14 @@ __device__ __forceinline__ uint64_t next_base13(uint64_t base, const P13 &P)
 
 template <bool kC64, bool kDiag>
 __global__ __launch_bounds__(kW13Threads) __attribute__((amdgpu_waves_per_eu(2))) void wide13_kernel(
-    void *__restrict__ state, const P13 P, const Tab13 *__restrict__ tabp, const double *__restrict__ ttab,
+    const Tab13 tab, void *__restrict__ state, const P13 P, const double *__restrict__ ttab,
     const double *__restrict__ tphase)
 {
     using T = typename UnitT<kC64>::T;
     __shared__ T lds[kW13Units];
     const int t = threadIdx.x;
     T *p = static_cast<T *>(state);
-    ZXA_CONST Tab13 *G = tables13(tabp);
+    ZXA_CONST Tab13 *G = tables13();
     uint64_t toff = 0;
 #pragma unroll
     for (int j = 0; j < 9; j++)
@@ -1140,7 +1146,7 @@ __global__ __launch_bounds__(kW13Threads) __attribute__((amdgpu_waves_per_eu(2))
     w13_work<kC64, kDiag>(a, lds, G, tphase, epi, t, t0);
     for (;;)
     {
-        G = tables13(tabp);
+        G = tables13();
         w13_store<kC64>(a, p, b0 ^ toff, G);
         if (!m1)
             break;
@@ -1150,7 +1156,7 @@ __global__ __launch_bounds__(kW13Threads) __attribute__((amdgpu_waves_per_eu(2))
         w13_load<kC64>(a, p, (m2 ? b2 : b1) ^ toff, G);
         w13_work<kC64, kDiag>(b, lds, G, tphase, epi, t, t1);
 
-        G = tables13(tabp);
+        G = tables13();
         w13_store<kC64>(b, p, b1 ^ toff, G);
         if (!m2)
             break;
@@ -2703,7 +2709,7 @@ bool wide13_plan(int prec, unsigned n, const std::vector<SingleGate> &gs, P13 &P
 }
 
 template <bool kC64, bool kDiag>
-int launch_wide13(Ctx &ctx, void *state, const P13 &P0, const Tab13 *tab, const double *ttab, const double *tphase)
+int launch_wide13(Ctx &ctx, void *state, const P13 &P0, const Tab13 &tab, const double *ttab, const double *tphase)
 {
     auto k = wide13_kernel<kC64, kDiag>;
     int per_cu = 0;
@@ -2713,7 +2719,7 @@ int launch_wide13(Ctx &ctx, void *state, const P13 &P0, const Tab13 *tab, const 
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(P0.ntiles, cap));
     P13 P = P0;
     P.gstep = deposit13(grid, P.sorted);
-    hipLaunchKernelGGL(k, dim3(unsigned(grid)), dim3(kW13Threads), 0, ctx.stream, state, P, tab, ttab, tphase);
+    hipLaunchKernelGGL(k, dim3(unsigned(grid)), dim3(kW13Threads), 0, ctx.stream, tab, state, P, ttab, tphase);
     return hip_check(hipGetLastError(), "wide13_kernel launch");
 }
 
@@ -2732,8 +2738,8 @@ int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vect
     }
     const double *ttab = nullptr;
     const double *tphase = nullptr;
+    // the gate table goes with the launch; only an epilogue stages tables
     PassBatch batch(ctx);
-    ZXA_TRY(batch.begin());
     if (ndiag > 0)
     {
         if (!window_is_polynomial(n, diag, ndiag))
@@ -2741,12 +2747,11 @@ int run_wide13_pass(Ctx &ctx, void *state, int prec, unsigned n, const std::vect
             set_error("wide tile pass: the window is not a phase polynomial");
             return EINVAL_;
         }
+        ZXA_TRY(batch.begin());
         ZXA_TRY(setup_epilogue(ctx, prec, n, ebit, diag, ndiag, P.ntiles, &ttab, &tphase, 9, batch.ph));
+        ZXA_TRY(batch.run());
     }
-    void *dtab = nullptr;
-    ZXA_TRY(ctx.ring.stage(&G, sizeof(G), ctx.stream, &dtab));
-    const Tab13 *tab = static_cast<const Tab13 *>(dtab);
-    ZXA_TRY(batch.run());
+    const Tab13 &tab = G;
     const uint64_t bytes = 2ull * (uint64_t(1) << n) * elem_bytes(prec);
     ZXA_TRY(ctx.tic());
     int rc;
