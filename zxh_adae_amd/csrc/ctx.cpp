@@ -70,7 +70,10 @@ int ParamRing::init()
 {
     if (host_)
         return OK;
-    ZXA_HIP(hipHostMalloc(&host_, size_t(kSlots) * kSlotBytes, hipHostMallocDefault));
+    // mapped and coherent: the copy kernel reads the slots where the host
+    // wrote them, uncached, so a reused slot is never read stale
+    ZXA_HIP(hipHostMalloc(&host_, size_t(kSlots) * kSlotBytes, hipHostMallocMapped | hipHostMallocCoherent));
+    ZXA_HIP(hipHostGetDevicePointer(&hdev_, host_, 0));
     ZXA_HIP(hipMalloc(&dev_, size_t(kSlots) * kSlotBytes));
     for (int i = 0; i < kSlots; i++)
         ZXA_HIP(hipEventCreateWithFlags(&ev_[i], hipEventDisableTiming));
@@ -120,7 +123,7 @@ void ParamRing::release()
         (void)hipFree(dev_);
     if (host_)
         (void)hipHostFree(host_);
-    dev_ = host_ = nullptr;
+    dev_ = host_ = hdev_ = nullptr;
     open_.clear();
     end_batch();
 }
@@ -170,7 +173,7 @@ int ParamRing::stage(const void *src, size_t bytes, hipStream_t stream, void **d
     char *h = static_cast<char *>(host_) + size_t(s) * kSlotBytes;
     char *d = static_cast<char *>(dev_) + size_t(s) * kSlotBytes;
     std::memcpy(h, src, bytes);
-    ZXA_HIP(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, stream));
+    ZXA_TRY(ring_copy(stream, d, static_cast<char *>(hdev_) + size_t(s) * kSlotBytes, bytes));
     *dev = d;
     return OK;
 }
@@ -192,9 +195,7 @@ int ParamRing::flush(hipStream_t stream)
     if (s < 0 || bytes == 0)
         return OK;
     const size_t at = size_t(s) * kSlotBytes;
-    ZXA_HIP(hipMemcpyAsync(static_cast<char *>(dev_) + at, static_cast<char *>(host_) + at, bytes,
-                           hipMemcpyHostToDevice, stream));
-    return OK;
+    return ring_copy(stream, static_cast<char *>(dev_) + at, static_cast<char *>(hdev_) + at, bytes);
 }
 
 void ParamRing::end_batch()

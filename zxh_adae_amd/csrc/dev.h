@@ -143,6 +143,7 @@ class ParamRing
   private:
     int open_slot(int *slot);
     void *host_ = nullptr;
+    void *hdev_ = nullptr; // host_ as the device addresses it (mapped, coherent)
     void *dev_ = nullptr;
     hipEvent_t ev_[kSlots] = {};
     bool used_[kSlots] = {};
@@ -151,6 +152,11 @@ class ParamRing
     int batch_ = -1;        // slot being packed, or -1
     size_t boff_ = 0;       // bytes packed in it
 };
+
+// Copies a staged block from the ring's mapped host slot (src: its device
+// address) to its device slot with a copy kernel on stream; bytes is rounded
+// up to 16, which the slots' 256-byte offsets and size keep inside the slot.
+int ring_copy(hipStream_t stream, void *dst, const void *src, size_t bytes);
 
 struct Ctx
 {

@@ -153,6 +153,18 @@ int copy_apply(Ctx &ctx, void *dst, const void *src, size_t bytes)
     return ctx.toc(F_COPY, 2ull * bytes);
 }
 
+int ring_copy(hipStream_t stream, void *dst, const void *src, size_t bytes)
+{
+    if (bytes == 0)
+        return OK;
+    const uint64_t n16 = (bytes + 15) / 16;
+    const uint64_t per_block = uint64_t(kThreads) * kUnroll;
+    const unsigned grid = unsigned((n16 + per_block - 1) / per_block);
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kThreads), 0, stream, static_cast<const float4 *>(src),
+                       static_cast<float4 *>(dst), n16);
+    return hip_check(hipGetLastError(), "ring copy launch");
+}
+
 int scale_apply(Ctx &ctx, void *state, int prec, unsigned n, double re, double im)
 {
     const uint64_t bytes = (uint64_t(1) << n) * elem_bytes(prec);
