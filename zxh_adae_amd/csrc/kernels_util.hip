@@ -39,6 +39,17 @@ __global__ __launch_bounds__(kThreads) void copy_kernel(const float4 *__restrict
     }
 }
 
+// One 16-byte load per thread across as many blocks as the block needs: the
+// source is host memory read over the bus, so every load is issued at once
+// rather than a few per thread in sequence.
+__global__ __launch_bounds__(kThreads) void ring_copy_kernel(const float4 *__restrict__ src,
+                                                             float4 *__restrict__ dst, uint64_t n16)
+{
+    const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+    if (i < n16)
+        dst[i] = src[i];
+}
+
 template <bool kC64>
 __global__ __launch_bounds__(kThreads) void scale_kernel(void *__restrict__ state, uint64_t n16, double re, double im)
 {
@@ -158,9 +169,8 @@ int ring_copy(hipStream_t stream, void *dst, const void *src, size_t bytes)
     if (bytes == 0)
         return OK;
     const uint64_t n16 = (bytes + 15) / 16;
-    const uint64_t per_block = uint64_t(kThreads) * kUnroll;
-    const unsigned grid = unsigned((n16 + per_block - 1) / per_block);
-    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kThreads), 0, stream, static_cast<const float4 *>(src),
+    const unsigned grid = unsigned((n16 + kThreads - 1) / kThreads);
+    hipLaunchKernelGGL(ring_copy_kernel, dim3(grid), dim3(kThreads), 0, stream, static_cast<const float4 *>(src),
                        static_cast<float4 *>(dst), n16);
     return hip_check(hipGetLastError(), "ring copy launch");
 }
