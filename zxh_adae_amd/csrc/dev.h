@@ -117,7 +117,7 @@ struct FamilyTiming
     uint64_t bytes = 0; // algorithmic HBM bytes of the timed launches
 };
 
-// Ring of parameter slots: pinned host staging + device copy + event.  A slot
+// Ring of parameter slots: mapped host staging + device copy + event.  A slot
 // is reused only after the event recorded behind its last consumer fired.
 class ParamRing
 {
@@ -126,14 +126,14 @@ class ParamRing
     static constexpr size_t kSlotBytes = 256 * 1024;
     int init();
     void release();
-    // Copies bytes from src into the next slot (async H2D on stream) and
+    // Copies bytes from src into the next slot (ring_copy on stream) and
     // returns the device address.  Call done() after the consuming launches.
     int stage(const void *src, size_t bytes, hipStream_t stream, void **dev);
     int done(hipStream_t stream);
     // Batched staging: between begin_batch() and flush(), stage() packs the
     // blocks into one slot at 256-byte aligned offsets and issues no copy (a
     // block that no longer fits copies the packed ones and opens the next
-    // slot); flush() copies the packed bytes with one async H2D and ends the
+    // slot); flush() copies the packed bytes with one ring_copy and ends the
     // batch.  Launches that read a block come after flush().  end_batch()
     // drops an unflushed batch (error returns).
     int begin_batch();
